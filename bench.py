@@ -1,0 +1,177 @@
+"""Benchmark: batched angle-DDM trajectories/s on one MI355X.
+
+Workload (one "step" = one pass of the hot path over one batch): 4096 angle-DDM
+parameter sets drawn uniformly from v in [-3,3], a in [0.3,3], z in [0.1,0.9],
+t in [0.001,2], theta in [-0.1,1.3]; per-set n_samples = floor(10^U(3,5));
+dt = 0.001, max_t = 20, float32 rts, int8 choices; all sets in ONE persistent
+kernel launch replayed from a HIP graph, launch shape chosen by the autotuner.
+
+Prints one JSON line (see the repository documentation for the fields).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "ssm-simulators_amd"))
+sys.path.insert(0, str(REPO))
+
+# VALU issue slots per EM step of the angle-DDM walk with Philox + Box–Muller,
+# counted from the algorithm (full-rate op = 1 slot, quarter-rate op = 4):
+#   Philox4x32-10 per 4 steps: 10 rounds x (2 mul32x32->64 @4 + 2 xor3 @1) = 100
+#   Box–Muller per 4 steps: 2 x (2 cvt + fma + mul + mul + log@4 + sqrt@4 + sin@4 + cos@4 + 2 mul) = 2 x 24 = 48
+#   per step: t_s (cvt, mul), bound (add), 3 compares + 2 ands, increment (mul, add, add),
+#             t += dt, ix += 1, 3 selects = 16
+# -> (100 + 48) / 4 + 16 = 53 slots per step.
+SLOTS_PER_STEP_ANGLE = 53.0
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 scalar-FP32 VALU lane-slots/s
+# (the published 157.3 TFLOP/s FP32 counts FMA = 2 and packed dual-FP32 issue).
+PEAK_VALU_SLOTS = 256 * 4 * 16 * 2.4e9
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline_leg(theta, ns, target_seconds: float = 15.0):
+    """Time the C++/OpenMP restatement (MT19937 + Ziggurat) on a bounded sample of the same workload."""
+    from oracle import cpu_baseline
+    from oracle import em_numpy as E
+
+    cpu_baseline.load()
+    cores = cpu_baseline.n_cores()
+    n_sets = theta.shape[0]
+    v, a, z, t, th = (theta[:, i] for i in range(5))
+    dt = 0.001
+
+    def run(idx, scale):
+        nsub = np.maximum(1, np.floor(ns[idx] * scale)).astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(nsub)]).astype(np.int64)
+        table = E.boundary_table("angle", {"a": a[idx], "theta": th[idx]}, idx.size, 20.0, dt)
+        dl = np.minimum(np.float32(20.0), (np.float32(999.0) - t[idx]).astype(np.float32))
+        t0 = time.perf_counter()
+        _, _, steps = cpu_baseline.em_simulate(0, v[idx], z[idx], np.full(idx.size, E.sqrt_dt32(dt), np.float32),
+                                               t[idx], np.full(idx.size, 999.0, np.float32), dl, table, offs, dt=dt,
+                                               seed=1, n_threads=cores)
+        return time.perf_counter() - t0, int(offs[-1]), steps
+
+    idx = np.arange(0, n_sets, 4)  # every 4th set keeps the parameter mix
+    sec, n, _ = run(idx, 0.01)
+    rate = n / max(sec, 1e-9)
+    scale = min(1.0, 0.01 * target_seconds / max(sec, 1e-6))
+    sec, n, steps = run(idx, scale)
+    return {
+        "value": n / sec,
+        "unit": "trajectories/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{idx.size} of {n_sets} sets (every 4th), n_samples scaled by {scale:.4f}: {n} trajectories, "
+                  f"{steps} EM steps in {sec:.2f} s ({steps / sec:.3e} steps/s)",
+        "calibration_rate": rate,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-sets", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the workload shards by parameter set with no exchange, "
+              "and no multi-GPU figure is produced in this build", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from ssmhip import batched
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    theta, ns = batched.sample_config2(args.n_sets, seed=0)
+    log(f"config2: {args.n_sets} sets, {int(ns.sum())} trajectories")
+    sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch="auto")
+    lp = sim.launch_params
+    log(f"autotuned launch: block={lp.block_size} blocks/CU={lp.blocks_per_cu} refill_min={lp.refill_min}")
+    sim.capture()
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        sim.replay()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        sim.replay()
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = start.elapsed_time(end)
+    ms = ms_total / args.steps
+    steps_per_launch = sim.steps()
+    traj = sim.n_total
+    value = traj / (ms / 1e3)
+    achieved_slots = steps_per_launch * SLOTS_PER_STEP_ANGLE / (ms / 1e3)
+    log(f"{ms:.3f} ms/launch, {value:.4e} traj/s, {steps_per_launch / (ms / 1e3):.4e} EM steps/s (wall {wall:.2f} s)")
+
+    # sanity: choices are +-1 and rts finite or the omission sentinel
+    ch = sim.choices.cpu().numpy()
+    rts = sim.rts.cpu().numpy()
+    assert np.isin(ch, (-1, 0, 1)).all() and np.isfinite(rts).all()
+
+    out = {
+        "metric": "angle-DDM trajectories/s",
+        "value": value,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched angle-DDM, 4096 sets, theta~U(angle bounds), n_i=floor(10^U(3,5)), dt=1e-3, "
+                        "max_t=20, one persistent hipGraph launch",
+            "n_sets": int(args.n_sets),
+            "trajectories": int(traj),
+            "em_steps_per_launch": int(steps_per_launch),
+            "em_steps_per_s": steps_per_launch / (ms / 1e3),
+            "launch": {"block": lp.block_size, "blocks_per_cu": lp.blocks_per_cu, "refill_min": lp.refill_min},
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_slots / 1e12,
+            "peak": PEAK_VALU_SLOTS / 1e12,
+            "unit": "Tslot/s",
+            "frac": achieved_slots / PEAK_VALU_SLOTS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline_leg(theta, ns, args.cpu_seconds)
+        cb.pop("calibration_rate", None)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,151 @@
+/*
+ * ssmhip.h — C ABI of the MI355X (gfx950) simulators, KDE labelling and
+ * likelihood-surrogate kernels.
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is a device pointer (hipMalloc or a
+ *     torch.empty(device="cuda") tensor); the library never takes ownership.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - Every entry point returns 0 on success or a negative SSM_E* status and
+ *     never throws; ssm_last_error() returns the calling thread's message.
+ *   - Calls are asynchronous on `stream` unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the ssm-simulators tree):
+ *   src/cssm/ddm_models.pyx:589-878   ddm_flexbound  -> ssm_em_simulate(model=DDM)
+ *   src/cssm/race_models.pyx:141-506  race_model     -> ssm_em_simulate(model=RACE)
+ *   src/cssm/race_models.pyx:509-920  lca            -> ssm_em_simulate(model=LCA)
+ *   src/cssm/lba_models.pyx:36-250    lba_vanilla / lba_angle -> ssm_lba_simulate
+ *   ssms/support_utils/kde_class.py:89-186,438-553  LogKDE fit -> ssm_kde_fit
+ *   ssms/support_utils/kde_class.py:188-329         kde_eval   -> ssm_kde_eval
+ *   ssms/support_utils/kde_class.py:331-434         kde_sample -> ssm_kde_sample
+ */
+#ifndef SSMHIP_H
+#define SSMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define SSM_OK 0
+#define SSM_EINVAL (-1)   /* invalid argument (maps to Python ValueError)   */
+#define SSM_EHIP (-2)     /* HIP runtime error (maps to RuntimeError)       */
+#define SSM_ENOMEM (-3)   /* allocation failure                              */
+#define SSM_EUNSUPPORTED (-4)
+
+/* ---- model / boundary selectors ------------------------------------------ */
+#define SSM_MODEL_DDM 0   /* ddm_flexbound: bounds +-b(t), start -b0 + 2 z b0 */
+#define SSM_MODEL_RACE 1  /* race_model: N accumulators clamped at 0           */
+#define SSM_MODEL_LCA 2   /* lca: race + leak g and lateral inhibition b       */
+
+#define SSM_BOUNDARY_CONSTANT 0 /* b(t) = a                                  */
+#define SSM_BOUNDARY_ANGLE 1    /* b(t) = a + t_s * slope, slope = -sin/cos  */
+#define SSM_BOUNDARY_WEIBULL 2  /* b(t) = a * exp(-(t_s/beta)^alpha)         */
+#define SSM_BOUNDARY_TABLE 3    /* b(t) = table[set][ix] (any Python boundary)*/
+
+#define SSM_MAX_PARTICLES 16
+
+/* Per-set parameter record: float32[SSM_SET_STRIDE] per parameter set. */
+#define SSM_SET_STRIDE 64
+#define SSM_P_T0 0           /* non-decision time t                          */
+#define SSM_P_DEADLINE 1     /* deadline (999 when the model has none)       */
+#define SSM_P_DEADLINE_TMP 2 /* min(max_t, deadline - t), float32            */
+#define SSM_P_A 3            /* boundary a                                    */
+#define SSM_P_B1 4           /* angle: slope (-sin th/cos th); weibull: alpha */
+#define SSM_P_B2 5           /* weibull: beta                                 */
+#define SSM_P_Z 6            /* DDM relative start z                          */
+#define SSM_P_V 7            /* DDM drift v                                   */
+#define SSM_P_SQRT_ST 8      /* DDM float32(sqrt(dt)) * s                     */
+#define SSM_P_G 9            /* LCA leak g                                    */
+#define SSM_P_BINH 10        /* LCA inhibition b                              */
+#define SSM_P_VJ 16          /* race/LCA v_j, j < 16                          */
+#define SSM_P_ZJ 32          /* race/LCA z_j                                  */
+#define SSM_P_SSTJ 48        /* race/LCA float32(sqrt(dt)) * s_j              */
+
+typedef struct ssm_em_problem {
+  int32_t model;         /* SSM_MODEL_*                                       */
+  int32_t boundary;      /* SSM_BOUNDARY_*                                    */
+  int32_t n_particles;   /* 1 for DDM; 1..16 for RACE/LCA                     */
+  int32_t n_sets;        /* number of parameter sets (trials)                 */
+  int64_t n_total;       /* offsets[n_sets]: total trajectories               */
+  int32_t n_steps;       /* boundary table length int(max_t/dt + 1)          */
+  float dt;              /* delta_t as float32                                */
+  int32_t smooth_unif;   /* 1: add the reference's uniform RT jitter          */
+  int32_t out_layout;    /* 0: out[offsets[k] + n]; 1: out[n * n_sets + k]    */
+  int32_t choice_bytes;  /* 1: int8 choices; 4: int32 (np.intc) choices       */
+  int32_t pad0;
+  uint64_t seed;         /* Philox key                                        */
+  int64_t replay_stride; /* 0: native Philox; >0: replay buffer row length    */
+} ssm_em_problem;
+
+typedef struct ssm_launch_config {
+  int32_t block_size;    /* threads per block (multiple of 64), 0 = default   */
+  int32_t blocks_per_cu; /* resident blocks per CU in the persistent grid     */
+  int32_t refill_min;    /* idle lanes (1..64) before a wavefront refills     */
+  int32_t pad0;
+} ssm_launch_config;
+
+/* ---- library ---------------------------------------------------------------*/
+int32_t ssm_abi_version(void);
+const char* ssm_last_error(void);
+/* Number of CUs of the current device (0 and an error when no GPU).          */
+int32_t ssm_device_cu_count(void);
+
+/* ---- Euler–Maruyama first-passage simulation ------------------------------
+ * Replaces cssm.ddm_flexbound (src/cssm/ddm_models.pyx:589-878, sequential
+ * loop :696-744), cssm.race_model (src/cssm/race_models.pyx:356-506) and
+ * cssm.lca (src/cssm/race_models.pyx:757-920), batched over ragged sets.
+ *   set_params_dev   float32[n_sets][SSM_SET_STRIDE]
+ *   offsets_dev      int64[n_sets + 1], offsets[0] = 0, non-decreasing
+ *   table_dev        float32[n_sets][n_steps] when boundary == TABLE, else NULL
+ *   replay_normals_dev float32[n_total][replay_stride] (replay mode) or NULL;
+ *                    draw d of trajectory i reads [i][d % replay_stride],
+ *                    d = step * n_particles + particle
+ *   replay_uniforms_dev float32[n_total] (replay + smooth_unif) or NULL
+ *   rts_dev          float32[n_total]; choices_dev int8/int32[n_total]
+ *   workspace_dev    uint64[SSM_EM_WORKSPACE_WORDS] scratch owned by the caller
+ *                    ([0] work-queue head, [1] executed EM steps)
+ * `launch` may be NULL (defaults). Two launches on `stream`: a workspace reset
+ * kernel and the persistent simulation kernel.                             */
+#define SSM_EM_WORKSPACE_WORDS 4
+int32_t ssm_em_simulate(const ssm_em_problem* prob, const ssm_launch_config* launch,
+                        const float* set_params_dev, const int64_t* offsets_dev,
+                        const float* table_dev, const float* replay_normals_dev,
+                        const float* replay_uniforms_dev, float* rts_dev,
+                        void* choices_dev, uint64_t* workspace_dev, void* stream);
+
+/* Capture the two launches of ssm_em_simulate into a HIP graph executable.
+ * The pointers are baked into the graph; replay with ssm_graph_launch.      */
+int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config* launch,
+                            const float* set_params_dev, const int64_t* offsets_dev,
+                            const float* table_dev, const float* replay_normals_dev,
+                            const float* replay_uniforms_dev, float* rts_dev,
+                            void* choices_dev, uint64_t* workspace_dev, void** graph_out);
+int32_t ssm_graph_launch(void* graph, void* stream);
+int32_t ssm_graph_destroy(void* graph);
+
+/* ---- LBA (closed form, no time stepping) ----------------------------------
+ * Replaces cssm.lba_vanilla / lba_angle (src/cssm/lba_models.pyx:36-250).
+ *   params_dev float32[n_sets][SSM_SET_STRIDE]: [SSM_P_A]=b threshold a,
+ *     [SSM_P_Z]=start-point range z, [SSM_P_T0]=t, [SSM_P_DEADLINE]=deadline,
+ *     [SSM_P_B1]=tan(theta) (angle) or 0, [SSM_P_VJ+j]=v_j, [SSM_P_SSTJ+j]=sd_j
+ *   replay_dev NULL (Philox) or float32[n_total][2*nact]: per sample nact
+ *     uniforms in [0,1) followed by nact standard normals.                  */
+int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t out_layout,
+                         int32_t choice_bytes, uint64_t seed, const float* params_dev,
+                         const int64_t* offsets_dev, const float* replay_dev, float* rts_dev,
+                         void* choices_dev, void* stream);
+
+/* ---- RNG diagnostics --------------------------------------------------------
+ * Philox4x32-10 (Salmon et al., SC'11) blocks for counters (c0 + i, c1, c2, c3),
+ * key = seed, i < n: raw_dev uint32[n][4] (nullable) and the Box–Muller
+ * normals the simulators use, normals_dev float32[n][4] (nullable).          */
+int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                        int64_t n, uint32_t* raw_dev, float* normals_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSMHIP_H */

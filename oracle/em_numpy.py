@@ -1,0 +1,229 @@
+"""Float32 NumPy transcription of the reference's sequential EM loops (test oracle).
+
+Follows, operation by operation in float32:
+  ddm_flexbound  src/cssm/ddm_models.pyx:696-744  (start :719, walk :721-733,
+                 smooth_unif :735-739, rt :741, sign :742, deadline :744)
+  race_model     src/cssm/race_models.pyx:417-464
+  lca            src/cssm/race_models.pyx:825-879
+  lba_vanilla / lba_angle  src/cssm/lba_models.pyx:101-115, 212-224
+with helpers compute_smooth_unif / enforce_deadline / compute_deadline_tmp
+(src/cssm/_utils.pyx:668-718) and boundary functions
+(ssms/basic_simulators/boundary_functions.py:14-122).
+
+Vectorised across trajectories (one array lane per trajectory), looping over
+time steps. Normal increments come from either
+  * replay: float32[n_total, stride]; draw d of trajectory i is
+    normals[i, d % stride] with d = step * n_particles + particle, or
+  * a NumPy Generator: fresh float32 standard normals every step.
+Trajectory i belongs to set k = i // n_samples (set-major layout).
+
+Pinning: the DDM walk is pinned by the closed-form first-passage
+probability / mean decision time (tests/test_oracle_em.py, Siegmund-corrected)
+and the noise-free crossing step; race and LCA by the one-accumulator
+first-passage closed form, the LCA(g=b=0) == race identity and the
+noise-free crossing step; LBA by its closed-form choice probability.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def time_grid(max_t: float, delta_t: float) -> np.ndarray:
+    """t_s = np.arange(0, max_t + dt, dt).astype(float32), _utils.pyx:636-640."""
+    dt = F32(delta_t)
+    stop = float(F32(F32(max_t) + dt))
+    return np.arange(0, stop, float(dt)).astype(F32)
+
+
+def n_steps(max_t: float, delta_t: float) -> int:
+    """Length of t_s: the boundary array the sequential loop indexes (ddm_models.pyx:668)."""
+    return int(time_grid(max_t, delta_t).shape[0])
+
+
+def b_constant(t, a):
+    return np.full_like(t, a)
+
+
+def b_angle(t, a, theta):
+    return a + np.multiply(t, (-np.sin(theta) / np.cos(theta)))
+
+
+def b_weibull(t, a, alpha, beta):
+    return a * np.exp(-np.power(np.divide(t, beta), alpha))
+
+
+BOUNDARIES = {"constant": (b_constant, ["a"]), "angle": (b_angle, ["a", "theta"]),
+              "weibull_cdf": (b_weibull, ["a", "alpha", "beta"])}
+
+
+def boundary_table(name: str, params: dict, n_trials: int, max_t: float, delta_t: float) -> np.ndarray:
+    """Per-trial float32 boundary over t_s (compute_boundary, _utils.pyx:655-666)."""
+    fun, names = BOUNDARIES[name]
+    t_s = time_grid(max_t, delta_t)
+    out = np.empty((n_trials, t_s.shape[0]), dtype=F32)
+    for k in range(n_trials):
+        kw = {p: F32(np.asarray(params[p], F32).reshape(n_trials, -1)[k, 0]) for p in names}
+        out[k] = np.asarray(fun(t_s, **kw)).astype(F32)
+    return out
+
+
+def sqrt_dt32(delta_t: float) -> np.float32:
+    return F32(np.sqrt(np.float64(F32(delta_t))))
+
+
+def smooth_unif32(t: np.ndarray, dl_tmp: np.ndarray, dt: np.float32, u: np.ndarray) -> np.ndarray:
+    """compute_smooth_unif (_utils.pyx:668-695), float32."""
+    half = (u * F32(0.5)).astype(F32) * dt
+    mid = ((F32(0.5) - u).astype(F32) * dt).astype(F32)
+    return np.where(t == F32(0.0), half, np.where(t < dl_tmp, mid, F32(0.0))).astype(F32)
+
+
+class _Normals:
+    def __init__(self, normals=None, rng=None, n_lanes=0):
+        self.normals = None if normals is None else np.asarray(normals, F32)
+        self.rng = rng
+        if self.normals is not None:
+            self.stride = self.normals.shape[1]
+
+    def draw(self, lanes: np.ndarray, d: np.ndarray) -> np.ndarray:
+        if self.normals is not None:
+            return self.normals[lanes, d % self.stride]
+        return self.rng.standard_normal(lanes.shape[0], dtype=F32)
+
+
+def _finish(rts, t_el, t0, dl_tmp, deadline, dt, smooth, uniforms):
+    su = smooth_unif32(t_el, dl_tmp, dt, uniforms) if smooth else np.zeros_like(t_el)
+    rt = ((t_el + t0).astype(F32) + su).astype(F32)
+    omit = (rt >= deadline) | (deadline <= F32(0.0))
+    rt[omit] = F32(-999.0)
+    return rt
+
+
+def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
+                  bparams=None, smooth_unif=False, normals=None, uniforms=None, rng=None, table=None):
+    """Returns (rts float32[n_total], choices int32[n_total], steps int64[n_total]), set-major."""
+    v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
+    K = v.shape[0]
+    N = int(n_samples)
+    dt = F32(delta_t)
+    bparams = dict(bparams or {"a": a})
+    B = table if table is not None else boundary_table(boundary, bparams, K, max_t, delta_t)
+    nst = B.shape[1]
+    dl_tmp_k = np.minimum(F32(max_t), (deadline - t).astype(F32)).astype(F32)
+    sqrt_st_k = (sqrt_dt32(delta_t) * s).astype(F32)
+    vdt_k = (v * dt).astype(F32)
+    total = K * N
+    lane_set = np.repeat(np.arange(K), N)
+    b0 = B[lane_set, 0]
+    y = ((-b0) + ((z[lane_set] * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
+    tt = np.zeros(total, F32)
+    ix = np.zeros(total, np.int64)
+    src = _Normals(normals, rng)
+    active = np.arange(total)
+    while active.size:
+        ks = lane_set[active]
+        ixa = ix[active]
+        inb = ixa < nst
+        b = B[ks, np.minimum(ixa, nst - 1)]
+        ya = y[active]
+        go = inb & (ya >= -b) & (ya <= b) & (tt[active] <= dl_tmp_k[ks])
+        active = active[go]
+        if not active.size:
+            break
+        ks = lane_set[active]
+        g = src.draw(active, ix[active])
+        inc = (vdt_k[ks] + (sqrt_st_k[ks] * g).astype(F32)).astype(F32)
+        y[active] = (y[active] + inc).astype(F32)
+        tt[active] = (tt[active] + dt).astype(F32)
+        ix[active] += 1
+    u = np.zeros(total, F32) if uniforms is None else np.asarray(uniforms, F32).reshape(-1)
+    if smooth_unif and uniforms is None:
+        u = (rng.uniform(0.0, 1.0, total)).astype(F32)
+    rts = _finish(None, tt, t[lane_set], dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
+    choices = ((y > 0).astype(np.int32) - (y < 0).astype(np.int32)).astype(np.int32)
+    return rts, choices, ix
+
+
+def accumulators(model, v, z, t, s, deadline, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
+                 bparams=None, g=None, b=None, smooth_unif=False, normals=None, uniforms=None, rng=None,
+                 table=None):
+    """race_model (model='race') or lca (model='lca'); returns (rts, choices, steps), set-major."""
+    v = np.asarray(v, F32)
+    K, P = v.shape
+    z = np.broadcast_to(np.asarray(z, F32), (K, P))
+    s = np.broadcast_to(np.asarray(s, F32).reshape(K, -1), (K, P))
+    t = np.asarray(t, F32).reshape(K, -1)[:, 0]
+    deadline = np.asarray(deadline, F32).reshape(-1)
+    N = int(n_samples)
+    dt = F32(delta_t)
+    bparams = dict(bparams or {})
+    B = table if table is not None else boundary_table(boundary, bparams, K, max_t, delta_t)
+    nst = B.shape[1]
+    dl_tmp_k = np.minimum(F32(max_t), (deadline - t).astype(F32)).astype(F32)
+    sst = (sqrt_dt32(delta_t) * s).astype(F32)
+    vdt = (v * dt).astype(F32)
+    gk = None if g is None else np.asarray(g, F32).reshape(K, -1)[:, 0]
+    bk = None if b is None else np.asarray(b, F32).reshape(K, -1)[:, 0]
+    total = K * N
+    lane_set = np.repeat(np.arange(K), N)
+    x = (z[lane_set] * B[lane_set, 0][:, None]).astype(F32)
+    tt = np.zeros(total, F32)
+    ix = np.zeros(total, np.int64)
+    src = _Normals(normals, rng)
+    active = np.arange(total)
+    while active.size:
+        ks = lane_set[active]
+        ixa = ix[active]
+        bnd = B[ks, np.minimum(ixa, nst - 1)]
+        crossed = np.any(x[active] > bnd[:, None], axis=1)
+        go = (ixa < nst) & ~crossed & (tt[active] <= dl_tmp_k[ks])
+        active = active[go]
+        if not active.size:
+            break
+        ks = lane_set[active]
+        xa = x[active]
+        noise = np.empty_like(xa)
+        d0 = ix[active] * P
+        for j in range(P):
+            noise[:, j] = src.draw(active, d0 + j)
+        if model == "race":
+            xn = (xa + (vdt[ks] + (sst[ks] * noise).astype(F32)).astype(F32)).astype(F32)
+        else:
+            tot = np.zeros(active.size, F32)
+            for j in range(P):  # csum: sequential float32 sum
+                tot = (tot + xa[:, j]).astype(F32)
+            red = ((-xa) + tot[:, None]).astype(F32)
+            drift = ((v[ks] - (gk[ks, None] * xa).astype(F32)).astype(F32) - (bk[ks, None] * red).astype(F32)).astype(F32)
+            drift = (drift * dt).astype(F32)
+            xn = (xa + (drift + (sst[ks] * noise).astype(F32)).astype(F32)).astype(F32)
+        x[active] = np.maximum(F32(0.0), xn)
+        tt[active] = (tt[active] + dt).astype(F32)
+        ix[active] += 1
+    u = np.zeros(total, F32) if uniforms is None else np.asarray(uniforms, F32).reshape(-1)
+    if smooth_unif and uniforms is None:
+        u = (rng.uniform(0.0, 1.0, total)).astype(F32)
+    rts = _finish(None, tt, t[lane_set], dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
+    choices = np.argmax(x, axis=1).astype(np.int32)
+    return rts, choices, ix
+
+
+def lba(v, a, z, t, deadline, sd, *, n_samples, tan_theta=None, uniforms=None, normals=None):
+    """lba_vanilla / lba_angle with explicit draws: uniforms, normals float32[n_total, nact]."""
+    v = np.asarray(v, F32)
+    K, P = v.shape
+    N = int(n_samples)
+    lane_set = np.repeat(np.arange(K), N)
+    a_ = np.asarray(a, F32).reshape(K, -1)[:, 0][lane_set, None]
+    z_ = np.asarray(z, F32).reshape(K, -1)[:, 0][lane_set, None]
+    sd_ = np.broadcast_to(np.asarray(sd, F32).reshape(K, -1), (K, P))[lane_set]
+    zs = (np.asarray(uniforms, F32) * z_).astype(F32)
+    vs = np.abs((v[lane_set] + (sd_ * np.asarray(normals, F32)).astype(F32)).astype(F32))
+    den = vs if tan_theta is None else (vs + np.asarray(tan_theta, F32).reshape(-1)[lane_set, None]).astype(F32)
+    x = ((a_ - zs).astype(F32) / den).astype(F32)
+    choices = np.argmin(x, axis=1).astype(np.int32)
+    rts = (np.min(x, axis=1) + np.asarray(t, F32).reshape(-1)[lane_set]).astype(F32)
+    rts[rts >= np.asarray(deadline, F32).reshape(-1)[lane_set]] = F32(-999.0)
+    return rts, choices

@@ -1,0 +1,69 @@
+"""Device plumbing: torch-ROCm tensors as HBM buffers and the current stream.
+
+PyTorch is only the allocator and stream provider here; every kernel is
+launched through libssmhip's C ABI with raw pointers.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+_torch = None
+
+
+def torch():
+    global _torch
+    if _torch is None:
+        import torch as t  # noqa: PLC0415
+
+        _torch = t
+    return _torch
+
+
+def require_gpu():
+    """Return the torch module after checking that a HIP device is usable."""
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError("ssmhip needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path")
+    _lib.load()
+    return t
+
+
+def device():
+    t = require_gpu()
+    return t.device("cuda", t.cuda.current_device())
+
+
+def stream_ptr(stream=None) -> int:
+    t = require_gpu()
+    s = stream if stream is not None else t.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def to_dev(a: np.ndarray, dtype=None):
+    t = require_gpu()
+    arr = np.ascontiguousarray(a if dtype is None else a.astype(dtype, copy=False))
+    return t.from_numpy(arr).to(device(), non_blocking=False)
+
+
+def empty(shape, dtype):
+    t = require_gpu()
+    return t.empty(shape, dtype=dtype, device=device())
+
+
+def zeros(shape, dtype):
+    t = require_gpu()
+    return t.zeros(shape, dtype=dtype, device=device())
+
+
+def ptr(x) -> int | None:
+    """Raw device pointer of a tensor (None -> NULL)."""
+    if x is None:
+        return None
+    if not x.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not x.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return int(x.data_ptr())

@@ -1,0 +1,99 @@
+"""Launch-shape autotuner for the persistent EM kernel.
+
+Chooses (block_size, blocks_per_cu, refill_min) — threads per block, resident
+blocks per CU in the persistent grid, and how many idle lanes a wavefront
+tolerates before it refills them from the work queue — by timing candidate
+shapes on a bounded sample of the real workload (a prefix of the flattened
+(set, sample) space, so the sample keeps the workload's mix of short and long
+walks). Results are cached per (device, model, boundary, particles) in memory
+and in a JSON file (SSMHIP_TUNE_CACHE, default ~/.cache/ssmhip/tune.json).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import _device as D
+from .em import LaunchParams
+
+DEFAULT_CANDIDATES = [
+    (256, 4, 8), (256, 8, 16), (256, 8, 32), (512, 4, 16), (512, 4, 32),
+    (256, 16, 16), (1024, 2, 16), (128, 16, 16), (256, 8, 48), (512, 8, 16),
+]
+
+_memo: dict[str, LaunchParams] = {}
+
+
+def _cache_path() -> Path:
+    return Path(os.environ.get("SSMHIP_TUNE_CACHE", str(Path.home() / ".cache" / "ssmhip" / "tune.json")))
+
+
+def _key(job) -> str:
+    t = D.torch()
+    name = t.cuda.get_device_name() if t.cuda.is_available() else "cpu"
+    p = job.prob
+    return f"{name}|m{p.model}|b{p.boundary}|p{p.n_particles}|r{int(p.replay_stride > 0)}"
+
+
+def _load_file() -> dict:
+    try:
+        return json.loads(_cache_path().read_text())
+    except Exception:
+        return {}
+
+
+def _store(key: str, lp: LaunchParams) -> None:
+    data = _load_file()
+    data[key] = [lp.block_size, lp.blocks_per_cu, lp.refill_min]
+    try:
+        path = _cache_path()
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(json.dumps(data, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def time_launch(job, lp: LaunchParams, reps: int = 2) -> float:
+    """Median wall time (ms) of `reps` launches, measured with events on the current stream."""
+    t = D.torch()
+    times = []
+    for _ in range(reps):
+        s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        s.record()
+        job.launch(lp)
+        e.record()
+        e.synchronize()
+        times.append(s.elapsed_time(e))
+    return float(np.median(times))
+
+
+def tune(job, candidates=None, use_cache: bool = True, verbose: bool = False) -> LaunchParams:
+    """Pick the fastest launch shape for `job` (an em.EmJob, already on the device)."""
+    key = _key(job)
+    if use_cache:
+        if key in _memo:
+            return _memo[key]
+        cached = _load_file().get(key)
+        if cached:
+            lp = LaunchParams(*cached)
+            _memo[key] = lp
+            return lp
+    cands = [LaunchParams(*c) for c in (candidates or DEFAULT_CANDIDATES)]
+    job.launch(cands[0])  # warm-up (module load, first-touch)
+    D.torch().cuda.synchronize()
+    best, best_ms = cands[0], float("inf")
+    for lp in cands:
+        ms = time_launch(job, lp)
+        if verbose:
+            print(f"[autotune] block={lp.block_size} per_cu={lp.blocks_per_cu} refill={lp.refill_min}: {ms:.3f} ms",
+                  flush=True)
+        if ms < best_ms:
+            best, best_ms = lp, ms
+    _memo[key] = best
+    if use_cache:
+        _store(key, best)
+    return best

@@ -1,0 +1,229 @@
+"""Drop-in replacements for the reference's compiled simulators (``cssm.*``).
+
+Each function has the keyword contract of its reference counterpart and
+returns ``{'rts': float32[n_samples, n_trials, 1], 'choices':
+intc[n_samples, n_trials, 1], 'metadata': {...}}``, so it can be installed as
+``model_config["simulator"]`` in the reference's ``Simulator``.  The walk runs
+in the gfx950 kernel with the sequential path's semantics (inclusive bounds,
+``smooth_unif`` honoured); ``n_threads`` is accepted and validated but the GPU
+ignores it.  Seeds drive a counter-based Philox stream, so the same seed gives
+the same output on every call (but not the reference's NumPy stream).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .boundary_functions import constant, kernel_boundary_code, time_grid
+from .em import accumulator_records, boundary_table, ddm_records, make_job, uniform_offsets
+
+MAX_THREADS = 256  # _constants.pxi:10
+OMISSION_SENTINEL = -999.0
+
+
+def _check_threads(n_threads: int) -> None:
+    # check_parallel_request, src/cssm/_openmp_status.pyx:177-208
+    if n_threads > MAX_THREADS:
+        raise ValueError(
+            f"n_threads={n_threads} exceeds maximum supported ({MAX_THREADS}). "
+            "This is a compile-time limit for thread-local RNG state arrays."
+        )
+
+
+def _seed(random_state) -> int:
+    if random_state is None:
+        return int(np.random.randint(0, 2**31))
+    return int(random_state) & 0xFFFFFFFFFFFFFFFF
+
+
+def _minimal_meta(name, choices, n_samples, n_trials, boundary_fun_name=None) -> dict:
+    # build_minimal_metadata, _utils.pyx:734-755
+    meta = {"simulator": name, "possible_choices": choices, "n_samples": n_samples, "n_trials": n_trials}
+    if boundary_fun_name is not None:
+        meta["boundary_fun_type"] = boundary_fun_name
+    return meta
+
+
+def _full_meta(minimal, params, sim_config, boundary_fun=None, boundary=None, traj=None, boundary_params=None,
+               extra=None) -> dict:
+    # build_full_metadata, _utils.pyx:757-792
+    meta = dict(minimal)
+    meta.update(params)
+    meta.update(sim_config)
+    if boundary_fun is not None:
+        meta["boundary_fun_type"] = boundary_fun.__name__
+    if boundary is not None:
+        meta["boundary"] = boundary
+    if traj is not None:
+        meta["trajectory"] = traj
+    if boundary_params:
+        meta.update(boundary_params)
+    if extra:
+        meta.update(extra)
+    return meta
+
+
+def _boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) -> np.ndarray:
+    t_s = time_grid(max_t, delta_t)
+    kw = {k: np.asarray(v).reshape(-1)[0] for k, v in boundary_params.items()}
+    return np.asarray(boundary_fun(t=t_s, **kw)).astype(np.float32)
+
+
+def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, seed, smooth_unif, boundary_fun,
+         boundary_params, replay_normals=None, replay_uniforms=None):
+    table = None
+    if code == L.BOUNDARY_TABLE:
+        table = boundary_table(boundary_fun, boundary_params, n_trials, max_t, delta_t)
+    job = make_job(
+        model, code, n_particles, rec, uniform_offsets(n_trials, n_samples),
+        delta_t=delta_t, max_t=max_t, seed=seed, smooth_unif=smooth_unif,
+        out_layout=1, choice_bytes=4, table=table,
+        replay_normals=replay_normals, replay_uniforms=replay_uniforms,
+    )
+    job.launch()
+    total = n_samples * n_trials
+    rts = job.rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
+    choices = job.choices[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
+    return rts, choices
+
+
+def ddm_flexbound(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20000, n_trials=1,
+                  boundary_fun=None, boundary_params={}, random_state=None, return_option="full",
+                  smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """Two-bound DDM with flexible boundary ±b(t) (ddm_models.pyx:589-878)."""
+    _check_threads(n_threads)
+    if return_option not in ("full", "minimal"):
+        raise ValueError('return_option must be either "full" or "minimal"')
+    boundary_fun = boundary_fun or constant
+    boundary_params = dict(boundary_params or {})
+    if not boundary_params and boundary_fun is constant:
+        boundary_params = {"a": a}
+    code = kernel_boundary_code(boundary_fun)
+    max_t = float(np.float32(max_t))
+    delta_t = float(np.float32(delta_t))
+    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params)
+    rts, choices = _run(L.MODEL_DDM, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t, _seed(random_state),
+                        smooth_unif, boundary_fun, boundary_params, replay_normals, replay_uniforms)
+    minimal = _minimal_meta("ddm_flexbound", [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
+    if return_option == "minimal":
+        return {"rts": rts, "choices": choices, "metadata": minimal}
+    traj = np.full((time_grid(max_t, delta_t).shape[0], 1), -999.0, dtype=np.float32)
+    full = _full_meta(
+        minimal,
+        {"v": v, "a": a, "z": z, "t": t, "s": s, "deadline": deadline},
+        {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads},
+        boundary_fun=boundary_fun,
+        boundary=_boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) if n_trials > 0 else None,
+        traj=traj,
+        boundary_params=boundary_params,
+    )
+    return {"rts": rts, "choices": choices, "metadata": full}
+
+
+def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n_trials, boundary_fun,
+                 boundary_params, random_state, return_option, smooth_unif, n_threads, g=None, b=None,
+                 replay_normals=None, replay_uniforms=None):
+    _check_threads(n_threads)
+    if return_option not in ("full", "minimal"):
+        raise ValueError('return_option must be either "full" or "minimal"')
+    v = np.asarray(v, np.float32)
+    n_particles = v.shape[1]
+    if n_particles > L.MAX_PARTICLES:
+        raise ValueError(
+            f"{name} parallel execution requires n_particles <= {L.MAX_PARTICLES}, got {n_particles}."
+        )
+    boundary_fun = boundary_fun or constant
+    boundary_params = dict(boundary_params or {})
+    code = kernel_boundary_code(boundary_fun)
+    max_t = float(np.float32(max_t))
+    delta_t = float(np.float32(delta_t))
+    rec = accumulator_records(v, z, t, s, deadline, max_t, delta_t, code, boundary_params, g=g, b=b)
+    rts, choices = _run(model, code, n_particles, rec, int(n_samples), int(n_trials), max_t, delta_t,
+                        _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
+                        replay_uniforms)
+    minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
+    if return_option == "minimal":
+        return {"rts": rts, "choices": choices, "metadata": minimal}
+    minimal["possible_choices"] = list(np.arange(0, n_particles, 1))
+    params = {"v": v, "z": z, "t": t, "deadline": deadline, "s": s}
+    if g is not None:
+        params = {"v": v, "z": z, "g": g, "b": b, "t": t, "deadline": deadline, "s": s}
+    extra = {f"v{i}": v[:, i] for i in range(n_particles)}
+    zz = np.asarray(z, np.float32)
+    extra.update({f"z{i}": zz[:, i] for i in range(zz.shape[1])})
+    traj = np.full((time_grid(max_t, delta_t).shape[0], n_particles), -999.0, dtype=np.float32)
+    full = _full_meta(
+        minimal, params, {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads},
+        boundary_fun=boundary_fun,
+        boundary=_boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) if n_trials > 0 else None,
+        traj=traj, boundary_params=boundary_params, extra=extra,
+    )
+    return {"rts": rts, "choices": choices, "metadata": full}
+
+
+def race_model(v, z, t, s, deadline, delta_t=0.001, max_t=20, n_samples=2000, n_trials=1, boundary_fun=None,
+               boundary_params={}, random_state=None, return_option="full", smooth_unif=False, n_threads=1,
+               replay_normals=None, replay_uniforms=None, **kwargs):
+    """N-accumulator race, particles clamped at 0 (race_models.pyx:141-506)."""
+    return _accumulator("race_model", L.MODEL_RACE, v, z, t, s, deadline, delta_t, max_t, n_samples, n_trials,
+                        boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads,
+                        replay_normals=replay_normals, replay_uniforms=replay_uniforms)
+
+
+def lca(v, z, g, b, t, s, deadline, delta_t=0.001, max_t=20, n_samples=2000, n_trials=1, boundary_fun=None,
+        boundary_params={}, random_state=None, return_option="full", smooth_unif=False, n_threads=1,
+        replay_normals=None, replay_uniforms=None, **kwargs):
+    """Leaky competing accumulator (race_models.pyx:509-920)."""
+    return _accumulator("lca", L.MODEL_LCA, v, z, t, s, deadline, delta_t, max_t, n_samples, n_trials,
+                        boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads, g=g,
+                        b=b, replay_normals=replay_normals, replay_uniforms=replay_uniforms)
+
+
+def _lba(name, v, a, z, deadline, sd, t, nact, n_samples, n_trials, max_t, tan_theta, random_state, replay):
+    import ctypes as C  # noqa: PLC0415
+
+    from . import _device as D  # noqa: PLC0415
+
+    torch = D.require_gpu()
+    v = np.asarray(v, np.float32).reshape(n_trials, -1)
+    rec = np.zeros((n_trials, L.SET_STRIDE), np.float32)
+    rec[:, L.P_A] = np.asarray(a, np.float32).reshape(n_trials, -1)[:, 0]
+    rec[:, L.P_Z] = np.asarray(z, np.float32).reshape(n_trials, -1)[:, 0]
+    rec[:, L.P_T0] = np.asarray(t, np.float32).reshape(-1)
+    rec[:, L.P_DEADLINE] = np.asarray(deadline, np.float32).reshape(-1)
+    rec[:, L.P_B1] = tan_theta
+    rec[:, L.P_VJ : L.P_VJ + nact] = v[:, :nact]
+    rec[:, L.P_SSTJ : L.P_SSTJ + nact] = np.broadcast_to(np.asarray(sd, np.float32).reshape(n_trials, -1), (n_trials, nact))
+    offs = uniform_offsets(n_trials, n_samples)
+    total = int(offs[-1])
+    d_rec = D.to_dev(rec)
+    d_off = D.to_dev(offs)
+    d_rep = D.to_dev(np.ascontiguousarray(replay, np.float32)) if replay is not None else None
+    rts = D.empty((max(total, 1),), torch.float32)
+    ch = D.empty((max(total, 1),), torch.int32)
+    rc = L.load().ssm_lba_simulate(nact, n_trials, total, 1, 4, _seed(random_state), D.ptr(d_rec), D.ptr(d_off),
+                                   D.ptr(d_rep), D.ptr(rts), D.ptr(ch), D.stream_ptr())
+    L.check(rc, "ssm_lba_simulate")
+    rts = rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
+    choices = ch[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
+    minimal = _minimal_meta(name, list(np.arange(0, nact, 1)), n_samples, n_trials)
+    params = {"a": a, "z": z, "deadline": deadline, "sd": sd, "t": t}
+    meta = _full_meta(minimal, params, {"max_t": max_t, "n_threads": 1},
+                      extra={f"v_{i}": v[:, i] for i in range(nact)})
+    return {"rts": rts, "choices": choices, "metadata": meta}
+
+
+def lba_vanilla(v, a, z, deadline, sd, t, nact=3, n_samples=2000, n_trials=1, max_t=20, random_state=None,
+                replay=None, **kwargs):
+    """Linear ballistic accumulator, closed form (lba_models.pyx:36-139)."""
+    return _lba("lba_vanilla", v, a, z, deadline, sd, t, int(nact), int(n_samples), int(n_trials), max_t,
+                np.float32(0.0), random_state, replay)
+
+
+def lba_angle(v, a, z, theta, deadline, sd, t, nact=3, n_samples=2000, n_trials=1, max_t=20, random_state=None,
+              replay=None, **kwargs):
+    """LBA with an angle-collapsing threshold, closed form (lba_models.pyx:144-250)."""
+    th = np.asarray(theta, np.float32).reshape(int(n_trials), -1)[:, 0]
+    return _lba("lba_angle", v, a, z, deadline, sd, t, int(nact), int(n_samples), int(n_trials), max_t,
+                np.tan(th).astype(np.float32), random_state, replay)

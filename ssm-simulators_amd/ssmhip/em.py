@@ -1,0 +1,236 @@
+"""Host side of the Euler–Maruyama kernels: per-set parameter records and launches.
+
+The per-set record (float32[64]) carries exactly the float32 quantities the
+reference computes once per trial before its inner loop:
+  ddm_flexbound  src/cssm/ddm_models.pyx:696-717  (boundary, deadline_tmp, sqrt_st)
+  race_model     src/cssm/race_models.pyx:378-414
+  lca            src/cssm/race_models.pyx:789-823
+so the kernel repeats the same float32 operations per step.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _device as D
+from . import _lib as L
+from .boundary_functions import time_grid
+
+F32 = np.float32
+
+
+def sqrt_dt32(delta_t: float) -> np.float32:
+    """float32(sqrt(double(float32 delta_t))) as _utils.pyx:652 / race_models.pyx:379."""
+    return F32(np.sqrt(np.float64(F32(delta_t))))
+
+
+def deadline_tmp32(max_t: float, deadline: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """compute_deadline_tmp (_utils.pyx:712-718): min(max_t, deadline - t) in float32."""
+    d = (np.asarray(deadline, F32) - np.asarray(t, F32)).astype(F32)
+    return np.minimum(F32(max_t), d).astype(F32)
+
+
+def angle_slope32(theta: np.ndarray) -> np.ndarray:
+    """-sin(theta)/cos(theta) evaluated per scalar in float32, as boundary_functions.angle does."""
+    th = np.asarray(theta, F32).reshape(-1)
+    return np.array([(-np.sin(x) / np.cos(x)) for x in th], dtype=F32)
+
+
+def boundary_table(fun, params: dict, n_sets: int, max_t: float, delta_t: float) -> np.ndarray:
+    """Per-set boundary over t_s, as compute_boundary (_utils.pyx:655-666) builds it per trial."""
+    t_s = time_grid(max_t, delta_t)
+    out = np.empty((n_sets, t_s.shape[0]), dtype=F32)
+    for k in range(n_sets):
+        kw = {key: np.asarray(val).reshape(n_sets, -1)[k, 0] if np.ndim(val) > 0 else val for key, val in params.items()}
+        out[k, :] = np.asarray(fun(t=t_s, **kw)).astype(F32)
+    return out
+
+
+def new_records(n_sets: int) -> np.ndarray:
+    return np.zeros((n_sets, L.SET_STRIDE), dtype=F32)
+
+
+def fill_boundary(rec: np.ndarray, code: int, params: dict) -> None:
+    """Write boundary parameters of the in-register boundary kinds into the records."""
+    n = rec.shape[0]
+
+    def col(name):
+        return np.asarray(params[name], F32).reshape(n, -1)[:, 0]
+
+    if code == L.BOUNDARY_ANGLE:
+        rec[:, L.P_B1] = angle_slope32(col("theta"))
+    elif code == L.BOUNDARY_WEIBULL:
+        rec[:, L.P_B1] = col("alpha")
+        rec[:, L.P_B2] = col("beta")
+
+
+def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams) -> np.ndarray:
+    v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
+    n = v.shape[0]
+    rec = new_records(n)
+    rec[:, L.P_V] = v
+    rec[:, L.P_Z] = z
+    rec[:, L.P_T0] = t
+    rec[:, L.P_DEADLINE] = deadline
+    rec[:, L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
+    rec[:, L.P_SQRT_ST] = (sqrt_dt32(delta_t) * s).astype(F32)
+    a_b = np.asarray(bparams.get("a", a), F32).reshape(n, -1)[:, 0]
+    rec[:, L.P_A] = a_b
+    fill_boundary(rec, code, bparams)
+    return rec
+
+
+def accumulator_records(v, z, t, s, deadline, max_t, delta_t, code, bparams, g=None, b=None) -> np.ndarray:
+    v = np.asarray(v, F32)
+    z = np.asarray(z, F32)
+    s = np.asarray(s, F32)
+    n, p = v.shape
+    if p > L.MAX_PARTICLES:
+        raise ValueError(f"n_particles must be <= {L.MAX_PARTICLES}, got {p}")
+    t = np.asarray(t, F32).reshape(n, -1)[:, 0]
+    deadline = np.asarray(deadline, F32).reshape(-1)
+    rec = new_records(n)
+    rec[:, L.P_T0] = t
+    rec[:, L.P_DEADLINE] = deadline
+    rec[:, L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
+    rec[:, L.P_VJ : L.P_VJ + p] = v
+    rec[:, L.P_ZJ : L.P_ZJ + p] = np.broadcast_to(z, (n, p))
+    rec[:, L.P_SSTJ : L.P_SSTJ + p] = (sqrt_dt32(delta_t) * np.broadcast_to(s, (n, p))).astype(F32)
+    if g is not None:
+        rec[:, L.P_G] = np.asarray(g, F32).reshape(n, -1)[:, 0]
+    if b is not None:
+        rec[:, L.P_BINH] = np.asarray(b, F32).reshape(n, -1)[:, 0]
+    if "a" in bparams:
+        rec[:, L.P_A] = np.asarray(bparams["a"], F32).reshape(n, -1)[:, 0]
+    fill_boundary(rec, code, bparams)
+    return rec
+
+
+@dataclass
+class LaunchParams:
+    block_size: int = 256
+    blocks_per_cu: int = 8
+    refill_min: int = 16
+
+    def as_struct(self) -> L.LaunchConfig:
+        return L.LaunchConfig(self.block_size, self.blocks_per_cu, self.refill_min, 0)
+
+
+@dataclass
+class EmJob:
+    """Everything one persistent launch needs, already resident on the device."""
+
+    prob: L.EmProblem
+    records: object  # torch float32 [n_sets, 64]
+    offsets: object  # torch int64 [n_sets + 1]
+    table: object | None
+    replay_normals: object | None
+    replay_uniforms: object | None
+    rts: object
+    choices: object
+    workspace: object  # torch int64 [4]
+
+    def pointers(self):
+        return (
+            D.ptr(self.records),
+            D.ptr(self.offsets),
+            D.ptr(self.table),
+            D.ptr(self.replay_normals),
+            D.ptr(self.replay_uniforms),
+            D.ptr(self.rts),
+            D.ptr(self.choices),
+            D.ptr(self.workspace),
+        )
+
+    def launch(self, launch: LaunchParams | None = None, stream=None) -> None:
+        lc = (launch or LaunchParams()).as_struct()
+        rc = L.load().ssm_em_simulate(C.byref(self.prob), C.byref(lc), *self.pointers(), D.stream_ptr(stream))
+        L.check(rc, "ssm_em_simulate")
+
+    def steps(self) -> int:
+        """EM steps executed by the last completed launch (device counter)."""
+        return int(self.workspace[1].item())
+
+
+def make_job(
+    model: int,
+    boundary: int,
+    n_particles: int,
+    records: np.ndarray,
+    offsets: np.ndarray,
+    *,
+    delta_t: float,
+    max_t: float,
+    seed: int,
+    smooth_unif: bool,
+    out_layout: int = 0,
+    choice_bytes: int = 4,
+    table: np.ndarray | None = None,
+    replay_normals: np.ndarray | None = None,
+    replay_uniforms: np.ndarray | None = None,
+) -> EmJob:
+    t = D.require_gpu()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    n_sets = records.shape[0]
+    if offsets.shape != (n_sets + 1,):
+        raise ValueError("offsets must have n_sets + 1 entries")
+    if np.any(np.diff(offsets) < 0) or offsets[0] != 0:
+        raise ValueError("offsets must start at 0 and be non-decreasing")
+    n_total = int(offsets[-1])
+    from .boundary_functions import n_steps as _n_steps
+
+    nst = _n_steps(max_t, delta_t)
+    stride = 0
+    if replay_normals is not None:
+        replay_normals = np.ascontiguousarray(replay_normals, dtype=F32)
+        if replay_normals.ndim != 2 or replay_normals.shape[0] != n_total:
+            raise ValueError("replay_normals must have shape (n_total, stride)")
+        stride = int(replay_normals.shape[1])
+        if stride < 1:
+            raise ValueError("replay stride must be >= 1")
+    if table is not None:
+        table = np.ascontiguousarray(table, dtype=F32)
+        if table.shape != (n_sets, nst):
+            raise ValueError(f"boundary table must have shape ({n_sets}, {nst})")
+    prob = L.EmProblem(
+        model=model,
+        boundary=boundary,
+        n_particles=n_particles,
+        n_sets=n_sets,
+        n_total=n_total,
+        n_steps=nst,
+        dt=float(F32(delta_t)),
+        smooth_unif=int(bool(smooth_unif)),
+        out_layout=out_layout,
+        choice_bytes=choice_bytes,
+        pad0=0,
+        seed=int(seed) & 0xFFFFFFFFFFFFFFFF,
+        replay_stride=stride,
+    )
+    cdt = t.int8 if choice_bytes == 1 else t.int32
+    return EmJob(
+        prob=prob,
+        records=D.to_dev(records, F32),
+        offsets=D.to_dev(offsets, np.int64),
+        table=D.to_dev(table) if table is not None else None,
+        replay_normals=D.to_dev(replay_normals) if replay_normals is not None else None,
+        replay_uniforms=D.to_dev(np.ascontiguousarray(replay_uniforms, F32)) if replay_uniforms is not None else None,
+        rts=D.empty((max(n_total, 1),), t.float32),
+        choices=D.empty((max(n_total, 1),), cdt),
+        workspace=D.zeros((L.EM_WORKSPACE_WORDS,), t.int64),
+    )
+
+
+def uniform_offsets(n_sets: int, n_samples: int) -> np.ndarray:
+    return (np.arange(n_sets + 1, dtype=np.int64) * int(n_samples)).astype(np.int64)
+
+
+def boundary_table_for(fun, bparams, n_sets, max_t, delta_t):
+    return boundary_table(fun, bparams, n_sets, max_t, delta_t)
+
+
+def time_grid_for(max_t, delta_t):
+    return time_grid(max_t, delta_t)

@@ -1,0 +1,55 @@
+"""The C ABI library loads and exports every entry point include/ssmhip.h declares (CPU, no GPU calls)."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+REPO = Path(__file__).resolve().parent.parent
+HEADER = REPO / "include" / "ssmhip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ssm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "ssm_em_simulate" in names and "ssm_last_error" in names
+    assert len(names) >= 8
+
+
+def test_library_exports_every_declared_symbol():
+    from ssmhip import _lib
+
+    if not _lib.lib_path().exists():
+        pytest.fail(f"libssmhip.so is not built at {_lib.lib_path()}; run __graft_entry__.build()")
+    lib = C.CDLL(str(_lib.lib_path()))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_abi_version_and_error_string_without_gpu():
+    from ssmhip import _lib
+
+    lib = _lib.load()
+    assert lib.ssm_abi_version() == 1
+    assert isinstance(lib.ssm_last_error(), bytes)
+
+
+def test_em_rejects_bad_descriptor_before_touching_the_gpu():
+    from ssmhip import _lib
+
+    lib = _lib.load()
+    prob = _lib.EmProblem(model=7, boundary=0, n_particles=1, n_sets=1, n_total=1, n_steps=10, dt=0.001)
+    rc = lib.ssm_em_simulate(C.byref(prob), None, *([None] * 8), None)
+    assert rc == _lib.SSM_EINVAL
+    assert "unknown model" in _lib.last_error()
+    prob = _lib.EmProblem(model=0, boundary=0, n_particles=2, n_sets=1, n_total=1, n_steps=10, dt=0.001)
+    assert lib.ssm_em_simulate(C.byref(prob), None, *([None] * 8), None) == _lib.SSM_EINVAL
+    assert "n_particles" in _lib.last_error()
+    with pytest.raises(ValueError, match="n_particles"):
+        _lib.check(_lib.SSM_EINVAL)

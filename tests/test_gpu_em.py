@@ -1,0 +1,287 @@
+"""GPU parity of the EM kernels against the oracle (run on an MI355X).
+
+Replay mode: the kernel and the float32 NumPy transcription consume the same
+normal-increment buffer; every trajectory must give the identical choice and
+an rt within one dt (in practice bit-identical). Native mode: Philox +
+Box–Muller on the GPU against the C++ MT19937 + Ziggurat restatement and the
+Wiener closed form, at 1e5 samples: KS p > 0.01 per choice, choice
+probabilities and the 0.1..0.9 rt quantiles within 3 Monte-Carlo SEs.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import cpu_baseline, ddm_analytic
+from oracle import em_numpy as E
+from oracle import philox as PH
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+DT = 0.001
+
+
+def _ssm():
+    from ssmhip import _lib as L
+    from ssmhip import em
+
+    return L, em
+
+
+# ----------------------------------------------------------------------------- RNG
+def test_philox_raw_bits_match_oracle(torch_gpu):
+    import ctypes as C
+
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+
+    torch = torch_gpu
+    n = 4096
+    raw = D.empty((n, 4), torch.int32)
+    nz = D.empty((n, 4), torch.float32)
+    seed = 0x0123456789ABCDEF
+    rc = L.load().ssm_philox_fill(seed, 5, 7, 11, 13, n, D.ptr(raw), D.ptr(nz), D.stream_ptr())
+    L.check(rc)
+    got = raw.cpu().numpy().view(np.uint32)
+    ctr = np.zeros((n, 4), np.uint32)
+    ctr[:, 0] = 5 + np.arange(n, dtype=np.uint32)
+    ctr[:, 1], ctr[:, 2], ctr[:, 3] = 7, 11, 13
+    expect = PH.philox4x32_10(ctr, PH.key_from_seed(seed))
+    assert np.array_equal(got, expect)
+    # Box–Muller with hardware log2/sin/cos vs float64
+    ref = PH.normals4(ctr, seed)
+    err = np.abs(nz.cpu().numpy().astype(np.float64) - ref)
+    assert err.max() < 5e-5 * max(1.0, np.abs(ref).max())
+    assert stats.kstest(nz.cpu().numpy().reshape(-1), "norm").pvalue > 1e-3
+
+
+# ----------------------------------------------------------------------------- replay parity
+def _ddm_case(bname, K, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-3, 3, K).astype(F32)
+    a = rng.uniform(0.3, 2.5, K).astype(F32)
+    z = rng.uniform(0.1, 0.9, K).astype(F32)
+    t = rng.uniform(0.0, 2.0, K).astype(F32)
+    bp = {"a": a, "theta": rng.uniform(-0.1, 1.3, K).astype(F32), "alpha": rng.uniform(0.3, 5, K).astype(F32),
+          "beta": rng.uniform(0.3, 7, K).astype(F32)}
+    bp = {k: bp[k] for k in E.BOUNDARIES[bname][1]}
+    return v, a, z, t, bp
+
+
+def _run_gpu_ddm(v, a, z, t, bp, bname, N, normals, unif, smooth, use_table):
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    K = v.shape[0]
+    s = np.ones(K, F32)
+    fun = {"constant": BF.constant, "angle": BF.angle, "weibull_cdf": BF.weibull_cdf}[bname]
+    code = L.BOUNDARY_TABLE if use_table else BF.kernel_boundary_code(fun)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), s, 20.0, DT, code, bp)
+    table = em.boundary_table(fun, bp, K, 20.0, DT) if use_table else None
+    job = em.make_job(L.MODEL_DDM, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=1,
+                      smooth_unif=smooth, table=table, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    return job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy(), job.steps()
+
+
+@pytest.mark.parametrize("bname,use_table", [("constant", False), ("angle", False), ("weibull_cdf", True),
+                                             ("angle", True)])
+def test_ddm_replay_matches_oracle_trajectory_by_trajectory(torch_gpu, bname, use_table):
+    K, N, L_ = 16, 500, 1031
+    print(f"[case] ddm replay {bname} table={use_table}", flush=True)
+    v, a, z, t, bp = _ddm_case(bname, K, 21)
+    rng = np.random.default_rng(3)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 boundary=bname, bparams=bp, smooth_unif=True, normals=normals, uniforms=unif)
+    r1, c1, steps = _run_gpu_ddm(v, a, z, t, bp, bname, N, normals, unif, True, use_table)
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - r1) <= DT)
+    assert np.mean(r0 == r1) > 0.999
+    assert steps == int(s0.sum())
+
+
+def test_ddm_weibull_in_register_vs_table(torch_gpu):
+    # analytic Weibull uses hardware exp2/log2, the table uses NumPy float32 exp/power:
+    # boundary values differ by ulps, so rare crossings may shift; bound the rate.
+    K, N, L_ = 16, 1000, 1031
+    v, a, z, t, bp = _ddm_case("weibull_cdf", K, 22)
+    rng = np.random.default_rng(4)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, _ = _run_gpu_ddm(v, a, z, t, bp, "weibull_cdf", N, normals, unif, False, True)
+    r1, c1, _ = _run_gpu_ddm(v, a, z, t, bp, "weibull_cdf", N, normals, unif, False, False)
+    assert np.mean(c0 != c1) < 2e-3
+    assert np.mean(np.abs(r0 - r1) > DT) < 2e-3
+
+
+@pytest.mark.parametrize("model", ["race", "lca"])
+@pytest.mark.parametrize("bname", ["constant", "weibull_cdf"])
+def test_accumulator_replay_matches_oracle(torch_gpu, model, bname):
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    print(f"[case] {model} replay {bname}", flush=True)
+    K, P, N, L_ = 8, 4, 400, 2053
+    rng = np.random.default_rng(31)
+    v = rng.uniform(0, 2.5, (K, P)).astype(F32)
+    z = rng.uniform(0, 0.5, (K, P)).astype(F32)
+    a = rng.uniform(1, 3, K).astype(F32)
+    t = rng.uniform(0, 2, K).astype(F32)
+    g = rng.uniform(-1, 1, K).astype(F32)
+    b = rng.uniform(-1, 1, K).astype(F32)
+    bp = {"a": a, "alpha": rng.uniform(0.3, 5, K).astype(F32), "beta": rng.uniform(0.3, 7, K).astype(F32)}
+    bp = {k: bp[k] for k in E.BOUNDARIES[bname][1]}
+    kw = dict(g=g, b=b) if model == "lca" else {}
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.accumulators(model, v, z, t, np.ones((K, 1), F32), np.full(K, 999, F32), n_samples=N,
+                                delta_t=DT, boundary=bname, bparams=bp, smooth_unif=True, normals=normals,
+                                uniforms=unif, **kw)
+    fun = {"constant": BF.constant, "weibull_cdf": BF.weibull_cdf}[bname]
+    rec = em.accumulator_records(v, z, t, np.ones((K, 1), F32), np.full(K, 999, F32), 20.0, DT, L.BOUNDARY_TABLE,
+                                 bp, **kw)
+    table = em.boundary_table(fun, bp, K, 20.0, DT)
+    mdl = L.MODEL_RACE if model == "race" else L.MODEL_LCA
+    job = em.make_job(mdl, L.BOUNDARY_TABLE, P, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=1,
+                      smooth_unif=True, table=table, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    r1 = job.rts[: K * N].cpu().numpy()
+    c1 = job.choices[: K * N].cpu().numpy()
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - r1) <= DT)
+    assert np.mean(r0 == r1) > 0.999
+    assert job.steps() == int(s0.sum())
+
+
+def test_replay_is_independent_of_launch_shape(torch_gpu):
+    import torch
+
+    L, em = _ssm()
+    K, N, L_ = 8, 700, 257
+    v, a, z, t, bp = _ddm_case("angle", K, 5)
+    rng = np.random.default_rng(6)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, L.BOUNDARY_ANGLE, bp)
+    outs = []
+    for lp in [em.LaunchParams(64, 1, 1), em.LaunchParams(256, 8, 16), em.LaunchParams(1024, 2, 64)]:
+        job = em.make_job(L.MODEL_DDM, L.BOUNDARY_ANGLE, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0,
+                          seed=1, smooth_unif=False, replay_normals=normals)
+        job.launch(lp)
+        torch.cuda.synchronize()
+        outs.append((job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy()))
+    for r, c in outs[1:]:
+        assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1])
+
+
+# ----------------------------------------------------------------------------- native RNG statistics
+def _quantile_check(x, y, qs=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)):
+    for q in qs:
+        def se(s):
+            d = math.sqrt(q * (1 - q) / s.size)
+            lo, hi = np.quantile(s, [max(q - d, 0), min(q + d, 1)])
+            return (hi - lo) / 2
+
+        tol = 3 * math.hypot(se(x), se(y))
+        assert abs(np.quantile(x, q) - np.quantile(y, q)) <= tol + 1e-6, (q, np.quantile(x, q), np.quantile(y, q), tol)
+
+
+def _compare_samples(r_gpu, c_gpu, r_cpu, c_cpu, choices):
+    n1, n2 = r_gpu.size, r_cpu.size
+    for c in choices:
+        p1, p2 = np.mean(c_gpu == c), np.mean(c_cpu == c)
+        se = math.sqrt(p1 * (1 - p1) / n1 + p2 * (1 - p2) / n2)
+        assert abs(p1 - p2) <= 3 * se + 1e-9, (c, p1, p2, se)
+        x, y = r_gpu[c_gpu == c], r_cpu[c_cpu == c]
+        if min(x.size, y.size) >= 200:
+            assert stats.ks_2samp(x, y).pvalue > 0.01, c
+            _quantile_check(x, y)
+
+
+def test_ddm_native_config1_matches_cpu_restatement_and_closed_form(torch_gpu):
+    import torch
+
+    L, em = _ssm()
+    N = 100_000
+    v, a, z, t = F32(1.0), F32(1.5), F32(0.5), F32(0.2)
+    rec = em.ddm_records([v], [a], [z], [t], [999.0], [1.0], 20.0, DT, L.BOUNDARY_CONSTANT, {"a": [a]})
+    job = em.make_job(L.MODEL_DDM, L.BOUNDARY_CONSTANT, 1, rec, em.uniform_offsets(1, N), delta_t=DT, max_t=20.0,
+                      seed=0, smooth_unif=False)
+    job.launch()
+    torch.cuda.synchronize()
+    rg, cg = job.rts[:N].cpu().numpy(), job.choices[:N].cpu().numpy()
+    table = E.boundary_table("constant", {"a": [a]}, 1, 20.0, DT)
+    rc_, cc_, _ = cpu_baseline.em_simulate(0, [v], [z], [E.sqrt_dt32(DT)], [t], [999.0], [20.0], table,
+                                           np.array([0, N], np.int64), dt=DT, seed=12345)
+    _compare_samples(rg, cg, rc_, cc_, (-1, 1))
+    a_eff = float(a) + ddm_analytic.SIEGMUND_BETA * math.sqrt(DT)
+    p = ddm_analytic.p_upper(1.0, a_eff, 0.5)
+    assert abs(np.mean(cg == 1) - p) < 3 * math.sqrt(p * (1 - p) / N)
+    # density against Navarro–Fuss at the Siegmund-corrected bound: histogram chi-square
+    for c in (-1, 1):
+        x = rg[cg == c] - 0.2
+        edges = np.quantile(x, np.linspace(0, 1, 21))
+        pdf_mass = []
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            grid = np.linspace(lo, hi, 64)
+            pdf_mass.append(np.trapz(ddm_analytic.density(grid, 1.0, a_eff, 0.5, c), grid))
+        pdf_mass = np.array(pdf_mass) / np.sum(pdf_mass)
+        obs = np.histogram(x, edges)[0]
+        exp_ = pdf_mass * obs.sum()
+        chi2 = np.sum((obs - exp_) ** 2 / exp_)
+        assert stats.chi2.sf(chi2, df=19) > 1e-3, (c, chi2)
+
+
+@pytest.mark.parametrize("model", ["race", "lca"])
+def test_accumulators_native_match_cpu_restatement(torch_gpu, model):
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    N, P = 100_000, 4
+    v = np.array([[1.2, 0.8, 0.5, 1.0]], F32)
+    z = np.array([[0.2, 0.1, 0.3, 0.25]], F32)
+    a = np.array([2.0], F32)
+    t = np.array([0.3], F32)
+    g = np.array([0.4], F32)
+    b = np.array([0.3], F32)
+    bp = {"a": a, "alpha": np.array([2.0], F32), "beta": np.array([3.0], F32)}
+    kw = dict(g=g, b=b) if model == "lca" else {}
+    rec = em.accumulator_records(v, z, t, np.ones((1, 1), F32), [999.0], 20.0, DT, L.BOUNDARY_WEIBULL, bp, **kw)
+    mdl = L.MODEL_RACE if model == "race" else L.MODEL_LCA
+    job = em.make_job(mdl, L.BOUNDARY_WEIBULL, P, rec, em.uniform_offsets(1, N), delta_t=DT, max_t=20.0, seed=9,
+                      smooth_unif=False)
+    job.launch()
+    torch.cuda.synchronize()
+    rg, cg = job.rts[:N].cpu().numpy(), job.choices[:N].cpu().numpy()
+    table = E.boundary_table("weibull_cdf", bp, 1, 20.0, DT)
+    rc_, cc_, _ = cpu_baseline.em_simulate(1 if model == "race" else 2, v, z, np.full((1, P), E.sqrt_dt32(DT)), t,
+                                           [999.0], [20.0], table, np.array([0, N], np.int64), dt=DT, seed=77, **kw)
+    _compare_samples(rg, cg, rc_, cc_, range(P))
+    assert BF.kernel_boundary_code(BF.weibull_cdf) == L.BOUNDARY_WEIBULL
+
+
+def test_same_seed_same_output_different_seed_different(torch_gpu):
+    import torch
+
+    L, em = _ssm()
+    rec = em.ddm_records([0.5], [1.0], [0.5], [0.3], [999.0], [1.0], 20.0, DT, L.BOUNDARY_CONSTANT, {"a": [1.0]})
+    outs = []
+    for seed in (5, 5, 6):
+        job = em.make_job(L.MODEL_DDM, L.BOUNDARY_CONSTANT, 1, rec, em.uniform_offsets(1, 5000), delta_t=DT,
+                          max_t=20.0, seed=seed, smooth_unif=True)
+        job.launch()
+        torch.cuda.synchronize()
+        outs.append(job.rts[:5000].cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    assert not np.array_equal(outs[0], outs[2])
