@@ -25,18 +25,6 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "ssm-simulators_amd"))
 sys.path.insert(0, str(REPO))
 
-# VALU issue slots per EM step of the angle-DDM walk with Philox + Box–Muller,
-# counted from the algorithm (full-rate op = 1 slot, quarter-rate op = 4):
-#   Philox4x32-10 per 4 steps: 10 rounds x (2 mul32x32->64 @4 + 2 xor3 @1) = 100
-#   Box–Muller per 4 steps: 2 x (2 cvt + fma + mul + mul + log@4 + sqrt@4 + sin@4 + cos@4 + 2 mul) = 2 x 24 = 48
-#   per step: t_s (cvt, mul), bound (add), 3 compares + 2 ands, increment (mul, add, add),
-#             t += dt, ix += 1, 3 selects = 16
-# -> (100 + 48) / 4 + 16 = 53 slots per step.
-SLOTS_PER_STEP_ANGLE = 53.0
-# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 scalar-FP32 VALU lane-slots/s
-# (the published 157.3 TFLOP/s FP32 counts FMA = 2 and packed dual-FP32 issue).
-PEAK_VALU_SLOTS = 256 * 4 * 16 * 2.4e9
-
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -88,6 +76,10 @@ def main() -> int:
     ap.add_argument("--n-sets", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--launch", type=str, default="auto", help="'auto' or block,blocks_per_cu,refill_min")
+    ap.add_argument("--no-probe", action="store_true", help="skip the VALU roof probe")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the workload shards by parameter set with no exchange, "
@@ -96,14 +88,16 @@ def main() -> int:
 
     import torch
 
-    from ssmhip import batched
+    from ssmhip import batched, roofline
+    from ssmhip.em import LaunchParams
 
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     theta, ns = batched.sample_config2(args.n_sets, seed=0)
     log(f"config2: {args.n_sets} sets, {int(ns.sum())} trajectories")
-    sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch="auto")
+    launch = "auto" if args.launch == "auto" else LaunchParams(*[int(x) for x in args.launch.split(",")])
+    sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch=launch)
     lp = sim.launch_params
     log(f"autotuned launch: block={lp.block_size} blocks/CU={lp.blocks_per_cu} refill_min={lp.refill_min}")
     sim.capture()
@@ -125,7 +119,9 @@ def main() -> int:
     steps_per_launch = sim.steps()
     traj = sim.n_total
     value = traj / (ms / 1e3)
-    achieved_slots = steps_per_launch * SLOTS_PER_STEP_ANGLE / (ms / 1e3)
+    steps_per_s = steps_per_launch / (ms / 1e3)
+    # roof: the same Philox + Box–Muller + angle-step loop, branch-free, at full occupancy
+    ideal = None if args.no_probe else roofline.probe_rate("em_block", iters=2048)
     log(f"{ms:.3f} ms/launch, {value:.4e} traj/s, {steps_per_launch / (ms / 1e3):.4e} EM steps/s (wall {wall:.2f} s)")
 
     # sanity: choices are +-1 and rts finite or the omission sentinel
@@ -157,13 +153,23 @@ def main() -> int:
         },
         "roofline": {
             "bound": "valu",
-            "achieved": achieved_slots / 1e12,
-            "peak": PEAK_VALU_SLOTS / 1e12,
-            "unit": "Tslot/s",
-            "frac": achieved_slots / PEAK_VALU_SLOTS,
+            "achieved": steps_per_s / 1e9,
+            "peak": None if ideal is None else ideal / 1e9,
+            "unit": "G EM-steps/s",
+            "frac": None if ideal is None else steps_per_s / ideal,
             "traffic": None,
+            "peak_basis": "measured: branch-free Philox4x32-10 + Box-Muller + angle-step loop at full occupancy "
+                          "(VALU issue bound; no MFMA, ~5 B of HBM traffic per trajectory)",
         },
     }
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * n * 4
     if not args.no_cpu:
         cb = cpu_baseline_leg(theta, ns, args.cpu_seconds)
         cb.pop("calibration_rate", None)

@@ -145,6 +145,15 @@ int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t 
 int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                         int64_t n, uint32_t* raw_dev, float* normals_dev, void* stream);
 
+/* ---- VALU roofline probes (no counterpart in the reference) ----------------
+ * Branch-free loops at the occupancy of `n_threads` lanes (multiple of 256),
+ * `iters` iterations each; out_dev float32[n_threads]. kind 0: the EM block
+ * (Philox4x32-10 + 2 Box–Muller pairs + 4 angle-DDM steps); kind 1: the KDE
+ * pair (sub, mul, fma, exp2, add) x4; kinds 2..8: 8 chains of one instruction
+ * (v_mad_u64_u32, v_xor_b32, v_fma_f32, v_log_f32, v_exp_f32, v_sin_f32,
+ * v_sqrt_f32). Time it with events on `stream`.                            */
+int32_t ssm_valu_probe(int32_t kind, int64_t iters, int64_t n_threads, float* out_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@ SOURCES = {
     "lba_kernel.hip": ["-ffp-contract=off"],
     "kde_kernel.hip": [],
     "mlp_kernel.hip": [],
+    "probe_kernel.hip": [],
 }
 
 ARCH = os.environ.get("SSMHIP_ARCH", "gfx950")

@@ -49,6 +49,8 @@ int32_t lba_enqueue(int32_t nact, int32_t n_sets, int64_t n_total, int32_t out_l
 int32_t philox_enqueue(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int64_t n,
                        uint32_t* raw, float* normals, hipStream_t stream);
 
+int32_t probe_enqueue(int32_t kind, int64_t iters, int64_t n_threads, float* out, hipStream_t stream);
+
 struct GraphHandle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -146,6 +148,10 @@ int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t 
 int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int64_t n,
                         uint32_t* raw_dev, float* normals_dev, void* stream) {
   return philox_enqueue(seed, c0, c1, c2, c3, n, raw_dev, normals_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_valu_probe(int32_t kind, int64_t iters, int64_t n_threads, float* out_dev, void* stream) {
+  return probe_enqueue(kind, iters, n_threads, out_dev, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -134,9 +134,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             if (j < P && x[j] > best) { best = x[j]; choice = j; }
         }
         const int64_t o = A.out_layout == 0 ? g : (int64_t)n * A.n_sets + k;
-        A.rts[o] = rt;
-        if (A.choice_bytes == 1) ((int8_t*)A.choices)[o] = (int8_t)choice;
-        else ((int32_t*)A.choices)[o] = choice;
+        if (o < A.total) {  // layout 1 assumes uniform offsets; never write past the buffers
+          A.rts[o] = rt;
+          if (A.choice_bytes == 1) ((int8_t*)A.choices)[o] = (int8_t)choice;
+          else ((int32_t*)A.choices)[o] = choice;
+        }
         my_steps += (unsigned long long)ix;
         state = kEmpty;
       }
@@ -306,10 +308,13 @@ using KernelFn = void (*)(EmArgs);
 
 template <int MODEL, int BOUND, bool REPLAY>
 static KernelFn pick_p(int p) {
-  if (MODEL == SSM_MODEL_DDM) return em_kernel<MODEL, BOUND, REPLAY, 1>;
-  if (p <= 4) return em_kernel<MODEL, BOUND, REPLAY, 4>;
-  if (p <= 8) return em_kernel<MODEL, BOUND, REPLAY, 8>;
-  return em_kernel<MODEL, BOUND, REPLAY, 16>;
+  if constexpr (MODEL == SSM_MODEL_DDM) {
+    return em_kernel<MODEL, BOUND, REPLAY, 1>;
+  } else {
+    if (p <= 4) return em_kernel<MODEL, BOUND, REPLAY, 4>;
+    if (p <= 8) return em_kernel<MODEL, BOUND, REPLAY, 8>;
+    return em_kernel<MODEL, BOUND, REPLAY, 16>;
+  }
 }
 
 template <int MODEL, bool REPLAY>
@@ -344,6 +349,8 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
   if (P->out_layout != 0 && P->out_layout != 1) return fail(SSM_EINVAL, "out_layout must be 0 or 1");
   if (P->choice_bytes != 1 && P->choice_bytes != 4) return fail(SSM_EINVAL, "choice_bytes must be 1 or 4");
   if (P->replay_stride < 0) return fail(SSM_EINVAL, "replay_stride must be >= 0");
+  if (P->out_layout == 1 && P->n_sets > 0 && P->n_total % P->n_sets != 0)
+    return fail(SSM_EINVAL, "out_layout 1 needs the same n_samples for every set");
   if (P->n_total > 0) {
     if (!set_params || !offsets || !rts || !choices || !ws) return fail(SSM_EINVAL, "NULL buffer");
     if (P->boundary == SSM_BOUNDARY_TABLE && !table) return fail(SSM_EINVAL, "TABLE boundary needs a table");

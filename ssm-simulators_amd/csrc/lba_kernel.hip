@@ -68,6 +68,7 @@ __global__ void __launch_bounds__(256) lba_kernel(LbaArgs A) {
   float rt = best + t0;
   if (rt >= deadline) rt = -999.f;
   const int64_t o = A.out_layout == 0 ? g : (int64_t)n * A.n_sets + k;
+  if (o >= A.total) return;
   A.rts[o] = rt;
   if (A.choice_bytes == 1) ((int8_t*)A.choices)[o] = (int8_t)choice;
   else ((int32_t*)A.choices)[o] = choice;

@@ -1,0 +1,111 @@
+// VALU throughput probes: the compute roofs the EM and KDE kernels are
+// measured against. Each probe runs a branch-free loop at full occupancy:
+//   kind 0  EM block: Philox4x32-10 + 2 Box–Muller pairs + 4 angle-DDM steps
+//           (the simulator's inner loop without walk termination or refill)
+//   kind 1  KDE pair: sub, mul, fma, exp2, add (the KDE evaluation inner loop)
+//   kind 2..8  single instructions, 8 independent chains per lane:
+//           2 v_mad_u64_u32, 3 v_xor_b32, 4 v_fma_f32, 5 v_log_f32,
+//           6 v_exp_f32, 7 v_sin_f32, 8 v_sqrt_f32
+// Every lane writes one float so nothing is optimised away.
+#include <hip/hip_runtime.h>
+
+#include "ssmhip_internal.h"
+
+namespace ssmhip {
+
+__global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float y = 0.1f * (float)(gid & 7), t = 0.f;
+  const float dt = 0.001f, vdt = 0.0005f, sst = 0.0316f, a = 1.5f, slope = -0.2f;
+  int ix = 0;
+  for (int64_t it = 0; it < iters; ++it) {
+    float nz[4];
+    philox_normals4((uint32_t)gid, 7u, (uint32_t)it, 0u, 0x1234u, 0x5678u, nz);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float b = a + ((float)ix * dt) * slope;
+      const bool go = (y >= -b) && (y <= b) && (t <= 20.f);
+      const float yn = y + (vdt + sst * nz[i]);
+      y = go ? yn : -y;
+      t = go ? t + dt : 0.f;
+      ix = go ? ix + 1 : 0;
+    }
+  }
+  out[gid] = y + t;
+}
+
+__global__ void __launch_bounds__(256) probe_kde_pair(int64_t iters, float* out) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float x = 0.01f * (float)(gid & 255);
+  const float c = -0.7213475f, K = 64.f;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  float s = 0.25f;
+  for (int64_t it = 0; it < iters; ++it) {
+    const float d0 = x - s, d1 = x - (s + 0.1f), d2 = x - (s + 0.2f), d3 = x - (s + 0.3f);
+    acc0 += __builtin_amdgcn_exp2f(__builtin_fmaf(d0 * d0, c, K));
+    acc1 += __builtin_amdgcn_exp2f(__builtin_fmaf(d1 * d1, c, K));
+    acc2 += __builtin_amdgcn_exp2f(__builtin_fmaf(d2 * d2, c, K));
+    acc3 += __builtin_amdgcn_exp2f(__builtin_fmaf(d3 * d3, c, K));
+    s += 1e-6f;
+  }
+  out[gid] = acc0 + acc1 + acc2 + acc3;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256) probe_single(int64_t iters, float* out) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float f[8];
+  uint32_t u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    f[j] = 1.0f + 0.01f * (float)((gid + j) & 15);
+    u[j] = (uint32_t)(gid * 8 + j) | 1u;
+  }
+  for (int64_t it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (KIND == 2) {
+        const uint64_t p = (uint64_t)0xD2511F53u * (uint64_t)u[j];
+        u[j] = (uint32_t)(p >> 32) ^ (uint32_t)p;
+      } else if constexpr (KIND == 3) {
+        u[j] = (u[j] ^ 0x9E3779B9u) ^ (u[j] << 1);
+      } else if constexpr (KIND == 4) {
+        f[j] = __builtin_fmaf(f[j], 0.999f, 0.001f);
+      } else if constexpr (KIND == 5) {
+        f[j] = __builtin_amdgcn_logf(f[j]) + 2.0f;
+      } else if constexpr (KIND == 6) {
+        f[j] = __builtin_amdgcn_exp2f(f[j]) * 0.25f;
+      } else if constexpr (KIND == 7) {
+        f[j] = __builtin_amdgcn_sinf(f[j]) + 0.5f;
+      } else {
+        f[j] = __builtin_amdgcn_sqrtf(f[j]) + 0.5f;
+      }
+    }
+  }
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc += f[j] + (float)u[j];
+  out[gid] = acc;
+}
+
+int32_t probe_enqueue(int32_t kind, int64_t iters, int64_t n_threads, float* out, hipStream_t stream) {
+  if (iters < 1 || n_threads < 256 || n_threads % 256) return fail(SSM_EINVAL, "probe needs iters >= 1, threads % 256 == 0");
+  if (!out) return fail(SSM_EINVAL, "probe output is NULL");
+  const dim3 grid((unsigned)(n_threads / 256)), block(256);
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(probe_em_block, grid, block, 0, stream, iters, out); break;
+    case 1: hipLaunchKernelGGL(probe_kde_pair, grid, block, 0, stream, iters, out); break;
+    case 2: hipLaunchKernelGGL(probe_single<2>, grid, block, 0, stream, iters, out); break;
+    case 3: hipLaunchKernelGGL(probe_single<3>, grid, block, 0, stream, iters, out); break;
+    case 4: hipLaunchKernelGGL(probe_single<4>, grid, block, 0, stream, iters, out); break;
+    case 5: hipLaunchKernelGGL(probe_single<5>, grid, block, 0, stream, iters, out); break;
+    case 6: hipLaunchKernelGGL(probe_single<6>, grid, block, 0, stream, iters, out); break;
+    case 7: hipLaunchKernelGGL(probe_single<7>, grid, block, 0, stream, iters, out); break;
+    case 8: hipLaunchKernelGGL(probe_single<8>, grid, block, 0, stream, iters, out); break;
+    default: return fail(SSM_EINVAL, "unknown probe kind");
+  }
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+}  // namespace ssmhip

@@ -63,7 +63,7 @@ __device__ __forceinline__ float u32_to_unit(uint32_t u) {
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
   const float u1 = u32_to_open01(a);
   const float rev = u32_to_unit(b);
-  const float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln2 log2(u1)
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln2 log2(u1)
   z0 = r * __builtin_amdgcn_cosf(rev);
   z1 = r * __builtin_amdgcn_sinf(rev);
 }

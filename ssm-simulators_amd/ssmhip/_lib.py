@@ -82,6 +82,7 @@ _SIGS = {
         C.c_int32,
         [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
+    "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (
         C.c_int32,
         [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, _VP, _VP, _VP],

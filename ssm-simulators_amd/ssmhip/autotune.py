@@ -3,9 +3,9 @@
 Chooses (block_size, blocks_per_cu, refill_min) — threads per block, resident
 blocks per CU in the persistent grid, and how many idle lanes a wavefront
 tolerates before it refills them from the work queue — by timing candidate
-shapes on a bounded sample of the real workload (a prefix of the flattened
-(set, sample) space, so the sample keeps the workload's mix of short and long
-walks). Results are cached per (device, model, boundary, particles) in memory
+shapes on a job the caller supplies (BatchedSimulation passes a bounded
+sample of its own sets, so the sample keeps the workload's mix of short and
+long walks). Results are cached per (device, model, boundary, particles) in memory
 and in a JSON file (SSMHIP_TUNE_CACHE, default ~/.cache/ssmhip/tune.json).
 """
 
@@ -57,8 +57,8 @@ def _store(key: str, lp: LaunchParams) -> None:
         pass
 
 
-def time_launch(job, lp: LaunchParams, reps: int = 2) -> float:
-    """Median wall time (ms) of `reps` launches, measured with events on the current stream."""
+def time_launch(job, lp: LaunchParams, reps: int = 3) -> float:
+    """Best-of-`reps` time (ms) of a launch, measured with events on the current stream."""
     t = D.torch()
     times = []
     for _ in range(reps):
@@ -68,7 +68,7 @@ def time_launch(job, lp: LaunchParams, reps: int = 2) -> float:
         e.record()
         e.synchronize()
         times.append(s.elapsed_time(e))
-    return float(np.median(times))
+    return float(np.min(times))
 
 
 def tune(job, candidates=None, use_cache: bool = True, verbose: bool = False) -> LaunchParams:

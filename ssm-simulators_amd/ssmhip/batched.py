@@ -88,10 +88,11 @@ class BatchedSimulation:
             self.launch_params = launch or LaunchParams()
 
     def _autotune(self, kind, code, P, rec, delta_t, max_t, seed, smooth_unif, table):
-        # tune on a bounded sample: every 8th set (keeps the parameter mix), at most 2e6 trajectories
-        idx = np.arange(0, self.n_sets, max(1, self.n_sets // 512))
+        # tune on a bounded sample: up to 1024 evenly spaced sets (keeps the parameter mix),
+        # at most 8e6 trajectories
+        idx = np.arange(0, self.n_sets, max(1, self.n_sets // 1024))
         ns = np.diff(self.offsets)[idx]
-        scale = min(1.0, 2e6 / max(1, ns.sum()))
+        scale = min(1.0, 8e6 / max(1, ns.sum()))
         ns = np.maximum(1, (ns * scale).astype(np.int64))
         offs = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
         sub = make_job(kind, code, P, rec[idx], offs, delta_t=delta_t, max_t=max_t, seed=seed,

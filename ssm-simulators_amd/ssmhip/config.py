@@ -1,0 +1,168 @@
+"""Model configurations for the models whose simulators run on the GPU.
+
+Same dictionary contract as the reference's registry entries
+(ssms/config/_modelconfig/{ddm,angle,weibull,race,lca,lba}.py): name, params,
+param_bounds, boundary_name, boundary, boundary_params, n_params,
+default_params, nchoices, choices, n_particles, simulator, and the
+simulation-time parameter transforms (ColumnStackParameters / ExpandDimension
+/ renames of ssms/transforms/simulation/common.py) expressed as callables
+``f(theta, config, n_trials) -> theta``.
+"""
+
+from __future__ import annotations
+
+from copy import deepcopy
+
+import numpy as np
+
+from . import boundary_functions as bf
+from . import cssm
+
+DEADLINE_PARAM = {"name": "deadline", "default_bounds": (0.001, 10.0), "default_value": 10.0}
+
+
+def column_stack(sources, target):
+    def f(theta, cfg, n):
+        theta[target] = np.column_stack([theta[s] for s in sources]).astype(np.float32)
+        return theta
+
+    return f
+
+
+def expand_dims(names):
+    def f(theta, cfg, n):
+        for p in names:
+            if p in theta:
+                theta[p] = np.expand_dims(theta[p], axis=1)
+        return theta
+
+    return f
+
+
+def _lba_transforms(nact):
+    def f(theta, cfg, n):
+        theta["nact"] = nact
+        theta["v"] = np.column_stack([theta[f"v{i}"] for i in range(nact)]).astype(np.float32)
+        theta["z"] = np.expand_dims(theta.pop("A"), axis=1)
+        theta["a"] = np.expand_dims(theta.pop("b"), axis=1)
+        theta["t"] = np.zeros(n).astype(np.float32)
+        return theta
+
+    return [f]
+
+
+def _lba_angle_3_transforms():
+    def f(theta, cfg, n):
+        theta["v"] = np.column_stack([theta[f"v{i}"] for i in range(3)]).astype(np.float32)
+        for p in ("a", "z", "theta"):
+            theta[p] = np.expand_dims(theta[p], axis=1)
+        theta["t"] = np.zeros(n).astype(np.float32)
+        return theta
+
+    return [f]
+
+
+def _ddm_like(name, params, bounds, defaults, bname):
+    fun, bparams, _ = bf.BOUNDARY_REGISTRY[bname]
+    return {
+        "name": name, "params": params, "param_bounds": bounds, "boundary_name": bname, "boundary": fun,
+        "boundary_params": [] if bname == "constant" else None, "n_params": len(params),
+        "default_params": defaults, "nchoices": 2, "choices": [-1, 1], "n_particles": 1,
+        "simulator": cssm.ddm_flexbound, "simulation_transforms": [],
+    }
+
+
+def _race(n, lca=False):
+    vs = [f"v{i}" for i in range(n)]
+    zs = [f"z{i}" for i in range(n)]
+    extra = ["g", "b"] if lca else []
+    params = vs + ["a"] + zs + extra + ["t"]
+    lo = [0.0] * n + [1.0] + [0.0] * n + ([-1.0, -1.0] if lca else []) + [0.0]
+    hi = [2.5] * n + [3.0] + [0.9] * n + ([1.0, 1.0] if lca else []) + [2.0]
+    dflt = [0.0] * n + [2.0] + [0.5] * n + ([0.0, 0.0] if lca else []) + [1e-3]
+    return {
+        "name": f"{'lca' if lca else 'race'}_{n}", "params": params, "param_bounds": [lo, hi],
+        "boundary_name": "constant", "boundary": bf.constant, "n_params": len(params), "default_params": dflt,
+        "nchoices": n, "choices": list(range(n)), "n_particles": n,
+        "simulator": cssm.lca if lca else cssm.race_model,
+        "simulation_transforms": [column_stack(zs, "z"), column_stack(vs, "v"),
+                                  expand_dims(["t", "a"] + (["g", "b"] if lca else []))],
+    }
+
+
+def _lba(n, bounds, defaults):
+    params = ["A", "b"] + [f"v{i}" for i in range(n)]
+    return {
+        "name": f"lba{n}", "params": params, "param_bounds": bounds, "boundary_name": "constant",
+        "boundary": bf.constant, "n_params": len(params), "default_params": defaults, "nchoices": n,
+        "choices": list(range(n)), "n_particles": n, "simulator": cssm.lba_vanilla,
+        "simulation_transforms": _lba_transforms(n),
+    }
+
+
+def _registry() -> dict:
+    reg = {
+        # ddm.py:7-26, angle.py:7-25, weibull.py:7-28
+        "ddm": _ddm_like("ddm", ["v", "a", "z", "t"], [[-3.0, 0.3, 0.1, 0.0], [3.0, 2.5, 0.9, 2.0]],
+                         [0.0, 1.0, 0.5, 1e-3], "constant"),
+        "angle": _ddm_like("angle", ["v", "a", "z", "t", "theta"],
+                           [[-3.0, 0.3, 0.1, 1e-3, -0.1], [3.0, 3.0, 0.9, 2.0, 1.3]], [0.0, 1.0, 0.5, 1e-3, 0.0],
+                           "angle"),
+        "weibull": _ddm_like("weibull", ["v", "a", "z", "t", "alpha", "beta"],
+                             [[-2.5, 0.3, 0.2, 1e-3, 0.31, 0.31], [2.5, 2.5, 0.8, 2.0, 4.99, 6.99]],
+                             [0.0, 1.0, 0.5, 1e-3, 3.0, 3.0], "weibull_cdf"),
+        # race.py:14-352, lca.py:15-112
+        "race_2": _race(2), "race_3": _race(3), "race_4": _race(4),
+        "lca_3": _race(3, lca=True), "lca_4": _race(4, lca=True),
+        # lba.py:94-222
+        "lba2": _lba(2, [[0.0, 0.0, 0.0, 0.1], [1.0, 1.0, 1.0, 1.1]], [0.3, 0.5, 0.5, 0.5]),
+        "lba3": _lba(3, [[0.0, 0.0, 0.0, 0.1, 0.1], [1.0, 1.0, 1.0, 1.1, 0.50]], [0.3, 0.5, 0.25, 0.5, 0.25]),
+        "lba4": _lba(4, [[0.0, 0.0, 0.0, 0.1, 0.1, 0.1], [1.0, 1.0, 1.0, 1.1, 0.50, 0.50]],
+                     [0.3, 0.5, 0.25, 0.25, 0.25, 0.25]),
+        "lba_angle_3": {
+            "name": "lba_angle_3", "params": ["v0", "v1", "v2", "a", "z", "theta"],
+            "param_bounds": [[0.0, 0.0, 0.0, 0.1, 0.0, 0], [6.0, 6.0, 6.0, 1.1, 0.5, 1.3]],
+            "boundary_name": "constant", "boundary": bf.constant, "n_params": 6,
+            "default_params": [0.5, 0.3, 0.2, 0.5, 0.2, 0.0], "nchoices": 3, "choices": [0, 1, 2],
+            "n_particles": 3, "simulator": cssm.lba_angle, "simulation_transforms": _lba_angle_3_transforms(),
+        },
+    }
+    for cfg in reg.values():
+        if cfg.get("boundary_params") is None:
+            cfg["boundary_params"] = list(bf.BOUNDARY_REGISTRY[cfg["boundary_name"]][1])
+    return reg
+
+
+_REGISTRY = _registry()
+
+
+def list_models() -> list[str]:
+    return sorted(_REGISTRY)
+
+
+def with_deadline(config: dict) -> dict:
+    """ModelConfigBuilder.with_deadline (ssms/config/model_config_builder.py:471-560)."""
+    if "deadline" in config.get("params", []):
+        return deepcopy(config)
+    cfg = deepcopy(config)
+    cfg["params"] = cfg["params"] + ["deadline"]
+    cfg["n_params"] = cfg["n_params"] + 1
+    if not cfg["name"].endswith("_deadline"):
+        cfg["name"] = cfg["name"] + "_deadline"
+    lo, hi = cfg["param_bounds"]
+    cfg["param_bounds"] = [lo + [DEADLINE_PARAM["default_bounds"][0]], hi + [DEADLINE_PARAM["default_bounds"][1]]]
+    cfg["default_params"] = cfg["default_params"] + [DEADLINE_PARAM["default_value"]]
+    cfg["deadline"] = True
+    return cfg
+
+
+def get(model_name: str, **overrides) -> dict:
+    """ModelConfigBuilder.from_model: registry lookup + '_deadline' variant + overrides."""
+    base = model_name.replace("_deadline", "")
+    if base not in _REGISTRY:
+        raise ValueError(f"Unknown model '{base}'. Available models: {list_models()}")
+    cfg = deepcopy(_REGISTRY[base])
+    cfg.update(overrides)
+    if "_deadline" in model_name:
+        cfg = with_deadline(cfg)
+    return cfg
