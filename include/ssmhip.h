@@ -145,6 +145,86 @@ int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t 
 int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                         int64_t n, uint32_t* raw_dev, float* normals_dev, void* stream);
 
+/* ---- KDE likelihood labelling ------------------------------------------------
+ * Per (set, choice) Gaussian KDE on log RT, C = n_choices <= 16 possible
+ * choice values choice_values_dev[C]; group g = set * C + c.
+ *
+ * ssm_kde_fit replaces LogKDE.__init__ (ssms/support_utils/kde_class.py:48-87:
+ * _attach_data_from_simulator :438-504, compute_bandwidths :89-126,
+ * bandwidth_silverman :508-553, the KernelDensity fit :174-176). For every
+ * set (rts/choices segment offsets[k]..offsets[k+1]) it writes log(rt) of the
+ * non-omitted (rt != -999) draws of each choice, grouped by choice, into
+ * samples_dev[total] and per group: kde_count (kept draws), choice_total (all
+ * draws, p_c = choice_total / N), bandwidth (Silverman h; 0 = no base data)
+ * and kde_start (absolute index into samples_dev).                           */
+int32_t ssm_kde_fit(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, const float* rts_dev,
+                    const void* choices_dev, int32_t choice_bytes, const int64_t* offsets_dev, float* samples_dev,
+                    int32_t* kde_count_dev, int32_t* choice_total_dev, float* bandwidth_dev,
+                    int64_t* kde_start_dev, void* stream);
+
+/* ssm_kde_sample replaces LogKDE.kde_sample (kde_class.py:331-434) and the
+ * uniform-row draws of ResampleMixtureStrategy.generate
+ * (ssms/dataset_generators/strategies/resample_mixture_strategy.py:160-200).
+ * Per set: n_kde KDE resamples (per-choice counts round(n_kde * p_c) half to
+ * even with the reference's +-1 fix-up, or n_by_override_dev[n_sets*C]) and
+ * n_unif points rt ~ U(unif_lo, max_t) with a uniform choice; m = n_kde +
+ * n_unif <= 1024. Points are written grouped by choice: pts_rt_dev /
+ * pts_choice_dev (choice index) [n_sets*m], grp_begin/grp_count [n_sets*C].   */
+int32_t ssm_kde_sample(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t n_unif, float max_t, float unif_lo,
+                       uint64_t seed, const float* samples_dev, const int32_t* kde_count_dev,
+                       const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
+                       const int64_t* offsets_dev, const int32_t* n_by_override_dev, float* pts_rt_dev,
+                       int8_t* pts_choice_dev, int64_t* grp_begin_dev, int32_t* grp_count_dev, void* stream);
+
+/* ssm_kde_eval replaces LogKDE.kde_eval / __kde_eval_ (kde_class.py:188-329)
+ * with sklearn KernelDensity.score_samples: for the grp_count[g] points at
+ * pts_rt_dev[grp_begin[g]...] of group g writes out_ll_dev (same indexing):
+ * log p_c + log mean_i N(log max(rt, eps); x_i, h) - log max(rt, eps), floored
+ * at lb (also for rt <= 0); groups without base data get -log(N * max_t).     */
+int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev, const int32_t* grp_count_dev,
+                     const float* pts_rt_dev, const float* samples_dev, const int32_t* kde_count_dev,
+                     const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
+                     const int64_t* offsets_dev, float max_t, float lb, float eps, float* out_ll_dev, void* stream);
+
+/* ssm_lan_assemble replaces the row assembly and shuffle of
+ * ResampleMixtureStrategy.generate (resample_mixture_strategy.py:129-214):
+ * rows_dev[n_sets*m][n_theta + 3] = (theta_dev[set][:], rt, choice value, ll),
+ * rows of each set in a Philox-keyed random order when shuffle != 0.          */
+int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
+                         const float* theta_dev, const float* pts_rt_dev, const int8_t* pts_choice_dev,
+                         const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream);
+
+/* ---- likelihood surrogate (no counterpart in the reference) ------------------
+ * MLP 7 -> 100 -> 100 -> 120 -> 1 with tanh hidden layers on BF16 MFMA with
+ * FP32 accumulation; inputs are the LAN rows' (5 theta, rt, choice)
+ * (ssms/dataset_generators/strategies/resample_mixture_strategy.py:136-139).
+ * params: FP32 master weights, flat nn.Linear layout W1[100][7], b1, W2, b2,
+ * W3[120][100], b3, W4[1][120], b4 (ssm_mlp_sizes gives every length).
+ * ssm_mlp_pack converts params to bf16 MFMA fragments + padded biases.
+ * ssm_mlp_forward: y[n] = net(x[n][7]).
+ * ssm_mlp_backward: forward + backward-data for n rows; mode 0 takes the
+ *   upstream gradient dL/dy in target_or_grad, mode 1 the Huber targets
+ *   (loss scaled by `scale`); writes dx[n][8] (dL/dx, column 7 is padding),
+ *   y, per-row loss, and, for ssm_mlp_wgrad, feature-major bf16 activations
+ *   acts[act_rows][ld] and deltas[delta_rows][ld] (ld >= n, ld % 32 == 0,
+ *   columns >= n zero). Any of these outputs may be NULL.
+ * ssm_mlp_wgrad: grad[n_params] = dL/dparams from acts/deltas.
+ * ssm_mlp_adam: Adam step `step` (>= 1) on params with moments m, v.      */
+int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
+int32_t ssm_mlp_pack(const float* params_dev, void* frags_dev, float* bias_dev, void* stream);
+int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev, float* y_dev,
+                        void* stream);
+int32_t ssm_mlp_backward(const float* x_dev, const float* target_or_grad_dev, int64_t n, int32_t mode, float delta,
+                         float scale, const void* frags_dev, const float* bias_dev, void* acts_dev, void* deltas_dev,
+                         int64_t ld, float* dx_dev, float* y_dev, float* loss_dev, void* stream);
+int32_t ssm_mlp_wgrad(const void* acts_dev, const void* deltas_dev, int64_t n, int64_t ld, float* grad_dev,
+                      void* stream);
+int32_t ssm_mlp_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, int32_t step, float lr,
+                     float beta1, float beta2, float eps, float weight_decay, void* stream);
+/* One v_mfma_f32_16x16x32_bf16 with per-lane operands a[64][8], b[64][8]
+ * (bf16) and result d[64][4]: pins the operand / result lane layout.        */
+int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream);
+
 /* ---- VALU roofline probes (no counterpart in the reference) ----------------
  * Branch-free loops at the occupancy of `n_threads` lanes (multiple of 256),
  * `iters` iterations each; out_dev float32[n_threads]. kind 0: the EM block

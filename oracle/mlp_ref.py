@@ -1,0 +1,82 @@
+"""NumPy reference of the surrogate MLP (test oracle).
+
+7 -> 100 -> 100 -> 120 -> 1 with tanh hidden layers, nn.Linear parameter
+layout, flat parameter vector [W1, b1, W2, b2, W3, b3, W4, b4].
+`bf16=True` rounds every GEMM operand (inputs, weights, hidden activations,
+backward deltas) to bfloat16 (round to nearest even) and accumulates in
+float64, i.e. the arithmetic the MFMA kernels perform up to summation order;
+the tanh derivative 1 - h^2 then uses the bf16 activations, as the kernels do.
+Pinned by: PyTorch autograd of the same nn.Sequential (float64) and central
+finite differences (tests/test_oracle_mlp.py).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+LAYERS = [(100, 7), (100, 100), (120, 100), (1, 120)]
+
+
+def unflatten(p: np.ndarray):
+    out, off = [], 0
+    for o, n in LAYERS:
+        W = p[off:off + o * n].reshape(o, n)
+        off += o * n
+        b = p[off:off + o]
+        off += o
+        out.append((W, b))
+    return out
+
+
+def bf16_round(a: np.ndarray) -> np.ndarray:
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    f = np.ascontiguousarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    lsb = (u >> np.uint64(16)) & np.uint64(1)
+    u = (u + np.uint64(0x7FFF) + lsb) & np.uint64(0xFFFF0000)
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False):
+    """Returns y [n] and the list of hidden activations."""
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    h = r(np.asarray(x, np.float32))
+    hs = [h]
+    layers = unflatten(np.asarray(p, np.float32))
+    for i, (W, b) in enumerate(layers):
+        z = h @ r(W).T + np.asarray(b, np.float64)
+        if i < 3:
+            a = np.tanh(z)
+            hs.append(a)
+            h = r(a.astype(np.float32)) if bf16 else a
+        else:
+            return z[:, 0], hs
+    raise AssertionError
+
+
+def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False):
+    """dL/dx [n, 7] and dL/dp (flat) for upstream gradient gy = dL/dy."""
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    layers = unflatten(np.asarray(p, np.float32))
+    _, hs = forward(p, x, bf16)
+    # operands the kernels feed to the MFMA
+    ins = [r(np.asarray(x, np.float32))] + [r(h.astype(np.float32)) if bf16 else h for h in hs[1:]]
+    dz = np.asarray(gy, np.float64)[:, None]
+    grads = [None] * 4
+    for i in range(3, -1, -1):
+        W, _ = layers[i]
+        dzr = r(dz.astype(np.float32)) if bf16 else dz
+        grads[i] = (dzr.T @ ins[i], dzr.sum(0))
+        dh = dzr @ r(W)
+        if i > 0:
+            h = ins[i]  # the kernels take the derivative from the bf16 activations they keep
+            dz = dh * (1.0 - h ** 2)
+        else:
+            dx = dh
+    flat = np.concatenate([np.concatenate([g[0].reshape(-1), g[1]]) for g in grads])
+    return dx, flat
+
+
+def huber(e: np.ndarray, delta: float) -> np.ndarray:
+    a = np.abs(e)
+    return np.where(a <= delta, 0.5 * e * e, delta * (a - 0.5 * delta))

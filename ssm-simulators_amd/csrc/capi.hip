@@ -51,6 +51,36 @@ int32_t philox_enqueue(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uin
 
 int32_t probe_enqueue(int32_t kind, int64_t iters, int64_t n_threads, float* out, hipStream_t stream);
 
+int32_t kde_fit_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice_values, const float* rts,
+                        const void* choices, int32_t choice_bytes, const int64_t* offsets, float* samples,
+                        int32_t* kde_count, int32_t* choice_total, float* bandwidth, int64_t* kde_start,
+                        hipStream_t stream);
+int32_t kde_sample_enqueue(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t n_unif, float max_t,
+                           float unif_lo, uint64_t seed, const float* samples, const int32_t* kde_count,
+                           const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
+                           const int64_t* offsets, const int32_t* n_by_override, float* pts_rt,
+                           int8_t* pts_choice, int64_t* grp_begin, int32_t* grp_count, hipStream_t stream);
+int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin, const int32_t* grp_count,
+                         const float* pts_rt, const float* samples, const int32_t* kde_count,
+                         const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
+                         const int64_t* offsets, float max_t, float lb, float eps, float* out_ll,
+                         hipStream_t stream);
+int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
+                             const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
+                             const int32_t* choice_values, float* rows, hipStream_t stream);
+
+int32_t mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
+int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStream_t s);
+int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, float* y,
+                            hipStream_t s);
+int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s);
+int32_t mlp_wgrad_enqueue(const __bf16* acts, const __bf16* deltas, int64_t n, int64_t ld, float* grad,
+                          hipStream_t s);
+int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t step, float lr, float b1, float b2,
+                         float eps, float wd, hipStream_t s);
+int32_t mfma_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s);
+
+
 struct GraphHandle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -148,6 +178,79 @@ int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t 
 int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int64_t n,
                         uint32_t* raw_dev, float* normals_dev, void* stream) {
   return philox_enqueue(seed, c0, c1, c2, c3, n, raw_dev, normals_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_kde_fit(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, const float* rts_dev,
+                    const void* choices_dev, int32_t choice_bytes, const int64_t* offsets_dev, float* samples_dev,
+                    int32_t* kde_count_dev, int32_t* choice_total_dev, float* bandwidth_dev,
+                    int64_t* kde_start_dev, void* stream) {
+  return kde_fit_enqueue(n_sets, n_choices, choice_values_dev, rts_dev, choices_dev, choice_bytes, offsets_dev,
+                         samples_dev, kde_count_dev, choice_total_dev, bandwidth_dev, kde_start_dev,
+                         (hipStream_t)stream);
+}
+
+int32_t ssm_kde_sample(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t n_unif, float max_t, float unif_lo,
+                       uint64_t seed, const float* samples_dev, const int32_t* kde_count_dev,
+                       const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
+                       const int64_t* offsets_dev, const int32_t* n_by_override_dev, float* pts_rt_dev,
+                       int8_t* pts_choice_dev, int64_t* grp_begin_dev, int32_t* grp_count_dev, void* stream) {
+  return kde_sample_enqueue(n_sets, n_choices, n_kde, n_unif, max_t, unif_lo, seed, samples_dev, kde_count_dev,
+                            choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, n_by_override_dev,
+                            pts_rt_dev, pts_choice_dev, grp_begin_dev, grp_count_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev, const int32_t* grp_count_dev,
+                     const float* pts_rt_dev, const float* samples_dev, const int32_t* kde_count_dev,
+                     const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
+                     const int64_t* offsets_dev, float max_t, float lb, float eps, float* out_ll_dev, void* stream) {
+  return kde_eval_enqueue(n_sets, n_choices, grp_begin_dev, grp_count_dev, pts_rt_dev, samples_dev, kde_count_dev,
+                          choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, max_t, lb, eps, out_ll_dev,
+                          (hipStream_t)stream);
+}
+
+int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
+                         const float* theta_dev, const float* pts_rt_dev, const int8_t* pts_choice_dev,
+                         const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream) {
+  return lan_assemble_enqueue(n_sets, n_theta, m, shuffle, seed, theta_dev, pts_rt_dev, pts_choice_dev, ll_dev,
+                              choice_values_dev, rows_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows,
+                      int32_t* delta_rows) {
+  return mlp_sizes(n_params, n_frag, n_bias, act_rows, delta_rows);
+}
+
+int32_t ssm_mlp_pack(const float* params_dev, void* frags_dev, float* bias_dev, void* stream) {
+  return mlp_pack_enqueue(params_dev, (__bf16*)frags_dev, bias_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev, float* y_dev,
+                        void* stream) {
+  return mlp_forward_enqueue(x_dev, n, (const __bf16*)frags_dev, bias_dev, y_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_mlp_backward(const float* x_dev, const float* target_or_grad_dev, int64_t n, int32_t mode, float delta,
+                         float scale, const void* frags_dev, const float* bias_dev, void* acts_dev, void* deltas_dev,
+                         int64_t ld, float* dx_dev, float* y_dev, float* loss_dev, void* stream) {
+  if (mode != 0 && mode != 1) return fail(SSM_EINVAL, "mode must be 0 (upstream gradient) or 1 (Huber)");
+  TrainArgs A{x_dev, target_or_grad_dev, (const __bf16*)frags_dev, bias_dev, (__bf16*)acts_dev,
+              (__bf16*)deltas_dev, dx_dev, y_dev, loss_dev, n, ld, mode, delta, scale};
+  return mlp_train_enqueue(A, (hipStream_t)stream);
+}
+
+int32_t ssm_mlp_wgrad(const void* acts_dev, const void* deltas_dev, int64_t n, int64_t ld, float* grad_dev,
+                      void* stream) {
+  return mlp_wgrad_enqueue((const __bf16*)acts_dev, (const __bf16*)deltas_dev, n, ld, grad_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_mlp_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, int32_t step, float lr,
+                     float beta1, float beta2, float eps, float weight_decay, void* stream) {
+  return mlp_adam_enqueue(params_dev, grad_dev, m_dev, v_dev, step, lr, beta1, beta2, eps, weight_decay,
+                          (hipStream_t)stream);
+}
+
+int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream) {
+  return mfma_probe_enqueue(a_dev, b_dev, d_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_valu_probe(int32_t kind, int64_t iters, int64_t n_threads, float* out_dev, void* stream) {

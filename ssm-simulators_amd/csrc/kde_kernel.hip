@@ -1,0 +1,466 @@
+// KDE likelihood labelling on gfx950: per-(set, choice) Gaussian KDE on log RT.
+//
+// Restates ssms/support_utils/kde_class.py:
+//   fit     _attach_data_from_simulator :438-504, compute_bandwidths :89-126,
+//           bandwidth_silverman :508-553 (h = (4/(3n))^(1/5) * max(std, 1e-3),
+//           n == 1 -> std = 10, n == 0 -> no base data)
+//   sample  kde_sample :331-434 + sklearn KernelDensity.sample
+//           (i = floor(u * n), x = N(data[i], h), rt = exp(x))
+//   eval    __kde_eval_ :251-329 + KernelDensity.score_samples:
+//           l = log p_c + log sum_i exp(-(x - x_i)^2 / 2h^2) - log n_c - log h
+//               - log(2 pi)/2 - x,  x = log(max(rt, eps)); l <= lb or rt <= 0 -> lb;
+//           a choice without base data gets -log(N * max_t).
+// and the LAN row mixture of
+//   ssms/dataset_generators/strategies/resample_mixture_strategy.py:64-214.
+//
+// Evaluation is exp-throughput bound: each (point, sample) pair costs one
+// sub, mul, fma, exp2 and add. Samples are staged through LDS in tiles and
+// read back as wave-uniform broadcasts; every thread evaluates four points so
+// one LDS read feeds four exp2. Sums carry a fixed 2^64 scale instead of a
+// running maximum: terms that underflow (> 131 nats below the scale) cannot
+// lift a log-likelihood above the floor lb = -66.774.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "ssmhip_internal.h"
+
+namespace ssmhip {
+
+constexpr int kFitThreads = 1024;
+constexpr float kOmission = -999.f;
+constexpr float kScaleLog2 = 64.f;  // sums are scaled by 2^64
+
+// ---- block-wide helpers (blockDim.x == NT, wave64) ------------------------------
+template <int NT>
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  float s = 0.f;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NT / 64; ++i) s += red[i];  // fixed order: deterministic
+    red[NT / 64] = s;
+  }
+  __syncthreads();
+  return red[NT / 64];
+}
+
+// exclusive prefix of integer v over the block; returns prefix, total in *tot
+template <int NT>
+__device__ __forceinline__ int block_scan_excl(int v, int* red, int* tot) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (l >= o) x += y;
+  }
+  __syncthreads();
+  if (l == 63) red[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < NT / 64; ++i) {
+      const int c = red[i];
+      red[i] = acc;
+      acc += c;
+    }
+    red[NT / 64] = acc;
+  }
+  __syncthreads();
+  const int res = red[w] + x - v;
+  *tot = red[NT / 64];
+  return res;
+}
+
+__device__ __forceinline__ int load_choice(const void* ch, int bytes, int64_t i) {
+  return bytes == 1 ? (int)((const int8_t*)ch)[i] : ((const int32_t*)ch)[i];
+}
+
+// ---- fit: split by choice, compact log RT, Silverman bandwidth ------------------
+struct FitArgs {
+  const float* rts;
+  const void* choices;
+  const int64_t* offsets;
+  const int32_t* choice_values;
+  float* samples;        // [total] compacted log rt, per set grouped by choice
+  int32_t* kde_count;    // [n_sets*C] samples kept (rt != -999)
+  int32_t* choice_total; // [n_sets*C] all draws with that choice (for p_c)
+  float* bandwidth;      // [n_sets*C] h, 0 when no base data
+  int64_t* kde_start;    // [n_sets*C] absolute index into samples
+  int32_t n_choices;
+  int32_t choice_bytes;
+};
+
+__global__ void __launch_bounds__(kFitThreads) kde_fit_kernel(FitArgs A) {
+  __shared__ float redf[kFitThreads / 64 + 1];
+  __shared__ int redi[kFitThreads / 64 + 1];
+  const int set = blockIdx.x;
+  const int64_t lo = A.offsets[set], hi = A.offsets[set + 1];
+  const int64_t n = hi - lo;
+  const int64_t chunk = (n + kFitThreads - 1) / kFitThreads;
+  const int64_t b = lo + (int64_t)threadIdx.x * chunk;
+  const int64_t e = (b + chunk < hi) ? b + chunk : hi;
+  int base = 0;
+  for (int c = 0; c < A.n_choices; ++c) {
+    const int cv = A.choice_values[c];
+    int mine = 0, mine_all = 0;
+    for (int64_t i = b; i < e; ++i) {
+      if (load_choice(A.choices, A.choice_bytes, i) == cv) {
+        ++mine_all;
+        if (A.rts[i] != kOmission) ++mine;
+      }
+    }
+    int tot = 0, tot_all = 0;
+    const int off = block_scan_excl<kFitThreads>(mine, redi, &tot);
+    block_scan_excl<kFitThreads>(mine_all, redi, &tot_all);
+    // compact log(rt) in sample order (deterministic) and accumulate the mean
+    float s = 0.f;
+    int w = 0;
+    for (int64_t i = b; i < e; ++i) {
+      if (load_choice(A.choices, A.choice_bytes, i) == cv) {
+        const float r = A.rts[i];
+        if (r != kOmission) {
+          const float lr = logf(r);
+          A.samples[lo + base + off + w] = lr;
+          s += lr;
+          ++w;
+        }
+      }
+    }
+    const float mean = block_sum<kFitThreads>(s, redf) / (float)(tot > 0 ? tot : 1);
+    float q = 0.f;
+    for (int j = 0; j < w; ++j) {
+      const float d = A.samples[lo + base + off + j] - mean;
+      q += d * d;
+    }
+    const float ss = block_sum<kFitThreads>(q, redf);
+    if (threadIdx.x == 0) {
+      const int g = set * A.n_choices + c;
+      float h = 0.f;
+      if (tot > 1) {
+        float sd = sqrtf(ss / (float)tot);  // np.std: population
+        if (sd < 1e-3f) sd = 1e-3f;
+        h = powf(4.0f / (3.0f * (float)tot), 0.2f) * sd;
+      } else if (tot == 1) {
+        h = powf(4.0f / 3.0f, 0.2f) * 10.0f;
+      }
+      A.kde_count[g] = tot;
+      A.choice_total[g] = tot_all;
+      A.bandwidth[g] = h;
+      A.kde_start[g] = lo + base;
+    }
+    base += tot;
+  }
+}
+
+// ---- sample: KDE resamples + uniform points, grouped by choice ------------------
+struct SampleArgs {
+  const float* samples;
+  const int32_t* kde_count;
+  const int32_t* choice_total;
+  const float* bandwidth;
+  const int64_t* kde_start;
+  const int64_t* offsets;
+  float* pts_rt;        // [n_sets*m]
+  int8_t* pts_choice;   // [n_sets*m] choice index
+  int64_t* grp_begin;   // [n_sets*C]
+  int32_t* grp_count;   // [n_sets*C]
+  const int32_t* n_by_override;  // [n_sets*C] per-choice KDE sample counts, or NULL
+  int32_t n_choices;
+  int32_t n_kde;
+  int32_t n_unif;
+  float max_t;
+  float unif_lo;
+  uint32_t k0, k1;
+};
+
+constexpr int kSampleThreads = 1024;
+
+__global__ void __launch_bounds__(kSampleThreads) kde_sample_kernel(SampleArgs A) {
+  __shared__ int n_by[SSM_MAX_PARTICLES];
+  __shared__ int redi[kSampleThreads / 64 + 1];
+  const int set = blockIdx.x;
+  const int C = A.n_choices;
+  const int m = A.n_kde + A.n_unif;
+  const int64_t N = A.offsets[set + 1] - A.offsets[set];
+  if (threadIdx.x == 0 && A.n_by_override) {
+    for (int c = 0; c < C; ++c) n_by[c] = A.n_by_override[set * C + c];
+  } else if (threadIdx.x == 0) {
+    // n_c = round(n_kde * p_c), half to even (Python round), computed exactly in integers
+    int sum = 0, arg = 0;
+    for (int c = 0; c < C; ++c) {
+      const int64_t q = (int64_t)A.n_kde * A.choice_total[set * C + c];
+      int64_t f = N > 0 ? q / N : 0;
+      const int64_t r2 = N > 0 ? 2 * (q - f * N) : 0;
+      if (r2 > N || (r2 == N && N > 0 && (f & 1))) ++f;
+      n_by[c] = (int)f;
+      sum += (int)f;
+      if (n_by[c] > n_by[arg]) arg = c;  // np.argmax: first maximum
+    }
+    if (sum > A.n_kde) n_by[arg] -= 1;
+    else if (sum < A.n_kde) n_by[arg] += 1;
+  }
+  __syncthreads();
+  // one point per thread (m <= 1024): draw it, then a stable counting sort by choice
+  const int j = threadIdx.x;
+  int c = -1;
+  float rt = 0.f;
+  if (j < A.n_kde) {
+    int acc = 0;
+    for (int q = 0; q < C; ++q) {
+      if (c < 0 && j < acc + n_by[q]) c = q;
+      acc += n_by[q];
+    }
+    if (c < 0) c = C - 1;
+    const int g = set * C + c;
+    const u32x4 r = philox4x32_10(u32x4{(uint32_t)j, (uint32_t)set, 0u, 11u}, A.k0, A.k1);
+    const int nc = A.kde_count[g];
+    if (nc > 0 && A.bandwidth[g] > 0.f) {
+      // KernelDensity.sample: i = floor(u * n), x = N(data[i], h); rt = exp(x)
+      int i = (int)(u32_to_unit(r.x) * (float)nc);
+      if (i >= nc) i = nc - 1;
+      float z0, z1;
+      box_muller(r.y, r.z, z0, z1);
+      rt = expf(A.samples[A.kde_start[g] + i] + A.bandwidth[g] * z0);
+    } else {
+      rt = u32_to_unit(r.x) * A.max_t;  // no base data: U(0, max_t) (kde_class.py:417-420)
+    }
+  } else if (j < m) {
+    // uniform rows: rt ~ U(unif_lo, max_t), choice uniform over the possible choices
+    const u32x4 r = philox4x32_10(u32x4{(uint32_t)j, (uint32_t)set, 0u, 12u}, A.k0, A.k1);
+    c = (int)(u32_to_unit(r.x) * (float)C);
+    if (c >= C) c = C - 1;
+    rt = A.unif_lo + u32_to_unit(r.y) * (A.max_t - A.unif_lo);
+  }
+  int running = 0;
+  for (int q = 0; q < C; ++q) {
+    int tot = 0;
+    const int pos = block_scan_excl<kSampleThreads>(c == q ? 1 : 0, redi, &tot);
+    if (c == q) {
+      const int64_t o = (int64_t)set * m + running + pos;
+      A.pts_rt[o] = rt;
+      A.pts_choice[o] = (int8_t)q;
+    }
+    if (threadIdx.x == 0) {
+      A.grp_begin[set * C + q] = (int64_t)set * m + running;
+      A.grp_count[set * C + q] = tot;
+    }
+    running += tot;
+  }
+}
+
+// ---- eval ----------------------------------------------------------------------------
+struct EvalArgs {
+  const float* pts_rt;
+  const int64_t* grp_begin;
+  const int32_t* grp_count;
+  const float* samples;
+  const int32_t* kde_count;
+  const int32_t* choice_total;
+  const float* bandwidth;
+  const int64_t* kde_start;
+  const int64_t* offsets;
+  float* out_ll;
+  int32_t n_choices;
+  float max_t;
+  float lb;
+  float eps;
+};
+
+constexpr int kEvalThreads = 256;
+constexpr int kPtsPerThread = 4;
+constexpr int kTile = 2048;
+
+__global__ void __launch_bounds__(kEvalThreads) kde_eval_kernel(EvalArgs A) {
+  __shared__ __attribute__((aligned(16))) float tile[kTile];
+  const int g = blockIdx.x;
+  const int set = g / A.n_choices;
+  const int cnt = A.grp_count[g];
+  const int64_t pb = A.grp_begin[g];
+  if (cnt <= 0) return;
+  const int64_t N = A.offsets[set + 1] - A.offsets[set];
+  const int nc = A.kde_count[g];
+  const float h = A.bandwidth[g];
+  if (nc == 0 || !(h > 0.f)) {
+    // choice without base data: -log(N * max_t) (kde_class.py:291-296), no floor
+    const float v = -logf((float)N * A.max_t);
+    for (int i = threadIdx.x; i < cnt; i += kEvalThreads) A.out_ll[pb + i] = v;
+    return;
+  }
+  const float p = (float)A.choice_total[g] / (float)N;
+  // log p - log n - log h - log(2 pi)/2 - 64 ln 2
+  const float cst = logf(p) - logf((float)nc) - logf(h) - 0.91893853320467274f - kScaleLog2 * 0.69314718055994531f;
+  const float nc2 = -1.4426950408889634f / (2.0f * h * h);  // -log2(e) / (2 h^2)
+  const float* smp = A.samples + A.kde_start[g];
+  for (int pbase = 0; pbase < cnt; pbase += kEvalThreads * kPtsPerThread) {
+    float x[kPtsPerThread], acc[kPtsPerThread];
+    bool valid[kPtsPerThread];
+#pragma unroll
+    for (int q = 0; q < kPtsPerThread; ++q) {
+      const int pi = pbase + q * kEvalThreads + threadIdx.x;
+      valid[q] = pi < cnt;
+      const float rt = valid[q] ? A.pts_rt[pb + pi] : 1.f;
+      x[q] = logf(fmaxf(rt, A.eps));
+      acc[q] = 0.f;
+    }
+    for (int t0 = 0; t0 < nc; t0 += kTile) {
+      const int tn = (nc - t0) < kTile ? (nc - t0) : kTile;
+      __syncthreads();
+      for (int i = threadIdx.x; i < kTile; i += kEvalThreads) tile[i] = i < tn ? smp[t0 + i] : 1e30f;
+      __syncthreads();
+      const int tn4 = (tn + 3) & ~3;
+      for (int i = 0; i < tn4; i += 4) {
+        const float4 s4 = *reinterpret_cast<const float4*>(&tile[i]);
+#pragma unroll
+        for (int q = 0; q < kPtsPerThread; ++q) {
+          float d;
+          d = x[q] - s4.x; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
+          d = x[q] - s4.y; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
+          d = x[q] - s4.z; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
+          d = x[q] - s4.w; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPtsPerThread; ++q) {
+      const int pi = pbase + q * kEvalThreads + threadIdx.x;
+      if (!valid[q]) continue;
+      const float rt = A.pts_rt[pb + pi];
+      float ll = cst + logf(acc[q]) - x[q];  // acc == 0 -> -inf -> floored
+      if (!(ll > A.lb) || !(rt > 0.f)) ll = A.lb;
+      A.out_ll[pb + pi] = ll;
+    }
+  }
+}
+
+// ---- LAN rows: [theta..., rt, choice value, ll], shuffled per set -------------------
+struct AssembleArgs {
+  const float* theta;        // [n_sets][n_theta]
+  const float* pts_rt;       // [n_sets*m]
+  const int8_t* pts_choice;  // [n_sets*m] choice index
+  const float* ll;           // [n_sets*m]
+  const int32_t* choice_values;
+  float* rows;               // [n_sets*m][n_theta + 3]
+  int32_t n_theta;
+  int32_t m;
+  int32_t shuffle;
+  uint32_t k0, k1;
+};
+
+constexpr int kSortN = 1024;
+
+__global__ void __launch_bounds__(kSortN) lan_assemble_kernel(AssembleArgs A) {
+  __shared__ uint32_t key[kSortN];
+  __shared__ int idx[kSortN];
+  const int set = blockIdx.x;
+  const int t = threadIdx.x;
+  const int m = A.m;
+  const int W = A.n_theta + 3;
+  // random keys (Philox) for a uniform random permutation; padding sorts last
+  if (t < m && A.shuffle) {
+    const u32x4 r = philox4x32_10(u32x4{(uint32_t)t, (uint32_t)set, 0u, 13u}, A.k0, A.k1);
+    key[t] = r.x & 0xFFFFFC00u | (uint32_t)t;  // distinct keys
+  } else {
+    key[t] = t < m ? (uint32_t)t : 0xFFFFFFFFu;
+  }
+  idx[t] = t;
+  __syncthreads();
+  for (int k = 2; k <= kSortN; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = t ^ j;
+      if (p > t) {
+        const bool up = (t & k) == 0;
+        const uint32_t a = key[t], b = key[p];
+        if ((a > b) == up) {
+          key[t] = b; key[p] = a;
+          const int ti = idx[t]; idx[t] = idx[p]; idx[p] = ti;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < m) {
+    const int src = idx[t];
+    const int64_t si = (int64_t)set * m + src;
+    float* row = A.rows + ((int64_t)set * m + t) * W;
+    for (int q = 0; q < A.n_theta; ++q) row[q] = A.theta[(int64_t)set * A.n_theta + q];
+    row[A.n_theta] = A.pts_rt[si];
+    row[A.n_theta + 1] = (float)A.choice_values[A.pts_choice[si]];
+    row[A.n_theta + 2] = A.ll[si];
+  }
+}
+
+// ---- host launchers --------------------------------------------------------------------
+int32_t kde_fit_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice_values, const float* rts,
+                        const void* choices, int32_t choice_bytes, const int64_t* offsets, float* samples,
+                        int32_t* kde_count, int32_t* choice_total, float* bandwidth, int64_t* kde_start,
+                        hipStream_t stream) {
+  if (n_sets < 0) return fail(SSM_EINVAL, "n_sets must be >= 0");
+  if (n_choices < 1 || n_choices > SSM_MAX_PARTICLES) return fail(SSM_EINVAL, "n_choices must be in [1, 16]");
+  if (choice_bytes != 1 && choice_bytes != 4) return fail(SSM_EINVAL, "choice_bytes must be 1 or 4");
+  if (n_sets == 0) return SSM_OK;
+  if (!choice_values || !rts || !choices || !offsets || !samples || !kde_count || !choice_total || !bandwidth ||
+      !kde_start)
+    return fail(SSM_EINVAL, "NULL buffer");
+  FitArgs A{rts, choices, offsets, choice_values, samples, kde_count, choice_total, bandwidth, kde_start,
+            n_choices, choice_bytes};
+  hipLaunchKernelGGL(kde_fit_kernel, dim3(n_sets), dim3(kFitThreads), 0, stream, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t kde_sample_enqueue(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t n_unif, float max_t,
+                           float unif_lo, uint64_t seed, const float* samples, const int32_t* kde_count,
+                           const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
+                           const int64_t* offsets, const int32_t* n_by_override, float* pts_rt,
+                           int8_t* pts_choice, int64_t* grp_begin, int32_t* grp_count, hipStream_t stream) {
+  if (n_sets < 0 || n_kde < 0 || n_unif < 0) return fail(SSM_EINVAL, "negative sizes");
+  if (n_choices < 1 || n_choices > SSM_MAX_PARTICLES) return fail(SSM_EINVAL, "n_choices must be in [1, 16]");
+  if (n_kde + n_unif > kSampleThreads) return fail(SSM_EINVAL, "n_kde + n_unif must be <= 1024 points per set");
+  if (n_sets == 0 || n_kde + n_unif == 0) return SSM_OK;
+  if (!samples || !kde_count || !choice_total || !bandwidth || !kde_start || !offsets || !pts_rt || !pts_choice ||
+      !grp_begin || !grp_count)
+    return fail(SSM_EINVAL, "NULL buffer");
+  SampleArgs A{samples,   kde_count, choice_total,  bandwidth, kde_start, offsets, pts_rt,  pts_choice,
+               grp_begin, grp_count, n_by_override, n_choices, n_kde,     n_unif,  max_t,   unif_lo,
+               (uint32_t)seed, (uint32_t)(seed >> 32)};
+  hipLaunchKernelGGL(kde_sample_kernel, dim3(n_sets), dim3(kSampleThreads), 0, stream, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin, const int32_t* grp_count,
+                         const float* pts_rt, const float* samples, const int32_t* kde_count,
+                         const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
+                         const int64_t* offsets, float max_t, float lb, float eps, float* out_ll,
+                         hipStream_t stream) {
+  if (n_sets < 0) return fail(SSM_EINVAL, "n_sets must be >= 0");
+  if (n_choices < 1 || n_choices > SSM_MAX_PARTICLES) return fail(SSM_EINVAL, "n_choices must be in [1, 16]");
+  if (n_sets == 0) return SSM_OK;
+  if (!grp_begin || !grp_count || !pts_rt || !samples || !kde_count || !choice_total || !bandwidth || !kde_start ||
+      !offsets || !out_ll)
+    return fail(SSM_EINVAL, "NULL buffer");
+  EvalArgs A{pts_rt, grp_begin, grp_count, samples, kde_count, choice_total, bandwidth, kde_start, offsets, out_ll,
+             n_choices, max_t, lb, eps};
+  hipLaunchKernelGGL(kde_eval_kernel, dim3(n_sets * n_choices), dim3(kEvalThreads), 0, stream, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
+                             const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
+                             const int32_t* choice_values, float* rows, hipStream_t stream) {
+  if (n_sets < 0 || n_theta < 0 || m < 0) return fail(SSM_EINVAL, "negative sizes");
+  if (m > kSortN) return fail(SSM_EINVAL, "at most 1024 rows per set");
+  if (n_sets == 0 || m == 0) return SSM_OK;
+  if (!theta || !pts_rt || !pts_choice || !ll || !choice_values || !rows) return fail(SSM_EINVAL, "NULL buffer");
+  AssembleArgs A{theta, pts_rt, pts_choice, ll, choice_values, rows, n_theta, m, shuffle, (uint32_t)seed,
+                 (uint32_t)(seed >> 32)};
+  hipLaunchKernelGGL(lan_assemble_kernel, dim3(n_sets), dim3(kSortN), 0, stream, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+}  // namespace ssmhip

@@ -1,0 +1,548 @@
+// Likelihood-surrogate MLP on gfx950: 7 -> 100 -> 100 -> 120 -> 1, tanh hidden
+// layers, BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation and
+// FP32 master weights. The reference has no counterpart (LAN training lives
+// outside ssm-simulators); the inputs are its lan_data columns
+// (5 angle-DDM theta, rt, choice; resample_mixture_strategy.py:136-139).
+//
+// Data flow inside a wavefront (16 rows per MFMA row tile):
+//   every layer computes Z^T = W * H^T (M = output features, N = rows,
+//   K = input features), so the accumulator of lane l holds row l%16 and
+//   output features 16a + 4(l/16) + i (tile a, i < 4). The next layer takes
+//   K block b from output tiles 2b and 2b+1: lane l's 8 K slots are exactly
+//   the 8 features it already holds, so activations never leave registers.
+//   The weight fragments are packed in that permuted K order (pack kernel).
+// Hidden widths are padded to 128 (8 tiles), the output to one 16-wide tile;
+// padded weights and biases are zero, so padded units stay at tanh(0) = 0.
+#include <hip/hip_runtime.h>
+
+#include "ssmhip_internal.h"
+
+namespace ssmhip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// real sizes
+constexpr int kIn = 7, kH1 = 100, kH2 = 100, kH3 = 120, kOut = 1;
+// parameter offsets in the flat FP32 master buffer (nn.Linear layout W[out][in])
+constexpr int oW1 = 0, oB1 = oW1 + kH1 * kIn, oW2 = oB1 + kH1, oB2 = oW2 + kH2 * kH1, oW3 = oB2 + kH2,
+              oB3 = oW3 + kH3 * kH2, oW4 = oB3 + kH3, oB4 = oW4 + kOut * kH3, kNParams = oB4 + kOut;
+// packed fragment offsets (bf16 elements); each fragment = 64 lanes x 8
+constexpr int kFrag = 64 * 8;
+constexpr int fF1 = 0;                      // fwd L1: 8 tiles x 1 block
+constexpr int fF2 = fF1 + 8 * 1 * kFrag;    // fwd L2: 8 x 4
+constexpr int fF3 = fF2 + 8 * 4 * kFrag;    // fwd L3: 8 x 4
+constexpr int fF4 = fF3 + 8 * 4 * kFrag;    // fwd L4: 1 x 4
+constexpr int fB1 = fF4 + 1 * 4 * kFrag;    // bwd L1 (dX): 1 x 4
+constexpr int fB2 = fB1 + 1 * 4 * kFrag;    // bwd L2: 8 x 4
+constexpr int fB3 = fB2 + 8 * 4 * kFrag;    // bwd L3: 8 x 4
+constexpr int fB4 = fB3 + 8 * 4 * kFrag;    // bwd L4: 8 x 1
+constexpr int kNFrag = fB4 + 8 * 1 * kFrag;
+// padded FP32 biases
+constexpr int pB1 = 0, pB2 = 128, pB3 = 256, pB4 = 384, kNBias = 400;
+// feature-major activation / delta buffers (bf16 [features][batch])
+constexpr int kActRows = 8 + 128 * 3;     // H0 (8 inputs), H1, H2, H3
+constexpr int kDeltaRows = 128 * 3 + 16;  // dZ1, dZ2, dZ3, dZ4
+
+__device__ __forceinline__ int chain_feature(int b, int g, int j) { return 16 * (2 * b + (j >> 2)) + 4 * g + (j & 3); }
+
+__device__ __forceinline__ float tanh_fast(float x) {
+  // tanh x = 1 - 2 / (exp(2x) + 1); exp via v_exp_f32 (2^y)
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Keep weight fragments in memory (L1/L2): without this the row loop's
+// invariant fragment loads are hoisted into hundreds of live registers.
+__device__ __forceinline__ const __bf16* opaque(const __bf16* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+__device__ __forceinline__ bf16x8 load_frag(const __bf16* frags, int off, int idx, int lane) {
+  return *reinterpret_cast<const bf16x8*>(frags + off + (idx * 64 + lane) * 8);
+}
+
+__device__ __forceinline__ bf16x8 pack_pair(f32x4 lo, f32x4 hi) {
+  bf16x8 r;
+  r[0] = (__bf16)lo[0]; r[1] = (__bf16)lo[1]; r[2] = (__bf16)lo[2]; r[3] = (__bf16)lo[3];
+  r[4] = (__bf16)hi[0]; r[5] = (__bf16)hi[1]; r[6] = (__bf16)hi[2]; r[7] = (__bf16)hi[3];
+  return r;
+}
+
+__device__ __forceinline__ f32x4 bias_tile(const float* bias, int off, int a, int g) {
+  const float* p = bias + off + 16 * a + 4 * g;
+  return f32x4{p[0], p[1], p[2], p[3]};
+}
+
+// ---- pack FP32 master weights into MFMA fragments -------------------------------
+__device__ float w_at(const float* P, int layer, int out, int in) {
+  switch (layer) {
+    case 1: return (out < kH1 && in < kIn) ? P[oW1 + out * kIn + in] : 0.f;
+    case 2: return (out < kH2 && in < kH1) ? P[oW2 + out * kH1 + in] : 0.f;
+    case 3: return (out < kH3 && in < kH2) ? P[oW3 + out * kH2 + in] : 0.f;
+    default: return (out < kOut && in < kH3) ? P[oW4 + out * kH3 + in] : 0.f;
+  }
+}
+
+__global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict__ frags, float* __restrict__ bias) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // element index into frags
+  if (e < kNFrag) {
+    // locate region
+    int layer, base, KB;
+    bool bwd;
+    if (e < fF2) { layer = 1; base = fF1; KB = 1; bwd = false; }
+    else if (e < fF3) { layer = 2; base = fF2; KB = 4; bwd = false; }
+    else if (e < fF4) { layer = 3; base = fF3; KB = 4; bwd = false; }
+    else if (e < fB1) { layer = 4; base = fF4; KB = 4; bwd = false; }
+    else if (e < fB2) { layer = 1; base = fB1; KB = 4; bwd = true; }
+    else if (e < fB3) { layer = 2; base = fB2; KB = 4; bwd = true; }
+    else if (e < fB4) { layer = 3; base = fB3; KB = 4; bwd = true; }
+    else { layer = 4; base = fB4; KB = 1; bwd = true; }
+    const int r = e - base;
+    const int j = r & 7, lane = (r >> 3) & 63, fi = r >> 9;
+    const int a = fi / KB, b = fi % KB, g = lane >> 4, m = 16 * a + (lane & 15);
+    float v;
+    if (!bwd) {
+      // A[m = out][k]: k from raw inputs (layer 1) or the previous layer's tiles
+      const int kin = (layer == 1) ? (8 * g + j) : chain_feature(b, g, j);
+      v = w_at(P, layer, m, kin);
+    } else {
+      // A[m = in][k = out]: k from this layer's output tiles
+      v = w_at(P, layer, chain_feature(b, g, j), m);
+    }
+    frags[e] = (__bf16)v;
+  }
+  if (e < kNBias) {
+    float v = 0.f;
+    if (e < pB2) v = e < kH1 ? P[oB1 + e] : 0.f;
+    else if (e < pB3) v = (e - pB2) < kH2 ? P[oB2 + e - pB2] : 0.f;
+    else if (e < pB4) v = (e - pB3) < kH3 ? P[oB3 + e - pB3] : 0.f;
+    else v = (e - pB4) < kOut ? P[oB4 + e - pB4] : 0.f;
+    bias[e] = v;
+  }
+}
+
+// ---- forward (inference) --------------------------------------------------------------
+// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. Two row tiles (32 rows) per wave.
+__global__ void __launch_bounds__(256) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
+                                                          const __bf16* __restrict__ frags,
+                                                          const float* __restrict__ bias, float* __restrict__ y) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r0 = wave * 32; r0 < n; r0 += nwaves * 32) {
+    frags = opaque(frags);
+    bf16x8 B[2][4];
+    f32x4 acc[2][8];
+    // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t row = r0 + 16 * t + r;
+      bf16x8 v = {};
+      if (g == 0 && row < n) {
+        const float* xr = x + row * kIn;
+#pragma unroll
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)xr[j];
+      }
+      B[t][0] = v;
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const bf16x8 w = load_frag(frags, fF1, a, lane);
+      const f32x4 bt = bias_tile(bias, pB1, a, g);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[t][a] = mfma(w, B[t][0], bt);
+    }
+    // hidden layers 2 and 3
+#pragma unroll
+    for (int L = 2; L <= 3; ++L) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) B[t][b] = pack_pair(acc[t][2 * b], acc[t][2 * b + 1]);
+      }
+      const int fo = (L == 2) ? fF2 : fF3;
+      const int bo = (L == 2) ? pB2 : pB3;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const f32x4 bt = bias_tile(bias, bo, a, g);
+        f32x4 c0 = bt, c1 = bt;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const bf16x8 w = load_frag(frags, fo, a * 4 + b, lane);
+          c0 = mfma(w, B[0][b], c0);
+          c1 = mfma(w, B[1][b], c1);
+        }
+        acc[0][a] = c0;
+        acc[1][a] = c1;
+      }
+    }
+    // output layer
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) B[t][b] = pack_pair(acc[t][2 * b], acc[t][2 * b + 1]);
+      f32x4 c = bias_tile(bias, pB4, 0, g);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), B[t][b], c);
+      const int64_t row = r0 + 16 * t + r;
+      if (g == 0 && row < n) y[row] = c[0];
+    }
+  }
+}
+
+// ---- fused forward + loss + backward-data (training and input gradients) -------------
+// mode 0: dy = grad_y[row]; mode 1: Huber(y - target, delta) averaged with weight `scale`.
+// Optional outputs: acts/deltas (feature-major bf16, ld = batch), dx [n][8] fp32, y, loss [n].
+// TrainArgs is declared in ssmhip_internal.h
+
+__device__ __forceinline__ void store_fm(__bf16* buf, int64_t ld, int feature, int64_t row, float v) {
+  buf[(int64_t)feature * ld + row] = (__bf16)v;
+}
+
+__device__ __forceinline__ float bf_at(const bf16x8 (&hb)[4], int a, int i) {
+  // element i of D tile a inside the packed operands: tile a sits in hb[a / 2] at (a & 1) * 4
+  return (float)hb[a >> 1][((a & 1) << 2) + i];
+}
+
+__global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r0 = wave * 16; r0 < A.n; r0 += nwaves * 16) {
+    A.frags = opaque(A.frags);
+    const int64_t row = r0 + r;
+    const bool live = row < A.n;
+    // hidden activations kept as the packed bf16 MFMA operands (the derivative uses them too)
+    bf16x8 x0, h1[4], h2[4], h3[4];
+    f32x4 acc[8];
+    {
+      bf16x8 v = {};
+      if (g == 0 && live) {
+#pragma unroll
+        for (int jj = 0; jj < kIn; ++jj) v[jj] = (__bf16)A.x[row * kIn + jj];
+      }
+      x0 = v;
+      if (A.acts && g == 0 && live) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) A.acts[(int64_t)jj * A.ld + row] = v[jj];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);
+      acc[a] = mfma(load_frag(A.frags, fF1, a, lane), x0, bias_tile(A.bias, pB1, a, g));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[a][i] = tanh_fast(acc[a][i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) h1[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);  // bound the fragment loads in flight (register pressure)
+      f32x4 c = bias_tile(A.bias, pB2, a, g);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fF2, a * 4 + b, lane), h1[b], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[a][i] = tanh_fast(c[i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) h2[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);  // bound the fragment loads in flight (register pressure)
+      f32x4 c = bias_tile(A.bias, pB3, a, g);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fF3, a * 4 + b, lane), h2[b], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[a][i] = tanh_fast(c[i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) h3[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+    f32x4 yo = bias_tile(A.bias, pB4, 0, g);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) yo = mfma(load_frag(A.frags, fF4, b, lane), h3[b], yo);
+    // lanes of group 0 hold y of their row in element 0
+    float dy = 0.f;
+    if (g == 0 && live) {
+      const float yv = yo[0];
+      if (A.y) A.y[row] = yv;
+      if (A.mode == 0) {
+        dy = A.target_or_grad[row];
+      } else {
+        const float e = yv - A.target_or_grad[row];
+        const float ae = fabsf(e);
+        if (A.loss) A.loss[row] = ae <= A.delta ? 0.5f * e * e : A.delta * (ae - 0.5f * A.delta);
+        dy = fminf(fmaxf(e, -A.delta), A.delta) * A.scale;
+      }
+    }
+    if (A.acts && live) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * a + 4 * g + i;
+          A.acts[(int64_t)(8 + f) * A.ld + row] = h1[a >> 1][((a & 1) << 2) + i];
+          A.acts[(int64_t)(8 + 128 + f) * A.ld + row] = h2[a >> 1][((a & 1) << 2) + i];
+          A.acts[(int64_t)(8 + 256 + f) * A.ld + row] = h3[a >> 1][((a & 1) << 2) + i];
+        }
+    }
+    // dZ4 tile: feature 0 only (lane group 0, element 0)
+    const f32x4 dz4 = {g == 0 ? dy : 0.f, 0.f, 0.f, 0.f};
+    if (A.deltas && live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) store_fm(A.deltas, A.ld, 384 + 4 * g + i, row, dz4[i]);
+    }
+    bf16x8 B[4];
+    // dh3 = W4^T dz4 (M = 128 inputs, K = one block: tile 0 = dz4, tile 1 = 0); dz3 = dh3 (1 - h3^2)
+    const bf16x8 bd4 = pack_pair(dz4, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);
+      acc[a] = mfma(load_frag(A.frags, fB4, a, lane), bd4, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float h = bf_at(h3, a, i);
+        acc[a][i] *= 1.0f - h * h;
+      }
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) store_fm(A.deltas, A.ld, 256 + 16 * a + 4 * g + i, row, acc[a][i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+    // dh2 = W3^T dz3; dz2 = dh2 (1 - h2^2)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);  // bound the fragment loads in flight (register pressure)
+      f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fB3, a * 4 + b, lane), B[b], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float h = bf_at(h2, a, i);
+        acc[a][i] = c[i] * (1.0f - h * h);
+      }
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) store_fm(A.deltas, A.ld, 128 + 16 * a + 4 * g + i, row, acc[a][i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+    // dh1 = W2^T dz2; dz1 = dh1 (1 - h1^2)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      __builtin_amdgcn_sched_barrier(0);  // bound the fragment loads in flight (register pressure)
+      f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fB2, a * 4 + b, lane), B[b], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float h = bf_at(h1, a, i);
+        acc[a][i] = c[i] * (1.0f - h * h);
+      }
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) store_fm(A.deltas, A.ld, 16 * a + 4 * g + i, row, acc[a][i]);
+    }
+    if (A.dx) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) B[b] = pack_pair(acc[2 * b], acc[2 * b + 1]);
+      f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fB1, b, lane), B[b], c);
+      // lane holds dx[row][4g + i]; inputs are features 0..6 (groups 0 and 1)
+      if (live && g < 2) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) A.dx[row * 8 + 4 * g + i] = c[i];
+      }
+    }
+  }
+}
+
+// ---- weight gradients: dW_L = dZ_L^T H_{L-1} over the batch -------------------------------
+// Tasks: L1 8x1 tiles, L2 8x8, L3 8x8, L4 1x8 (out tile a, in tile c); one wave per task.
+// grad: [kNParams] fp32 (nn.Linear layout), written (not accumulated).
+__global__ void __launch_bounds__(64) mlp_wgrad_kernel(const __bf16* __restrict__ acts,
+                                                      const __bf16* __restrict__ deltas, int64_t n, int64_t ld,
+                                                      float* __restrict__ grad) {
+  int task = blockIdx.x;
+  int layer, a, c;
+  if (task < 8) { layer = 1; a = task; c = 0; }
+  else if (task < 72) { layer = 2; a = (task - 8) / 8; c = (task - 8) % 8; }
+  else if (task < 136) { layer = 3; a = (task - 72) / 8; c = (task - 72) % 8; }
+  else { layer = 4; a = 0; c = task - 136; }
+  const int lane = threadIdx.x, g = lane >> 4, m = lane & 15;
+  const int drow = (layer == 1 ? 0 : layer == 2 ? 128 : layer == 3 ? 256 : 384) + 16 * a + m;  // dZ feature
+  const int arow = (layer == 1 ? 0 : 8 + 128 * (layer - 2)) + 16 * c + m;                     // H feature
+  const bool in_ok = layer > 1 || m < 8;  // H0 has 8 rows (7 inputs + zero)
+  const __bf16* dz = deltas + (int64_t)drow * ld;
+  const __bf16* hh = acts + (int64_t)(in_ok ? arow : 0) * ld;
+  f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+  float bsum = 0.f;
+  int64_t k = 0;
+  for (; k + 64 <= n; k += 64) {
+    bf16x8 av0 = *reinterpret_cast<const bf16x8*>(dz + k + 8 * g);
+    bf16x8 av1 = *reinterpret_cast<const bf16x8*>(dz + k + 32 + 8 * g);
+    bf16x8 bv0 = {}, bv1 = {};
+    if (in_ok) {
+      bv0 = *reinterpret_cast<const bf16x8*>(hh + k + 8 * g);
+      bv1 = *reinterpret_cast<const bf16x8*>(hh + k + 32 + 8 * g);
+    }
+    acc0 = mfma(av0, bv0, acc0);
+    acc1 = mfma(av1, bv1, acc1);
+    if (c == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bsum += (float)av0[j] + (float)av1[j];
+    }
+  }
+  for (; k < n; k += 32) {  // tail (ld is padded to a multiple of 32 with zeros)
+    bf16x8 av = *reinterpret_cast<const bf16x8*>(dz + k + 8 * g);
+    bf16x8 bv = {};
+    if (in_ok) bv = *reinterpret_cast<const bf16x8*>(hh + k + 8 * g);
+    acc0 = mfma(av, bv, acc0);
+    if (c == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bsum += (float)av[j];
+    }
+  }
+  const f32x4 acc = acc0 + acc1;
+  // D lane holds dW[out 16a + 4g + i][in 16c + m]
+  const int nin = layer == 1 ? kIn : layer == 2 ? kH1 : layer == 3 ? kH2 : kH3;
+  const int nout = layer == 1 ? kH1 : layer == 2 ? kH2 : layer == 3 ? kH3 : kOut;
+  const int ow = layer == 1 ? oW1 : layer == 2 ? oW2 : layer == 3 ? oW3 : oW4;
+  const int ob = layer == 1 ? oB1 : layer == 2 ? oB2 : layer == 3 ? oB3 : oB4;
+  const int in = 16 * c + m;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int out = 16 * a + 4 * g + i;
+    if (out < nout && in < nin) grad[ow + out * nin + in] = acc[i];
+  }
+  if (c == 0) {
+    // bias: sum over the 4 lane groups of the same m
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    const int out = 16 * a + m;
+    if (g == 0 && out < nout) grad[ob + out] = bsum;
+  }
+}
+
+// ---- Adam on the FP32 master weights ---------------------------------------------------------
+__global__ void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
+                                float* __restrict__ V, int n, float lr, float b1, float b2, float eps, float bc1,
+                                float bc2, float wd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = G[i] + wd * P[i];
+  const float m = b1 * M[i] + (1.f - b1) * g;
+  const float v = b2 * V[i] + (1.f - b2) * g * g;
+  M[i] = m;
+  V[i] = v;
+  P[i] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+}
+
+// one MFMA with per-lane operands, to pin the operand / result lane layout
+__global__ void mfma_probe_kernel(const bf16x8* a, const bf16x8* b, f32x4* d) {
+  const int l = threadIdx.x;
+  d[l] = mfma(a[l], b[l], f32x4{0.f, 0.f, 0.f, 0.f});
+}
+
+// ---- host -------------------------------------------------------------------------------------
+int32_t mfma_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s) {
+  if (!a || !b || !d) return fail(SSM_EINVAL, "NULL buffer");
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, (const bf16x8*)a, (const bf16x8*)b, (f32x4*)d);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows) {
+  if (n_params) *n_params = kNParams;
+  if (n_frag) *n_frag = kNFrag;
+  if (n_bias) *n_bias = kNBias;
+  if (act_rows) *act_rows = kActRows;
+  if (delta_rows) *delta_rows = kDeltaRows;
+  return SSM_OK;
+}
+
+int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStream_t s) {
+  if (!params || !frags || !bias) return fail(SSM_EINVAL, "NULL buffer");
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3((kNFrag + 255) / 256), dim3(256), 0, s, params, frags, bias);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, float* y,
+                            hipStream_t s) {
+  if (n < 0) return fail(SSM_EINVAL, "n must be >= 0");
+  if (n == 0) return SSM_OK;
+  if (!x || !frags || !bias || !y) return fail(SSM_EINVAL, "NULL buffer");
+  const int cus = device_cu_count();
+  if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
+  const int64_t waves_needed = (n + 31) / 32;
+  int64_t blocks = (waves_needed + 3) / 4;
+  const int64_t cap = (int64_t)cus * 16;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, frags, bias, y);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s) {
+  if (A.n < 0) return fail(SSM_EINVAL, "n must be >= 0");
+  if (A.n == 0) return SSM_OK;
+  if (!A.x || !A.frags || !A.bias || !A.target_or_grad) return fail(SSM_EINVAL, "NULL buffer");
+  if ((A.acts || A.deltas) && (A.ld < A.n || A.ld % 32)) return fail(SSM_EINVAL, "ld must be >= n and a multiple of 32");
+  const int cus = device_cu_count();
+  if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
+  const int64_t waves_needed = (A.n + 15) / 16;
+  int64_t blocks = (waves_needed + 3) / 4;
+  const int64_t cap = (int64_t)cus * 16;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mlp_wgrad_enqueue(const __bf16* acts, const __bf16* deltas, int64_t n, int64_t ld, float* grad,
+                          hipStream_t s) {
+  if (n < 1 || ld < n || ld % 32) return fail(SSM_EINVAL, "need n >= 1 and ld >= n, ld % 32 == 0");
+  if (!acts || !deltas || !grad) return fail(SSM_EINVAL, "NULL buffer");
+  hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(144), dim3(64), 0, s, acts, deltas, n, ld, grad);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t step, float lr, float b1, float b2,
+                         float eps, float wd, hipStream_t s) {
+  if (!P || !G || !M || !V) return fail(SSM_EINVAL, "NULL buffer");
+  if (step < 1) return fail(SSM_EINVAL, "step counts from 1");
+  const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, G, M, V, kNParams, lr, b1,
+                     b2, eps, bc1, bc2, wd);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+}  // namespace ssmhip

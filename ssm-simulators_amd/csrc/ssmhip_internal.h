@@ -23,6 +23,27 @@ int32_t hip_fail(hipError_t e, const char* what);
 
 int device_cu_count();
 
+// ---- surrogate MLP training launch arguments (mlp_kernel.hip) ------------------------
+// mode 0: dy = grad_y[row]; mode 1: Huber(y - target, delta) times `scale`.
+// acts / deltas: feature-major bf16 [rows][ld] (NULL = not stored); dx [n][8];
+// y [n]; loss [n] (all optional).
+struct TrainArgs {
+  const float* x;
+  const float* target_or_grad;
+  const __bf16* frags;
+  const float* bias;
+  __bf16* acts;
+  __bf16* deltas;
+  float* dx;
+  float* y;
+  float* loss;
+  int64_t n;
+  int64_t ld;
+  int32_t mode;
+  float delta;
+  float scale;
+};
+
 // ---- Philox4x32-10 -------------------------------------------------------------
 // Counter-based generator of Salmon, Moraes, Dror and Shaw, "Parallel random
 // numbers: as easy as 1, 2, 3" (SC'11): 10 rounds of two 32x32->64 multiplies

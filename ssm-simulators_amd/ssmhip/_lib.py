@@ -82,6 +82,31 @@ _SIGS = {
         C.c_int32,
         [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
+    "ssm_kde_fit": (C.c_int32, [C.c_int32, C.c_int32, _VP, _VP, _VP, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "ssm_kde_sample": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP,
+         _VP, _VP, _VP, _VP, _VP, _VP],
+    ),
+    "ssm_kde_eval": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float, _VP, _VP],
+    ),
+    "ssm_lan_assemble": (
+        C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+    ),
+    "ssm_mlp_sizes": (C.c_int32, [C.POINTER(C.c_int32)] * 5),
+    "ssm_mlp_pack": (C.c_int32, [_VP, _VP, _VP, _VP]),
+    "ssm_mlp_forward": (C.c_int32, [_VP, C.c_int64, _VP, _VP, _VP, _VP]),
+    "ssm_mlp_backward": (
+        C.c_int32,
+        [_VP, _VP, C.c_int64, C.c_int32, C.c_float, C.c_float, _VP, _VP, _VP, _VP, C.c_int64, _VP, _VP, _VP, _VP],
+    ),
+    "ssm_mlp_wgrad": (C.c_int32, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
+    "ssm_mlp_adam": (
+        C.c_int32, [_VP, _VP, _VP, _VP, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _VP]
+    ),
+    "ssm_mfma_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (
         C.c_int32,

@@ -1,0 +1,142 @@
+"""GPU checks of the BF16-MFMA surrogate kernels.
+
+Forward / backward against the NumPy oracle with the same bf16 operand
+rounding (tolerance set by float32 vs float64 accumulation order and the
+hardware exp/rcp inside tanh), against PyTorch's autograd of the same network
+in float32 (tolerance set by bf16 operands, ~1e-2 relative), and the input
+gradient against central finite differences of the float64 network.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(n, seed=0):
+    rng = np.random.default_rng(seed)
+    lo = np.array([-3.0, 0.3, 0.1, 1e-3, -0.1])
+    hi = np.array([3.0, 3.0, 0.9, 2.0, 1.3])
+    th = rng.uniform(lo, hi, (n, 5))
+    rt = rng.uniform(0.0, 20.0, (n, 1))
+    ch = rng.choice([-1.0, 1.0], (n, 1))
+    return np.concatenate([th, rt, ch], axis=1).astype(np.float32)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
+
+
+def test_mfma_lane_layout(torch_gpu):
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+
+    torch = torch_gpu
+    rng = np.random.default_rng(0)
+    A = mlp_ref.bf16_round(rng.normal(size=(16, 32)).astype(np.float32))
+    B = mlp_ref.bf16_round(rng.normal(size=(32, 16)).astype(np.float32))
+    a = np.zeros((64, 8), np.float32)
+    b = np.zeros((64, 8), np.float32)
+    for l in range(64):
+        for j in range(8):
+            a[l, j] = A[l % 16, 8 * (l // 16) + j]
+            b[l, j] = B[8 * (l // 16) + j, l % 16]
+    da = D.to_dev(a).to(torch.bfloat16)
+    db = D.to_dev(b).to(torch.bfloat16)
+    dd = D.empty((64, 4), torch.float32)
+    L.check(L.load().ssm_mfma_probe(D.ptr(da), D.ptr(db), D.ptr(dd), D.stream_ptr()))
+    d = dd.cpu().numpy()
+    ref = A @ B
+    for l in range(64):
+        for i in range(4):
+            assert abs(d[l, i] - ref[4 * (l // 16) + i, l % 16]) < 1e-3
+
+
+def test_forward_matches_oracle_and_torch(torch_gpu):
+    from ssmhip import _device as D
+    from ssmhip.surrogate import MLPSurrogate
+
+    torch = torch_gpu
+    net = MLPSurrogate(seed=1)
+    x = _inputs(4099, seed=1)  # not a multiple of 32: exercises the row tail
+    y = net(D.to_dev(x)).cpu().numpy()
+    p = net.params.cpu().numpy()
+    yb, _ = mlp_ref.forward(p, x, bf16=True)
+    assert np.max(np.abs(y - yb)) < 5e-3
+    yt = net.torch_reference().cuda()(torch.from_numpy(x).cuda())[:, 0].detach().cpu().numpy()
+    assert _rel(y, yt) < 2e-2
+
+
+def test_autograd_backward_matches_oracle_and_torch(torch_gpu):
+    from ssmhip import _device as D
+    from ssmhip.surrogate import MLPSurrogate, param_slices
+
+    torch = torch_gpu
+    net = MLPSurrogate(seed=2)
+    x_np = _inputs(1000, seed=2)
+    x = D.to_dev(x_np).requires_grad_(True)
+    params = net.params.clone().requires_grad_(True)
+    from ssmhip.surrogate import mlp_apply
+
+    y = mlp_apply(x, params, net.kern)
+    gy = np.random.default_rng(3).normal(size=1000).astype(np.float32)
+    (y * D.to_dev(gy)).sum().backward()
+    dx = x.grad.cpu().numpy()
+    dp = params.grad.cpu().numpy()
+    p = net.params.cpu().numpy()
+    dx_b, dp_b = mlp_ref.backward(p, x_np, gy.astype(np.float64), bf16=True)
+    assert _rel(dx, dx_b) < 1e-2
+    assert _rel(dp, dp_b) < 1e-2
+    # PyTorch autograd of the same network, float32, no bf16 rounding
+    ref = net.torch_reference().cuda()
+    xt = torch.from_numpy(x_np).cuda().requires_grad_(True)
+    (ref(xt)[:, 0] * torch.from_numpy(gy).cuda()).sum().backward()
+    assert _rel(dx, xt.grad.cpu().numpy()) < 3e-2
+    lin = [m for m in ref if isinstance(m, torch.nn.Linear)]
+    tg = np.concatenate([np.concatenate([m.weight.grad.cpu().numpy().reshape(-1), m.bias.grad.cpu().numpy()])
+                         for m in lin])
+    assert _rel(dp, tg) < 3e-2
+    for name, sl, _ in param_slices():
+        assert _rel(dp[sl], tg[sl]) < 6e-2, name
+
+
+def test_theta_gradient_matches_finite_differences(torch_gpu):
+    from ssmhip import _device as D
+    from ssmhip.surrogate import MLPSurrogate
+
+    net = MLPSurrogate(seed=4)
+    x = _inputs(16, seed=4)
+    g = net.input_gradient(D.to_dev(x)).cpu().numpy()
+    p = net.params.cpu().numpy()
+    h = 1e-3
+    fd = np.zeros((16, 7))
+    for j in range(7):
+        e = np.zeros_like(x, dtype=np.float64)
+        e[:, j] = h
+        fp, _ = mlp_ref.forward(p, (x + e).astype(np.float32))
+        fm, _ = mlp_ref.forward(p, (x - e).astype(np.float32))
+        fd[:, j] = (fp - fm) / (2 * h)
+    assert _rel(g[:, :5], fd[:, :5]) < 3e-2
+
+
+def test_training_reduces_huber_loss_and_checkpoint_roundtrip(torch_gpu):
+    from ssmhip import _device as D
+    from ssmhip.surrogate import MLPSurrogate
+
+    torch = torch_gpu
+    x = _inputs(65536, seed=5)
+    target = (np.sin(x[:, 0]) + 0.5 * x[:, 1] - 0.3 * np.log1p(x[:, 5]) * x[:, 6]).astype(np.float32)
+    rows = D.to_dev(np.concatenate([x, target[:, None]], axis=1))
+    net = MLPSurrogate(seed=5)
+    hist = net.fit(rows, epochs=6, batch=8192, lr=3e-3, lr_final=3e-4)
+    assert hist[-1] < 0.25 * hist[0]
+    pred = net(rows[:, :7].contiguous()).cpu().numpy()
+    assert np.mean(np.abs(pred - target)) < 0.2
+    sd = net.state_dict()
+    other = MLPSurrogate(seed=99)
+    other.load_state_dict(sd)
+    assert torch.equal(other(rows[:100, :7].contiguous()), net(rows[:100, :7].contiguous()))

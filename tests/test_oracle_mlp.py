@@ -1,0 +1,76 @@
+"""Pins of the MLP oracle (CPU): PyTorch autograd of the same network and finite differences."""
+
+import numpy as np
+import torch
+
+from oracle import mlp_ref
+
+
+def _params(seed=0):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for o, n in mlp_ref.LAYERS:
+        b = 1 / np.sqrt(n)
+        parts += [rng.uniform(-b, b, o * n), rng.uniform(-b, b, o)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _torch_net(p):
+    net = torch.nn.Sequential(torch.nn.Linear(7, 100), torch.nn.Tanh(), torch.nn.Linear(100, 100), torch.nn.Tanh(),
+                              torch.nn.Linear(100, 120), torch.nn.Tanh(), torch.nn.Linear(120, 1)).double()
+    lin = [m for m in net if isinstance(m, torch.nn.Linear)]
+    with torch.no_grad():
+        for (W, b), m in zip(mlp_ref.unflatten(p), lin):
+            m.weight.copy_(torch.from_numpy(W.astype(np.float64)))
+            m.bias.copy_(torch.from_numpy(b.astype(np.float64)))
+    return net, lin
+
+
+def test_forward_and_backward_match_torch_autograd():
+    p = _params(1)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(64, 7)).astype(np.float32)
+    gy = rng.normal(size=64)
+    y, _ = mlp_ref.forward(p, x)
+    dx, dp = mlp_ref.backward(p, x, gy)
+    net, lin = _torch_net(p)
+    xt = torch.from_numpy(x.astype(np.float64)).requires_grad_(True)
+    yt = net(xt)[:, 0]
+    assert np.max(np.abs(yt.detach().numpy() - y)) < 1e-12
+    (yt * torch.from_numpy(gy)).sum().backward()
+    assert np.max(np.abs(xt.grad.numpy() - dx)) < 1e-12
+    tg = np.concatenate([np.concatenate([m.weight.grad.numpy().reshape(-1), m.bias.grad.numpy()]) for m in lin])
+    assert np.max(np.abs(tg - dp)) < 1e-10
+
+
+def test_backward_matches_central_differences():
+    p = _params(3).astype(np.float64)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(5, 7))
+    gy = np.ones(5)
+    dx, dp = mlp_ref.backward(p.astype(np.float32), x.astype(np.float32), gy)
+    p32 = p.astype(np.float32)
+
+    def f(pp, xx):
+        return mlp_ref.forward(pp, xx)[0].sum()
+
+    h = 1e-3
+    for j in range(7):
+        e = np.zeros_like(x)
+        e[:, j] = h
+        fd = (f(p32, (x + e).astype(np.float32)) - f(p32, (x - e).astype(np.float32))) / (2 * h)
+        assert abs(fd - dx[:, j].sum()) < 1e-3 * max(1, abs(fd))
+    for k in rng.choice(p.size, 12, replace=False):
+        e = np.zeros_like(p32)
+        e[k] = h
+        fd = (f(p32 + e, x.astype(np.float32)) - f(p32 - e, x.astype(np.float32))) / (2 * h)
+        assert abs(fd - dp[k]) < 2e-3 * max(1, abs(fd))
+
+
+def test_bf16_rounding_is_nearest_even():
+    v = np.array([1.0, 1.00390625, 1.005859375, 3.0e-3, -2.5], np.float32)
+    r = mlp_ref.bf16_round(v)
+    assert r[0] == 1.0
+    assert r[1] == 1.0  # halfway between 1.0 and 1.0078125 -> even
+    assert r[2] == 1.0078125
+    assert np.allclose(r, torch.tensor(v).bfloat16().double().numpy())
