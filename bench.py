@@ -68,6 +68,211 @@ def cpu_baseline_leg(theta, ns, target_seconds: float = 15.0):
     }
 
 
+def sample_config3(model: str, n_sets: int = 1024, seed: int = 3):
+    """Config-3 parameter sets: LCA-4 (v_i in [0,2.5], a in [1,3], z_i in [0,0.5], g,b in [-1,1], t in [0,2])
+    or race-4 with a Weibull-collapsing bound (race_4 bounds, alpha in [0.3,5], beta in [0.3,7])."""
+    rng = np.random.default_rng(seed)
+    if model == "lca_4":
+        lo = [0.0] * 4 + [1.0] + [0.0] * 4 + [-1.0, -1.0, 0.0]
+        hi = [2.5] * 4 + [3.0] + [0.5] * 4 + [1.0, 1.0, 2.0]
+    else:
+        lo = [0.0] * 4 + [1.0] + [0.0] * 4 + [0.0, 0.3, 0.3]
+        hi = [2.5] * 4 + [3.0] + [0.9] * 4 + [2.0, 5.0, 7.0]
+    return rng.uniform(lo, hi, size=(n_sets, len(lo))).astype(np.float32)
+
+
+def time_graph(sim, steps: int, warmup: int):
+    import torch
+
+    sim.capture()
+    for _ in range(warmup):
+        sim.replay()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        sim.replay()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def config3_leg(args):
+    """LCA-4 and race-4/Weibull, 1024 sets x 20,000 samples each, against the CPU restatement."""
+    from oracle import cpu_baseline
+    from oracle import em_numpy as E
+    from ssmhip import batched
+
+    out = {}
+    for model, kind in (("lca_4", 2), ("race_4_weibull", 1)):
+        theta = sample_config3(model)
+        sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto")
+        ms = time_graph(sim, max(2, args.steps), 1)
+        steps = sim.steps()
+        rec = {"trajectories": sim.n_total, "ms_per_launch": ms, "trajectories_per_s": sim.n_total / (ms / 1e3),
+               "em_steps_per_s": steps / (ms / 1e3)}
+        if not args.no_cpu:
+            idx = np.arange(0, theta.shape[0], 32)
+            n_cpu = 20000
+            v = theta[idx, 0:4]
+            z = theta[idx, 5:9]
+            a = theta[idx, 4]
+            if model == "lca_4":
+                g, b, t = theta[idx, 9], theta[idx, 10], theta[idx, 11]
+                table = E.boundary_table("constant", {"a": a}, idx.size, 20.0, 0.001)
+            else:
+                g = b = None
+                t = theta[idx, 9]
+                table = E.boundary_table("weibull_cdf", {"a": a, "alpha": theta[idx, 10], "beta": theta[idx, 11]},
+                                         idx.size, 20.0, 0.001)
+            offs = (np.arange(idx.size + 1) * n_cpu).astype(np.int64)
+            dl = np.minimum(np.float32(20.0), (np.float32(999.0) - t).astype(np.float32))
+            t0 = time.perf_counter()
+            cpu_baseline.em_simulate(kind, v, z, np.full((idx.size, 4), E.sqrt_dt32(0.001), np.float32), t,
+                                     np.full(idx.size, 999.0, np.float32), dl, table, offs, dt=0.001, seed=1, g=g,
+                                     b=b, n_threads=cpu_baseline.n_cores())
+            sec = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"value": idx.size * n_cpu / sec, "unit": "trajectories/s",
+                                   "cores": cpu_baseline.n_cores(), "kind": "port",
+                                   "sample": f"{idx.size} of 1024 sets (every 32nd) x {n_cpu} samples in {sec:.2f} s"}
+            rec["gpu_over_cpu"] = rec["trajectories_per_s"] / rec["cpu_baseline"]["value"]
+        out[model] = rec
+        log(f"config3 {model}: {rec['trajectories_per_s']:.3e} traj/s"
+            + (f", {rec['gpu_over_cpu']:.0f}x CPU" if "gpu_over_cpu" in rec else ""))
+    return out
+
+
+def config4_leg(args):
+    """KDE-labelled LAN rows: 1024 angle sets x 1e5 simulations -> 800 KDE + 200 uniform rows per set."""
+    import torch
+
+    from oracle import cpu_baseline, kde_ref
+    from oracle import em_numpy as E
+    from ssmhip import roofline
+    from ssmhip.lan import LanKDELabeller, sample_config4
+
+    theta = sample_config4(1024, seed=1)
+    lab = LanKDELabeller("angle", theta, n_samples=100_000, seed=4, launch="auto")
+    lab.run(timed=True)  # warm-up
+    torch.cuda.synchronize()
+    reps = max(2, min(args.steps, 5))
+    tot = []
+    for _ in range(reps):
+        rows = lab.run(timed=True)
+        tot.append(lab.times)
+    torch.cuda.synchronize()
+    st = min(tot, key=lambda x: x.total_ms)
+    n_rows = int(rows.shape[0])
+    cnt = lab.bank.kde_count.cpu().numpy().astype(np.int64)
+    gcount = lab.points.grp_count.cpu().numpy().astype(np.int64)
+    pairs = int(np.sum(cnt * gcount))
+    rec = {"rows": n_rows, "ms_per_pass": st.total_ms, "rows_per_s": n_rows / (st.total_ms / 1e3),
+           "stage_ms": {"simulate": st.simulate_ms, "kde_fit": st.fit_ms, "kde_sample": st.sample_ms,
+                        "kde_eval": st.eval_ms, "assemble": st.assemble_ms},
+           "kde_evals_per_s": n_rows / (st.eval_ms / 1e3), "kde_pairs": pairs,
+           "kde_pairs_per_s": pairs / (st.eval_ms / 1e3)}
+    if not args.no_probe:
+        roof = roofline.probe_rate("kde_pair", iters=2048)
+        rec["kde_eval_roofline"] = {"bound": "valu (exp)", "achieved": rec["kde_pairs_per_s"] / 1e12,
+                                    "peak": roof / 1e12, "unit": "T pairs/s", "frac": rec["kde_pairs_per_s"] / roof}
+    if not args.no_cpu:
+        # CPU: per set, simulate 1e5 (MT19937 + Ziggurat, all cores) + exact float64 KDE at the same 1000 points
+        pts_rt = lab.points.rt.cpu().numpy().reshape(1024, -1)
+        pts_ch = lab.points.choice.cpu().numpy().reshape(1024, -1)
+        nsets = 2
+        t0 = time.perf_counter()
+        for k in range(nsets):
+            v, a, z, t, th = theta[k]
+            table = E.boundary_table("angle", {"a": [a], "theta": [th]}, 1, 20.0, 0.001)
+            rts, ch, _ = cpu_baseline.em_simulate(0, [v], [z], [E.sqrt_dt32(0.001)], [t], [999.0],
+                                                  [min(20.0, 999.0 - t)], table, np.array([0, 100_000], np.int64),
+                                                  dt=0.001, seed=k, smooth_unif=True,
+                                                  n_threads=cpu_baseline.n_cores())
+            fit = kde_ref.KDEFit(rts, ch, [-1, 1])
+            fit.eval(pts_rt[k].astype(np.float64), np.array([-1, 1])[pts_ch[k]])
+        sec = (time.perf_counter() - t0) / nsets
+        rec["cpu_baseline"] = {"value": lab.rows_per_set / sec, "unit": "rows/s", "cores": cpu_baseline.n_cores(),
+                               "kind": "port",
+                               "sample": f"{nsets} sets: 1e5 simulations + 1000 exact float64 KDE evaluations each, "
+                                         f"{sec:.2f} s per set"}
+        rec["gpu_over_cpu"] = rec["rows_per_s"] / rec["cpu_baseline"]["value"]
+    log(f"config4: {rec['rows_per_s']:.3e} rows/s, KDE {rec['kde_evals_per_s']:.3e} evals/s")
+    return rec, lab, rows
+
+
+def config5_leg(args, lab, rows, kde_evals_per_s):
+    """Surrogate: train on the config-4 rows, then 4M-row inference vs GPU KDE and the analytic density."""
+    import torch
+
+    from oracle import ddm_analytic
+    from ssmhip.surrogate import MLPSurrogate
+
+    rows = rows.contiguous()
+    n = rows.shape[0]
+    perm = torch.randperm(n, device=rows.device, generator=torch.Generator(device=rows.device).manual_seed(0))
+    val = rows.index_select(0, perm[: n // 20])
+    train = rows.index_select(0, perm[n // 20:])
+    net = MLPSurrogate(seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.fit(train, epochs=args.surrogate_epochs, batch=8192, lr=2e-3, lr_final=1e-4, val_rows=val)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    rng = np.random.default_rng(5)
+    N = 4_000_000
+    lo = np.array([-3.0, 0.3, 0.1, 1e-3, -0.1])
+    hi = np.array([3.0, 3.0, 0.9, 2.0, 1.3])
+    x = np.concatenate([rng.uniform(lo, hi, (N, 5)), rng.uniform(0, 20, (N, 1)), rng.choice([-1.0, 1.0], (N, 1))],
+                       axis=1).astype(np.float32)
+    dx = torch.from_numpy(x).cuda()
+    net(dx)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    s.record()
+    for _ in range(reps):
+        net(dx)
+    e.record()
+    e.synchronize()
+    ms = s.elapsed_time(e) / reps
+    evals = N / (ms / 1e3)
+    flops = 2 * (7 * 100 + 100 * 100 + 100 * 120 + 120) * N / (ms / 1e3)
+    # accuracy at theta = 0 (angle == plain DDM): rows drawn from the model itself
+    k = 2000
+    th = rng.uniform(lo[:4], hi[:4], (k, 4)).astype(np.float32)
+    from ssmhip import batched
+
+    sim = batched.BatchedSimulation("ddm", th, 50, seed=9, launch=None)
+    sim.run()
+    rts = sim.rts.cpu().numpy().reshape(k, 50)
+    chs = sim.choices.cpu().numpy().reshape(k, 50)
+    xs, ref = [], []
+    for i in range(k):
+        m = rts[i] > 0
+        v, a, z, t = th[i]
+        ll = ddm_analytic.log_density_rt(rts[i][m], chs[i][m], float(v), float(a), float(z), float(t))
+        ok = np.isfinite(ll)
+        xi = np.zeros((int(ok.sum()), 7), np.float32)
+        xi[:, :4] = th[i]
+        xi[:, 4] = 0.0
+        xi[:, 5] = rts[i][m][ok]
+        xi[:, 6] = chs[i][m][ok]
+        xs.append(xi)
+        ref.append(ll[ok])
+    xs = np.concatenate(xs)
+    ref = np.concatenate(ref)
+    pred = net(torch.from_numpy(xs).cuda()).cpu().numpy()
+    mae = float(np.mean(np.abs(pred - ref)))
+    rec = {"train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
+           "train_rows_per_s": train.shape[0] * args.surrogate_epochs / train_s, "huber_history": hist[-3:],
+           "val_mae_vs_kde": net.val_mae, "inference_rows": N, "ms_per_inference": ms, "evals_per_s": evals,
+           "over_gpu_kde": evals / kde_evals_per_s,
+           "mfma_tflops_algorithmic": flops / 1e12, "mfma_frac_of_bf16_peak": flops / 2.5e15,
+           "mae_vs_analytic_theta0": mae, "analytic_rows": int(ref.size)}
+    log(f"config5: {evals:.3e} evals/s ({rec['over_gpu_kde']:.0f}x GPU KDE), MAE vs analytic {mae:.3f} nats")
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +283,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--launch", type=str, default="auto", help="'auto' or block,blocks_per_cu,refill_min")
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU roof probe")
+    ap.add_argument("--headline-only", action="store_true", help="skip the config-3/4/5 legs")
+    ap.add_argument("--surrogate-epochs", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -175,6 +382,13 @@ def main() -> int:
         cb.pop("calibration_rate", None)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+    if not args.headline_only:
+        sim.close()
+        del sim
+        out["config3"] = config3_leg(args)
+        c4, lab, rows = config4_leg(args)
+        out["config4"] = c4
+        out["config5"] = config5_leg(args, lab, rows, c4["kde_evals_per_s"])
     print(json.dumps(out), flush=True)
     return 0
 

@@ -1,0 +1,99 @@
+"""Host-side logic that needs no GPU: preprocessing, records, configs, strategies."""
+
+import numpy as np
+import pytest
+
+F32 = np.float32
+
+
+def test_records_carry_the_reference_per_trial_float32_quantities():
+    from oracle import em_numpy as E
+    from ssmhip import _lib as L
+    from ssmhip import em
+
+    v, a, z, t = (np.array(x, F32) for x in ([0.5, -1.2], [1.5, 0.7], [0.4, 0.6], [0.3, 1.1]))
+    th = np.array([0.2, -0.05], F32)
+    rec = em.ddm_records(v, a, z, t, np.array([999.0, 1.0], F32), np.ones(2, F32), 20.0, 0.001, L.BOUNDARY_ANGLE,
+                         {"a": a, "theta": th})
+    assert np.array_equal(rec[:, L.P_DEADLINE_TMP], np.minimum(F32(20.0), (np.array([999.0, 1.0], F32) - t)))
+    assert rec[0, L.P_SQRT_ST] == E.sqrt_dt32(0.001)
+    # the slope the kernel uses reproduces NumPy's float32 boundary exactly
+    table = E.boundary_table("angle", {"a": a, "theta": th}, 2, 20.0, 0.001)
+    ix = np.arange(table.shape[1], dtype=F32)
+    for k in range(2):
+        b = (a[k] + (ix * F32(0.001)).astype(F32) * rec[k, L.P_B1]).astype(F32)
+        assert np.array_equal(b, table[k])
+
+
+def test_kernel_boundary_codes():
+    from ssmhip import _lib as L
+    from ssmhip import boundary_functions as BF
+
+    assert BF.kernel_boundary_code(BF.constant) == L.BOUNDARY_CONSTANT
+    assert BF.kernel_boundary_code(BF.angle) == L.BOUNDARY_ANGLE
+    assert BF.kernel_boundary_code(BF.weibull_cdf) == L.BOUNDARY_WEIBULL
+
+    def angle(t, a, theta):  # a user function that merely shares the name
+        return a - t * theta
+
+    assert BF.kernel_boundary_code(angle) == L.BOUNDARY_TABLE
+    assert BF.n_steps(20.0, 0.001) == 20001
+
+
+def test_theta_preprocessing_matches_reference_rules():
+    from ssmhip.simulator import _make_valid_dict, _preprocess_theta, _theta_array_to_dict, make_noise_vec
+
+    d = _make_valid_dict({"v": 1.0, "a": [1.0, 2.0], "z": np.array([0.5, 0.4])})
+    assert all(x.dtype == F32 and x.shape == (2,) for x in d.values())
+    with pytest.raises(ValueError):
+        _make_valid_dict({"v": [1.0, 2.0], "a": [1.0, 2.0, 3.0]})
+    arr = _preprocess_theta([[0.5, 1.0, 0.5, 0.3]])
+    assert arr.dtype == F32
+    with pytest.raises(ValueError):
+        _theta_array_to_dict(arr, ["v", "a", "z"])
+    assert make_noise_vec(1.0, 3, 1).shape == (3,)
+    nv = make_noise_vec(0.1, 3, 4)
+    assert nv.shape == (3, 4) and np.all(nv == F32(0.1))
+
+
+def test_model_configs_and_deadline_variant():
+    from ssmhip import config
+
+    for name in ("ddm", "angle", "weibull", "race_4", "lca_4", "lba2", "lba_angle_3"):
+        c = config.get(name)
+        assert c["n_params"] == len(c["params"]) == len(c["param_bounds"][0]) == len(c["default_params"])
+    c = config.get("angle_deadline")
+    assert c["params"][-1] == "deadline" and c["name"] == "angle_deadline"
+    with pytest.raises(ValueError):
+        config.get("nonexistent_model")
+
+
+def test_batched_records_shape_checks():
+    from ssmhip import batched
+
+    theta, ns = batched.sample_config2(8, seed=0)
+    assert theta.shape == (8, 5) and np.all((ns >= 1000) & (ns <= 100000))
+    with pytest.raises(ValueError):
+        batched.records_for("angle", theta[:, :4], 0.001, 20.0)
+
+
+def test_resample_mixture_strategy_with_a_stub_estimator():
+    from ssmhip.estimators import ResampleMixtureStrategy
+
+    class Stub:
+        def get_metadata(self):
+            return {"possible_choices": [-1, 1], "max_t": 20.0}
+
+        def sample(self, n_samples, random_state=None):
+            return {"rts": np.linspace(0.3, 2, n_samples), "choices": np.ones(n_samples)}
+
+        def evaluate(self, rts, choices):
+            return -np.asarray(rts)
+
+    gen = {"training": {"n_samples_per_param": 1000, "mixture_probabilities": [0.8, 0.1, 0.1],
+                        "separate_response_channels": False, "negative_rt_log_likelihood": -66.77497}}
+    mc = {"params": ["v", "a", "z", "t"], "nchoices": 2}
+    out = ResampleMixtureStrategy(gen, mc).generate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3}, Stub(), random_state=0)
+    assert out.shape == (1000, 7) and out.dtype == F32
+    assert np.sum(out[:, -1] == F32(-66.77497)) == 100
+    assert np.all(out[:, 0] == 1.0)
