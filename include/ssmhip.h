@@ -201,6 +201,8 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
  * params: FP32 master weights, flat nn.Linear layout W1[100][7], b1, W2, b2,
  * W3[120][100], b3, W4[1][120], b4 (ssm_mlp_sizes gives every length).
  * ssm_mlp_pack converts params to bf16 MFMA fragments + padded biases.
+ * xnorm_dev (nullable) = shift[7], scale[7]: the network sees (x - shift) * scale
+ *   (input standardisation); dx is returned with respect to the raw x.
  * ssm_mlp_forward: y[n] = net(x[n][7]).
  * ssm_mlp_backward: forward + backward-data for n rows; mode 0 takes the
  *   upstream gradient dL/dy in target_or_grad, mode 1 the Huber targets
@@ -212,11 +214,12 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
  * ssm_mlp_adam: Adam step `step` (>= 1) on params with moments m, v.      */
 int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
 int32_t ssm_mlp_pack(const float* params_dev, void* frags_dev, float* bias_dev, void* stream);
-int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev, float* y_dev,
-                        void* stream);
+int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev,
+                        const float* xnorm_dev, float* y_dev, void* stream);
 int32_t ssm_mlp_backward(const float* x_dev, const float* target_or_grad_dev, int64_t n, int32_t mode, float delta,
-                         float scale, const void* frags_dev, const float* bias_dev, void* acts_dev, void* deltas_dev,
-                         int64_t ld, float* dx_dev, float* y_dev, float* loss_dev, void* stream);
+                         float scale, const void* frags_dev, const float* bias_dev, const float* xnorm_dev,
+                         void* acts_dev, void* deltas_dev, int64_t ld, float* dx_dev, float* y_dev, float* loss_dev,
+                         void* stream);
 int32_t ssm_mlp_wgrad(const void* acts_dev, const void* deltas_dev, int64_t n, int64_t ld, float* grad_dev,
                       void* stream);
 int32_t ssm_mlp_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, int32_t step, float lr,

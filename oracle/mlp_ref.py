@@ -37,10 +37,19 @@ def bf16_round(a: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False):
+def standardise(x: np.ndarray, xnorm) -> np.ndarray:
+    """(x - shift) * scale in float32, as the kernels apply it before the first layer."""
+    x = np.asarray(x, np.float32)
+    if xnorm is None:
+        return x
+    xn = np.asarray(xnorm, np.float32)
+    return ((x - xn[:7]).astype(np.float32) * xn[7:]).astype(np.float32)
+
+
+def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None):
     """Returns y [n] and the list of hidden activations."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
-    h = r(np.asarray(x, np.float32))
+    h = r(standardise(x, xnorm))
     hs = [h]
     layers = unflatten(np.asarray(p, np.float32))
     for i, (W, b) in enumerate(layers):
@@ -54,13 +63,13 @@ def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False):
     raise AssertionError
 
 
-def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False):
-    """dL/dx [n, 7] and dL/dp (flat) for upstream gradient gy = dL/dy."""
+def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False, xnorm=None):
+    """dL/dx [n, 7] (raw inputs) and dL/dp (flat) for upstream gradient gy = dL/dy."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     layers = unflatten(np.asarray(p, np.float32))
-    _, hs = forward(p, x, bf16)
+    _, hs = forward(p, x, bf16, xnorm)
     # operands the kernels feed to the MFMA
-    ins = [r(np.asarray(x, np.float32))] + [r(h.astype(np.float32)) if bf16 else h for h in hs[1:]]
+    ins = [r(standardise(x, xnorm))] + [r(h.astype(np.float32)) if bf16 else h for h in hs[1:]]
     dz = np.asarray(gy, np.float64)[:, None]
     grads = [None] * 4
     for i in range(3, -1, -1):
@@ -74,6 +83,8 @@ def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False):
         else:
             dx = dh
     flat = np.concatenate([np.concatenate([g[0].reshape(-1), g[1]]) for g in grads])
+    if xnorm is not None:
+        dx = dx * np.asarray(xnorm, np.float64)[7:]
     return dx, flat
 
 

@@ -71,8 +71,8 @@ int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t
 
 int32_t mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
 int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStream_t s);
-int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, float* y,
-                            hipStream_t s);
+int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, const float* xnorm,
+                            float* y, hipStream_t s);
 int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s);
 int32_t mlp_wgrad_enqueue(const __bf16* acts, const __bf16* deltas, int64_t n, int64_t ld, float* grad,
                           hipStream_t s);
@@ -224,16 +224,17 @@ int32_t ssm_mlp_pack(const float* params_dev, void* frags_dev, float* bias_dev, 
   return mlp_pack_enqueue(params_dev, (__bf16*)frags_dev, bias_dev, (hipStream_t)stream);
 }
 
-int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev, float* y_dev,
-                        void* stream) {
-  return mlp_forward_enqueue(x_dev, n, (const __bf16*)frags_dev, bias_dev, y_dev, (hipStream_t)stream);
+int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev,
+                        const float* xnorm_dev, float* y_dev, void* stream) {
+  return mlp_forward_enqueue(x_dev, n, (const __bf16*)frags_dev, bias_dev, xnorm_dev, y_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_mlp_backward(const float* x_dev, const float* target_or_grad_dev, int64_t n, int32_t mode, float delta,
-                         float scale, const void* frags_dev, const float* bias_dev, void* acts_dev, void* deltas_dev,
-                         int64_t ld, float* dx_dev, float* y_dev, float* loss_dev, void* stream) {
+                         float scale, const void* frags_dev, const float* bias_dev, const float* xnorm_dev,
+                         void* acts_dev, void* deltas_dev, int64_t ld, float* dx_dev, float* y_dev, float* loss_dev,
+                         void* stream) {
   if (mode != 0 && mode != 1) return fail(SSM_EINVAL, "mode must be 0 (upstream gradient) or 1 (Huber)");
-  TrainArgs A{x_dev, target_or_grad_dev, (const __bf16*)frags_dev, bias_dev, (__bf16*)acts_dev,
+  TrainArgs A{x_dev, target_or_grad_dev, (const __bf16*)frags_dev, bias_dev, xnorm_dev, (__bf16*)acts_dev,
               (__bf16*)deltas_dev, dx_dev, y_dev, loss_dev, n, ld, mode, delta, scale};
   return mlp_train_enqueue(A, (hipStream_t)stream);
 }

@@ -150,11 +150,14 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         if (lane_id() == leader) base = atomicAdd(A.ws, (unsigned long long)cnt);
         base = readlane_u64(base, leader);
         if (base + (unsigned long long)cnt >= (unsigned long long)A.total) exhausted = true;
+        // set of the wave's first new index (uniform search), then a short per-lane walk forward
+        const int kbase = (base < (unsigned long long)A.total) ? find_set(A.offsets, A.n_sets, (int64_t)base) : 0;
         if (state == kEmpty) {
           const int64_t gi = (int64_t)base + lanes_below(want);
           if (gi < A.total) {
             g = gi;
-            k = find_set(A.offsets, A.n_sets, gi);
+            k = kbase;
+            while (k < A.n_sets - 1 && A.offsets[k + 1] <= gi) ++k;
             n = (int)(gi - A.offsets[k]);
             const float* sp = A.set_params + (int64_t)k * SSM_SET_STRIDE;
             t0 = sp[SSM_P_T0];
@@ -213,20 +216,15 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         bool alive = true;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          if (alive) {
-            bool go = ix < A.n_steps;
-            if (go) {
-              const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-              go = (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
-            }
-            if (go) {
-              x[0] += vdt[0] + sst[0] * nz[i];  // ddm_models.pyx:720
-              t += dt;
-              ++ix;
-            } else {
-              alive = false;
-            }
-          }
+          // predicated step (no branch): a lane that stopped keeps its state
+          const int ixc = ix < A.n_steps ? ix : A.n_steps - 1;  // never read past a boundary table
+          const float b = boundary_at<BOUND>(ixc, dt, a, b1, b2, row);
+          const bool go = alive && (ix < A.n_steps) && (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
+          const float xn = x[0] + (vdt[0] + sst[0] * nz[i]);  // ddm_models.pyx:720
+          x[0] = go ? xn : x[0];
+          t = go ? t + dt : t;
+          ix += go ? 1 : 0;
+          alive = go;
         }
         if (!alive) state = kDone;
       } else {

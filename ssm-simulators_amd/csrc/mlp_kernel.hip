@@ -129,9 +129,11 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 
 // ---- forward (inference) --------------------------------------------------------------
 // x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. Two row tiles (32 rows) per wave.
+// xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
 __global__ void __launch_bounds__(256) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
                                                           const __bf16* __restrict__ frags,
-                                                          const float* __restrict__ bias, float* __restrict__ y) {
+                                                          const float* __restrict__ bias,
+                                                          const float* __restrict__ xnorm, float* __restrict__ y) {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(256) mlp_forward_kernel(const float* __restric
       if (g == 0 && row < n) {
         const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)xr[j];
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)(xnorm ? (xr[j] - xnorm[j]) * xnorm[kIn + j] : xr[j]);
       }
       B[t][0] = v;
     }
@@ -233,7 +235,10 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       bf16x8 v = {};
       if (g == 0 && live) {
 #pragma unroll
-        for (int jj = 0; jj < kIn; ++jj) v[jj] = (__bf16)A.x[row * kIn + jj];
+        for (int jj = 0; jj < kIn; ++jj) {
+          const float xv = A.x[row * kIn + jj];
+          v[jj] = (__bf16)(A.xnorm ? (xv - A.xnorm[jj]) * A.xnorm[kIn + jj] : xv);
+        }
       }
       x0 = v;
       if (A.acts && g == 0 && live) {
@@ -376,7 +381,10 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       // lane holds dx[row][4g + i]; inputs are features 0..6 (groups 0 and 1)
       if (live && g < 2) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) A.dx[row * 8 + 4 * g + i] = c[i];
+        for (int i = 0; i < 4; ++i) {
+          const int k = 4 * g + i;  // d/dx_raw = d/dx_norm * scale
+          A.dx[row * 8 + k] = (A.xnorm && k < kIn) ? c[i] * A.xnorm[kIn + k] : c[i];
+        }
       }
     }
   }
@@ -493,8 +501,8 @@ int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStr
   return SSM_OK;
 }
 
-int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, float* y,
-                            hipStream_t s) {
+int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, const float* bias, const float* xnorm,
+                            float* y, hipStream_t s) {
   if (n < 0) return fail(SSM_EINVAL, "n must be >= 0");
   if (n == 0) return SSM_OK;
   if (!x || !frags || !bias || !y) return fail(SSM_EINVAL, "NULL buffer");
@@ -504,7 +512,7 @@ int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, cons
   int64_t blocks = (waves_needed + 3) / 4;
   const int64_t cap = (int64_t)cus * 16;
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, frags, bias, y);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, frags, bias, xnorm, y);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

@@ -32,6 +32,7 @@ struct TrainArgs {
   const float* target_or_grad;
   const __bf16* frags;
   const float* bias;
+  const float* xnorm;  // optional shift[7], scale[7]: the network sees (x - shift) * scale
   __bf16* acts;
   __bf16* deltas;
   float* dx;

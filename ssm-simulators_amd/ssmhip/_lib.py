@@ -97,10 +97,11 @@ _SIGS = {
     ),
     "ssm_mlp_sizes": (C.c_int32, [C.POINTER(C.c_int32)] * 5),
     "ssm_mlp_pack": (C.c_int32, [_VP, _VP, _VP, _VP]),
-    "ssm_mlp_forward": (C.c_int32, [_VP, C.c_int64, _VP, _VP, _VP, _VP]),
+    "ssm_mlp_forward": (C.c_int32, [_VP, C.c_int64, _VP, _VP, _VP, _VP, _VP]),
     "ssm_mlp_backward": (
         C.c_int32,
-        [_VP, _VP, C.c_int64, C.c_int32, C.c_float, C.c_float, _VP, _VP, _VP, _VP, C.c_int64, _VP, _VP, _VP, _VP],
+        [_VP, _VP, C.c_int64, C.c_int32, C.c_float, C.c_float, _VP, _VP, _VP, _VP, _VP, C.c_int64, _VP, _VP, _VP,
+         _VP],
     ),
     "ssm_mlp_wgrad": (C.c_int32, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_mlp_adam": (

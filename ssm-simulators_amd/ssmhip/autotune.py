@@ -21,8 +21,8 @@ from . import _device as D
 from .em import LaunchParams
 
 DEFAULT_CANDIDATES = [
-    (256, 4, 8), (256, 8, 16), (256, 8, 32), (512, 4, 16), (512, 4, 32),
-    (256, 16, 16), (1024, 2, 16), (128, 16, 16), (256, 8, 48), (512, 8, 16),
+    (256, 8, 8), (256, 8, 16), (256, 8, 32), (512, 4, 4), (512, 4, 8), (512, 4, 16), (512, 4, 32),
+    (1024, 2, 8), (1024, 2, 16), (128, 16, 8), (512, 8, 8), (256, 16, 8),
 ]
 
 _memo: dict[str, LaunchParams] = {}

@@ -117,16 +117,24 @@ class PointBatch:
         self.rt, self.choice, self.grp_begin, self.grp_count, self.m = rt, choice, grp_begin, grp_count, m
 
     @classmethod
-    def from_host(cls, rts: np.ndarray, choice_idx: np.ndarray, n_sets: int, C: int):
-        """Points of ONE set (n_sets must be 1) from host arrays; returns (batch, permutation)."""
-        if n_sets != 1:
-            raise ValueError("from_host builds one-set batches")
+    def from_host(cls, rts: np.ndarray, choice_idx: np.ndarray, n_sets: int, C: int, set_idx=None):
+        """Points from host arrays, grouped on the host by (set, choice); returns (batch, order).
+
+        `order[j]` is the input position of the j-th device point, so
+        ``out[order] = ll`` restores input order. With n_sets == 1, set_idx may be omitted.
+        """
         rts = np.asarray(rts, np.float32).reshape(-1)
         ci = np.asarray(choice_idx, np.int64).reshape(-1)
-        order = np.argsort(ci, kind="stable")
-        counts = np.bincount(ci, minlength=C).astype(np.int32)
+        si = np.zeros_like(ci) if set_idx is None else np.asarray(set_idx, np.int64).reshape(-1)
+        if set_idx is None and n_sets != 1:
+            raise ValueError("set_idx is required when n_sets > 1")
+        if ci.size and (ci.min() < 0 or ci.max() >= C or si.min() < 0 or si.max() >= n_sets):
+            raise ValueError("choice or set index out of range")
+        g = si * C + ci
+        order = np.argsort(g, kind="stable")
+        counts = np.bincount(g, minlength=n_sets * C).astype(np.int32)
         begins = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
-        t = D.require_gpu()
+        D.require_gpu()
         return cls(D.to_dev(rts[order]), D.to_dev(ci[order].astype(np.int8)), D.to_dev(begins), D.to_dev(counts),
                    rts.shape[0]), order
 

@@ -60,6 +60,7 @@ class _Kernels:
         self.sz = sizes()
         self.frags = D.empty((self.sz["n_frag"],), t.bfloat16)
         self.bias = D.empty((self.sz["n_bias"],), t.float32)
+        self.xnorm = None  # device float32[14]: shift[7], scale[7], or None
 
     def pack(self, params) -> None:
         L.check(L.load().ssm_mlp_pack(D.ptr(params), D.ptr(self.frags), D.ptr(self.bias), D.stream_ptr()),
@@ -69,8 +70,8 @@ class _Kernels:
         t = D.torch()
         n = x.shape[0]
         y = D.empty((max(n, 1),), t.float32)
-        L.check(L.load().ssm_mlp_forward(D.ptr(x), n, D.ptr(self.frags), D.ptr(self.bias), D.ptr(y),
-                                         D.stream_ptr()), "ssm_mlp_forward")
+        L.check(L.load().ssm_mlp_forward(D.ptr(x), n, D.ptr(self.frags), D.ptr(self.bias), D.ptr(self.xnorm),
+                                         D.ptr(y), D.stream_ptr()), "ssm_mlp_forward")
         return y[:n]
 
     def backward(self, x, target_or_grad, mode: int, *, delta=1.0, scale=1.0, want_dx=True, want_w=True):
@@ -83,7 +84,8 @@ class _Kernels:
         y = D.empty((n,), t.float32)
         loss = D.empty((n,), t.float32) if mode == 1 else None
         rc = L.load().ssm_mlp_backward(D.ptr(x), D.ptr(target_or_grad), n, mode, float(delta), float(scale),
-                                       D.ptr(self.frags), D.ptr(self.bias), D.ptr(acts), D.ptr(deltas), ld,
+                                       D.ptr(self.frags), D.ptr(self.bias), D.ptr(self.xnorm), D.ptr(acts),
+                                       D.ptr(deltas), ld,
                                        D.ptr(dx), D.ptr(y), D.ptr(loss), D.stream_ptr())
         L.check(rc, "ssm_mlp_backward")
         grad = None
@@ -181,6 +183,8 @@ class MLPSurrogate:
         t = D.torch()
         rows = rows.contiguous()
         n = rows.shape[0]
+        if self.kern.xnorm is None:
+            self.set_input_normalisation(rows[:, :IN])
         steps_per_epoch = max(1, n // batch)
         total = epochs * steps_per_epoch
         gen = t.Generator(device=rows.device)
@@ -210,9 +214,18 @@ class MLPSurrogate:
             self.val_mae = float((pred - val_rows[:, IN]).abs().mean().item())
         return history
 
+    def set_input_normalisation(self, x) -> None:
+        """Standardise the inputs: the network sees (x - mean) / std (std floored at 1e-6)."""
+        t = D.torch()
+        mu = x.mean(0)
+        sd = x.std(0).clamp_min(1e-6)
+        self.kern.xnorm = t.cat([mu, 1.0 / sd]).float().contiguous()
+
     # ---- checkpoint ----------------------------------------------------------
     def state_dict(self) -> dict:
         d = {"params": self.params.detach().cpu(), "adam_m": self.m.cpu(), "adam_v": self.v.cpu(), "step": self.step}
+        if self.kern.xnorm is not None:
+            d["xnorm"] = self.kern.xnorm.cpu()
         if self.bounds is not None:
             d["bounds_lo"], d["bounds_hi"] = self.bounds
         return d
@@ -222,6 +235,8 @@ class MLPSurrogate:
         self.m.copy_(d["adam_m"].to(self.m.device))
         self.v.copy_(d["adam_v"].to(self.v.device))
         self.step = int(d["step"])
+        if "xnorm" in d:
+            self.kern.xnorm = d["xnorm"].to(self.params.device).float().contiguous()
         if "bounds_lo" in d:
             self.bounds = (np.asarray(d["bounds_lo"]), np.asarray(d["bounds_hi"]))
         self.kern.pack(self.params)
@@ -229,8 +244,22 @@ class MLPSurrogate:
     def torch_reference(self):
         """The same network as a torch.nn.Sequential (float32) with the current master weights."""
         torch = D.torch()
-        net = torch.nn.Sequential(torch.nn.Linear(IN, H1), torch.nn.Tanh(), torch.nn.Linear(H1, H2), torch.nn.Tanh(),
-                                  torch.nn.Linear(H2, H3), torch.nn.Tanh(), torch.nn.Linear(H3, OUT))
+        layers = []
+        if self.kern.xnorm is not None:
+            xn = self.kern.xnorm.detach().cpu()
+
+            class Standardise(torch.nn.Module):
+                def __init__(self):
+                    super().__init__()
+                    self.register_buffer("shift", xn[:IN].clone())
+                    self.register_buffer("scale", xn[IN:].clone())
+
+                def forward(self, x):
+                    return (x - self.shift) * self.scale
+
+            layers.append(Standardise())
+        net = torch.nn.Sequential(*layers, torch.nn.Linear(IN, H1), torch.nn.Tanh(), torch.nn.Linear(H1, H2),
+                                  torch.nn.Tanh(), torch.nn.Linear(H2, H3), torch.nn.Tanh(), torch.nn.Linear(H3, OUT))
         p = self.params.detach().cpu()
         lin = [m for m in net if isinstance(m, torch.nn.Linear)]
         sl = param_slices()

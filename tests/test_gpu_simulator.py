@@ -1,0 +1,125 @@
+"""The Simulator / cssm drop-ins on the GPU: output contract, preprocessing paths, errors.
+
+Mirrors the reference's API tests (tests/test_simulator.py, tests/test_simulator_class.py):
+shapes, dtypes, metadata, same-seed determinism, input formats, deadline handling.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ddm_output_contract(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    out = Simulator("ddm").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.2}, n_samples=1000, random_state=0)
+    assert out["rts"].shape == (1000, 1) and out["rts"].dtype == np.float32
+    assert out["choices"].shape == (1000, 1) and out["choices"].dtype == np.intc
+    md = out["metadata"]
+    assert md["model"] == "ddm" and md["simulator"] == "ddm_flexbound" and md["possible_choices"] == [-1, 1]
+    assert md["boundary"].shape == (20001,) and np.all(md["boundary"] == np.float32(1.5))
+    assert set(np.unique(out["choices"])) <= {-1, 1}
+    assert np.all(out["rts"] > 0.2)
+
+
+def test_input_formats_and_squeeze(torch_gpu):
+    import pandas as pd
+
+    from ssmhip.simulator import Simulator
+
+    sim = Simulator("angle")
+    theta = {"v": [0.5, -0.5, 1.0], "a": 1.2, "z": 0.5, "t": 0.3, "theta": [0.1, 0.2, 0.3]}
+    a = sim.simulate(theta, n_samples=200, random_state=1)
+    assert a["rts"].shape == (200, 3, 1)
+    arr = np.array([[0.5, 1.2, 0.5, 0.3, 0.1], [-0.5, 1.2, 0.5, 0.3, 0.2], [1.0, 1.2, 0.5, 0.3, 0.3]])
+    b = sim.simulate(arr, n_samples=200, random_state=1)
+    c = sim.simulate(pd.DataFrame(arr, columns=["v", "a", "z", "t", "theta"]), n_samples=200, random_state=1)
+    assert np.array_equal(a["rts"], b["rts"]) and np.array_equal(a["rts"], c["rts"])
+    one = sim.simulate(arr, n_samples=1, random_state=2)
+    assert one["rts"].shape == (3, 1)  # n_trials > 1 and n_samples == 1 squeezes axis 0
+
+
+def test_reproducibility_and_seed_validation(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    sim = Simulator("weibull")
+    th = {"v": 0.5, "a": 1.5, "z": 0.5, "t": 0.3, "alpha": 2.0, "beta": 3.0}
+    r1 = sim.simulate(th, n_samples=500, random_state=42)
+    r2 = sim.simulate(th, n_samples=500, random_state=42)
+    r3 = sim.simulate(th, n_samples=500, random_state=43)
+    assert np.array_equal(r1["rts"], r2["rts"]) and np.array_equal(r1["choices"], r2["choices"])
+    assert not np.array_equal(r1["rts"], r3["rts"])
+    with pytest.raises(ValueError):
+        sim.simulate(th, n_samples=10, random_state=2**31)
+
+
+def test_deadline_variant_marks_omissions(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    sim = Simulator("ddm_deadline")
+    out = sim.simulate({"v": 0.0, "a": 2.5, "z": 0.5, "t": 0.3, "deadline": 1.0}, n_samples=4000, random_state=0)
+    rts = out["rts"].reshape(-1)
+    assert np.any(rts == -999.0)
+    assert np.all((rts == -999.0) | (rts < 1.0))
+
+
+def test_multi_accumulator_models(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    r = Simulator("race_4").simulate({"v0": 1.0, "v1": 0.5, "v2": 0.5, "v3": 0.2, "a": 2.0, "z0": 0.1, "z1": 0.1,
+                                      "z2": 0.1, "z3": 0.1, "t": 0.3}, n_samples=3000, random_state=0)
+    assert set(np.unique(r["choices"])) <= {0, 1, 2, 3}
+    assert np.mean(r["choices"] == 0) > 0.4
+    assert r["metadata"]["possible_choices"] == [0, 1, 2, 3]
+    l = Simulator("lca_4").simulate({"v0": 1.0, "v1": 0.5, "v2": 0.5, "v3": 0.2, "a": 2.0, "z0": 0.1, "z1": 0.1,
+                                     "z2": 0.1, "z3": 0.1, "g": 0.2, "b": 0.2, "t": 0.3}, n_samples=3000,
+                                    random_state=0)
+    assert np.mean(l["choices"] == 0) > 0.4
+    lba = Simulator("lba3").simulate({"A": 0.3, "b": 0.6, "v0": 0.9, "v1": 0.5, "v2": 0.3}, n_samples=3000,
+                                     random_state=0)
+    assert lba["rts"].shape == (3000, 1) and np.mean(lba["choices"] == 0) > 0.5
+    with pytest.raises(ValueError):
+        Simulator("lba3").simulate({"A": 0.7, "b": 0.6, "v0": 0.9, "v1": 0.5, "v2": 0.3}, n_samples=10)
+
+
+def test_cssm_errors_and_custom_boundary(torch_gpu):
+    from ssmhip import cssm
+    from ssmhip.simulator import Simulator
+
+    f32 = lambda x: np.array([x], np.float32)  # noqa: E731
+    with pytest.raises(ValueError, match="n_threads"):
+        cssm.ddm_flexbound(f32(0.5), f32(1.0), f32(0.5), f32(0.3), f32(999.0), f32(1.0), n_threads=300)
+    with pytest.raises(ValueError, match="return_option"):
+        cssm.ddm_flexbound(f32(0.5), f32(1.0), f32(0.5), f32(0.3), f32(999.0), f32(1.0), return_option="x")
+
+    def linear_collapse(t, a, slope):
+        return a - slope * t
+
+    sim = Simulator("ddm", boundary=linear_collapse, boundary_params=["a", "slope"], params=["v", "a", "z", "t",
+                                                                                              "slope"])
+    out = sim.simulate({"v": 0.5, "a": 1.5, "z": 0.5, "t": 0.3, "slope": 0.2}, n_samples=2000, random_state=0)
+    assert out["metadata"]["boundary_fun_type"] == "linear_collapse"
+    # a collapsing bound shortens decisions relative to the constant bound
+    base = Simulator("ddm").simulate({"v": 0.5, "a": 1.5, "z": 0.5, "t": 0.3}, n_samples=2000, random_state=0)
+    assert np.mean(out["rts"]) < np.mean(base["rts"])
+
+
+def test_graph_replay_matches_eager_launch(torch_gpu):
+    import torch
+
+    from ssmhip import batched
+
+    theta, ns = batched.sample_config2(64, seed=3, lo=100, hi=2000)
+    sim = batched.BatchedSimulation("angle", theta, ns, seed=7, launch=None)
+    sim.run()
+    torch.cuda.synchronize()
+    eager = (sim.rts.cpu().numpy().copy(), sim.choices.cpu().numpy().copy())
+    for _ in range(3):
+        sim.rts.fill_(0)
+        sim.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(sim.rts.cpu().numpy(), eager[0])
+        assert np.array_equal(sim.choices.cpu().numpy(), eager[1])
+        torch.zeros(1000, device="cuda").sum()  # unrelated eager work between replays
+    sim.close()
