@@ -331,7 +331,9 @@ def main() -> int:
     theta, ns = batched.sample_config2(args.n_sets, seed=0)
     log(f"config2: {args.n_sets} sets, {int(ns.sum())} trajectories")
     launch = "auto" if args.launch == "auto" else LaunchParams(*[int(x) for x in args.launch.split(",")])
-    sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch=launch)
+    # the headline batch tunes its launch shape on itself (~3 s), not on a sample
+    sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch=launch,
+                                    tune_sample=None)
     lp = sim.launch_params
     log(f"autotuned launch: block={lp.block_size} blocks/CU={lp.blocks_per_cu} refill_min={lp.refill_min}")
     sim.capture()

@@ -180,11 +180,16 @@ int32_t ssm_kde_sample(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t
  * with sklearn KernelDensity.score_samples: for the grp_count[g] points at
  * pts_rt_dev[grp_begin[g]...] of group g writes out_ll_dev (same indexing):
  * log p_c + log mean_i N(log max(rt, eps); x_i, h) - log max(rt, eps), floored
- * at lb (also for rt <= 0); groups without base data get -log(N * max_t).     */
+ * at lb (also for rt <= 0); groups without base data get -log(N * max_t).
+ * max_samples bounds every set's size (offsets[k+1] - offsets[k]);
+ * workspace_dev: float32[n_points * ceil(max_samples / SSM_KDE_CHUNK)] where
+ * n_points covers every point index of the group table.                    */
+#define SSM_KDE_CHUNK 8192
 int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev, const int32_t* grp_count_dev,
                      const float* pts_rt_dev, const float* samples_dev, const int32_t* kde_count_dev,
                      const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
-                     const int64_t* offsets_dev, float max_t, float lb, float eps, float* out_ll_dev, void* stream);
+                     const int64_t* offsets_dev, float max_t, float lb, float eps, int64_t max_samples,
+                     float* workspace_dev, float* out_ll_dev, void* stream);
 
 /* ssm_lan_assemble replaces the row assembly and shuffle of
  * ResampleMixtureStrategy.generate (resample_mixture_strategy.py:129-214):

@@ -5,7 +5,9 @@ layout, flat parameter vector [W1, b1, W2, b2, W3, b3, W4, b4].
 `bf16=True` rounds every GEMM operand (inputs, weights, hidden activations,
 backward deltas) to bfloat16 (round to nearest even) and accumulates in
 float64, i.e. the arithmetic the MFMA kernels perform up to summation order;
-the tanh derivative 1 - h^2 then uses the bf16 activations, as the kernels do.
+the tanh derivative 1 - h^2 then uses the bf16 activations, as the kernels do,
+and hidden-layer forward weights are rounded as bf16(2 log2(e) W) (the scale
+the kernels fold into their packed weights).
 Pinned by: PyTorch autograd of the same nn.Sequential (float64) and central
 finite differences (tests/test_oracle_mlp.py).
 """
@@ -15,6 +17,7 @@ from __future__ import annotations
 import numpy as np
 
 LAYERS = [(100, 7), (100, 100), (120, 100), (1, 120)]
+TANH_SCALE = 2.8853900817779268  # 2 log2(e)
 
 
 def unflatten(p: np.ndarray):
@@ -53,7 +56,13 @@ def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None):
     hs = [h]
     layers = unflatten(np.asarray(p, np.float32))
     for i, (W, b) in enumerate(layers):
-        z = h @ r(W).T + np.asarray(b, np.float64)
+        if bf16 and i < 3:
+            # hidden-layer weights are stored as bf16(2 log2(e) W): the forward kernels fold
+            # the tanh argument scale into the packed weights
+            Wr = bf16_round((np.float32(TANH_SCALE) * W).astype(np.float32)) / TANH_SCALE
+        else:
+            Wr = r(W)
+        z = h @ Wr.T + np.asarray(b, np.float64)
         if i < 3:
             a = np.tanh(z)
             hs.append(a)

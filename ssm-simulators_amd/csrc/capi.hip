@@ -63,8 +63,8 @@ int32_t kde_sample_enqueue(int32_t n_sets, int32_t n_choices, int32_t n_kde, int
 int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin, const int32_t* grp_count,
                          const float* pts_rt, const float* samples, const int32_t* kde_count,
                          const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
-                         const int64_t* offsets, float max_t, float lb, float eps, float* out_ll,
-                         hipStream_t stream);
+                         const int64_t* offsets, float max_t, float lb, float eps, int64_t max_samples,
+                         float* workspace, float* out_ll, hipStream_t stream);
 int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
                              const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
                              const int32_t* choice_values, float* rows, hipStream_t stream);
@@ -130,7 +130,7 @@ int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config*
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) {
     delete h;
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
     return hip_fail(e, "hipStreamBeginCapture");
   }
   rc = em_enqueue(prob, launch, set_params_dev, offsets_dev, table_dev, replay_normals_dev, replay_uniforms_dev,
@@ -141,10 +141,10 @@ int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config*
     e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
   }
-  hipStreamDestroy(s);
+  (void)hipStreamDestroy(s);
   if (rc != SSM_OK) {
-    if (h->exec) hipGraphExecDestroy(h->exec);
-    if (h->graph) hipGraphDestroy(h->graph);
+    if (h->exec) (void)hipGraphExecDestroy(h->exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
     delete h;
     return rc;
   }
@@ -162,8 +162,8 @@ int32_t ssm_graph_launch(void* graph, void* stream) {
 int32_t ssm_graph_destroy(void* graph) {
   if (!graph) return SSM_OK;
   GraphHandle* h = (GraphHandle*)graph;
-  if (h->exec) hipGraphExecDestroy(h->exec);
-  if (h->graph) hipGraphDestroy(h->graph);
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   delete h;
   return SSM_OK;
 }
@@ -202,10 +202,11 @@ int32_t ssm_kde_sample(int32_t n_sets, int32_t n_choices, int32_t n_kde, int32_t
 int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev, const int32_t* grp_count_dev,
                      const float* pts_rt_dev, const float* samples_dev, const int32_t* kde_count_dev,
                      const int32_t* choice_total_dev, const float* bandwidth_dev, const int64_t* kde_start_dev,
-                     const int64_t* offsets_dev, float max_t, float lb, float eps, float* out_ll_dev, void* stream) {
+                     const int64_t* offsets_dev, float max_t, float lb, float eps, int64_t max_samples,
+                     float* workspace_dev, float* out_ll_dev, void* stream) {
   return kde_eval_enqueue(n_sets, n_choices, grp_begin_dev, grp_count_dev, pts_rt_dev, samples_dev, kde_count_dev,
-                          choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, max_t, lb, eps, out_ll_dev,
-                          (hipStream_t)stream);
+                          choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, max_t, lb, eps, max_samples,
+                          workspace_dev, out_ll_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,

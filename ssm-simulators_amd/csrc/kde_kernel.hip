@@ -262,8 +262,10 @@ struct EvalArgs {
   const float* bandwidth;
   const int64_t* kde_start;
   const int64_t* offsets;
+  float* partial;  // [n_points][n_chunks] scaled partial sums
   float* out_ll;
   int32_t n_choices;
+  int32_t n_chunks;
   float max_t;
   float lb;
   float eps;
@@ -273,40 +275,34 @@ constexpr int kEvalThreads = 256;
 constexpr int kPtsPerThread = 4;
 constexpr int kTile = 2048;
 
-__global__ void __launch_bounds__(kEvalThreads) kde_eval_kernel(EvalArgs A) {
+// Pass 1: block (group g, chunk c) sums the kernel over samples
+// [c * SSM_KDE_CHUNK, (c + 1) * SSM_KDE_CHUNK) of group g for every point of g.
+// Splitting groups along their samples balances the very unequal
+// (points x samples) work of the groups across the chip.
+__global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs A) {
   __shared__ __attribute__((aligned(16))) float tile[kTile];
   const int g = blockIdx.x;
-  const int set = g / A.n_choices;
+  const int chunk = blockIdx.y;
   const int cnt = A.grp_count[g];
-  const int64_t pb = A.grp_begin[g];
-  if (cnt <= 0) return;
-  const int64_t N = A.offsets[set + 1] - A.offsets[set];
   const int nc = A.kde_count[g];
   const float h = A.bandwidth[g];
-  if (nc == 0 || !(h > 0.f)) {
-    // choice without base data: -log(N * max_t) (kde_class.py:291-296), no floor
-    const float v = -logf((float)N * A.max_t);
-    for (int i = threadIdx.x; i < cnt; i += kEvalThreads) A.out_ll[pb + i] = v;
-    return;
-  }
-  const float p = (float)A.choice_total[g] / (float)N;
-  // log p - log n - log h - log(2 pi)/2 - 64 ln 2
-  const float cst = logf(p) - logf((float)nc) - logf(h) - 0.91893853320467274f - kScaleLog2 * 0.69314718055994531f;
+  const int s0 = chunk * SSM_KDE_CHUNK;
+  if (cnt <= 0 || nc == 0 || !(h > 0.f) || s0 >= nc) return;
+  const int s1 = (s0 + SSM_KDE_CHUNK < nc) ? s0 + SSM_KDE_CHUNK : nc;
+  const int64_t pb = A.grp_begin[g];
   const float nc2 = -1.4426950408889634f / (2.0f * h * h);  // -log2(e) / (2 h^2)
   const float* smp = A.samples + A.kde_start[g];
   for (int pbase = 0; pbase < cnt; pbase += kEvalThreads * kPtsPerThread) {
     float x[kPtsPerThread], acc[kPtsPerThread];
-    bool valid[kPtsPerThread];
 #pragma unroll
     for (int q = 0; q < kPtsPerThread; ++q) {
       const int pi = pbase + q * kEvalThreads + threadIdx.x;
-      valid[q] = pi < cnt;
-      const float rt = valid[q] ? A.pts_rt[pb + pi] : 1.f;
+      const float rt = pi < cnt ? A.pts_rt[pb + pi] : 1.f;
       x[q] = logf(fmaxf(rt, A.eps));
       acc[q] = 0.f;
     }
-    for (int t0 = 0; t0 < nc; t0 += kTile) {
-      const int tn = (nc - t0) < kTile ? (nc - t0) : kTile;
+    for (int t0 = s0; t0 < s1; t0 += kTile) {
+      const int tn = (s1 - t0) < kTile ? (s1 - t0) : kTile;
       __syncthreads();
       for (int i = threadIdx.x; i < kTile; i += kEvalThreads) tile[i] = i < tn ? smp[t0 + i] : 1e30f;
       __syncthreads();
@@ -326,12 +322,40 @@ __global__ void __launch_bounds__(kEvalThreads) kde_eval_kernel(EvalArgs A) {
 #pragma unroll
     for (int q = 0; q < kPtsPerThread; ++q) {
       const int pi = pbase + q * kEvalThreads + threadIdx.x;
-      if (!valid[q]) continue;
-      const float rt = A.pts_rt[pb + pi];
-      float ll = cst + logf(acc[q]) - x[q];  // acc == 0 -> -inf -> floored
-      if (!(ll > A.lb) || !(rt > 0.f)) ll = A.lb;
-      A.out_ll[pb + pi] = ll;
+      if (pi < cnt) A.partial[(pb + pi) * A.n_chunks + chunk] = acc[q];
     }
+  }
+}
+
+// Pass 2: per group, sum the chunks in a fixed order (deterministic) and finish l.
+__global__ void __launch_bounds__(kEvalThreads) kde_eval_finish_kernel(EvalArgs A) {
+  const int g = blockIdx.x;
+  const int set = g / A.n_choices;
+  const int cnt = A.grp_count[g];
+  if (cnt <= 0) return;
+  const int64_t pb = A.grp_begin[g];
+  const int64_t N = A.offsets[set + 1] - A.offsets[set];
+  const int nc = A.kde_count[g];
+  const float h = A.bandwidth[g];
+  if (nc == 0 || !(h > 0.f)) {
+    // choice without base data: -log(N * max_t) (kde_class.py:291-296), no floor
+    const float v = -logf((float)N * A.max_t);
+    for (int i = threadIdx.x; i < cnt; i += kEvalThreads) A.out_ll[pb + i] = v;
+    return;
+  }
+  int nch = (nc + SSM_KDE_CHUNK - 1) / SSM_KDE_CHUNK;
+  if (nch > A.n_chunks) nch = A.n_chunks;  // stay inside the workspace even if max_samples was too small
+  const float p = (float)A.choice_total[g] / (float)N;
+  // log p - log n - log h - log(2 pi)/2 - 64 ln 2
+  const float cst = logf(p) - logf((float)nc) - logf(h) - 0.91893853320467274f - kScaleLog2 * 0.69314718055994531f;
+  for (int i = threadIdx.x; i < cnt; i += kEvalThreads) {
+    const float rt = A.pts_rt[pb + i];
+    const float x = logf(fmaxf(rt, A.eps));
+    float sum = 0.f;
+    for (int c = 0; c < nch; ++c) sum += A.partial[(pb + i) * A.n_chunks + c];
+    float ll = cst + logf(sum) - x;  // sum == 0 -> -inf -> floored
+    if (!(ll > A.lb) || !(rt > 0.f)) ll = A.lb;
+    A.out_ll[pb + i] = ll;
   }
 }
 
@@ -434,17 +458,24 @@ int32_t kde_sample_enqueue(int32_t n_sets, int32_t n_choices, int32_t n_kde, int
 int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin, const int32_t* grp_count,
                          const float* pts_rt, const float* samples, const int32_t* kde_count,
                          const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
-                         const int64_t* offsets, float max_t, float lb, float eps, float* out_ll,
-                         hipStream_t stream) {
-  if (n_sets < 0) return fail(SSM_EINVAL, "n_sets must be >= 0");
+                         const int64_t* offsets, float max_t, float lb, float eps, int64_t max_samples,
+                         float* workspace, float* out_ll, hipStream_t stream) {
+  if (n_sets < 0 || max_samples < 0) return fail(SSM_EINVAL, "negative sizes");
   if (n_choices < 1 || n_choices > SSM_MAX_PARTICLES) return fail(SSM_EINVAL, "n_choices must be in [1, 16]");
   if (n_sets == 0) return SSM_OK;
   if (!grp_begin || !grp_count || !pts_rt || !samples || !kde_count || !choice_total || !bandwidth || !kde_start ||
-      !offsets || !out_ll)
+      !offsets || !out_ll || !workspace)
     return fail(SSM_EINVAL, "NULL buffer");
-  EvalArgs A{pts_rt, grp_begin, grp_count, samples, kde_count, choice_total, bandwidth, kde_start, offsets, out_ll,
-             n_choices, max_t, lb, eps};
-  hipLaunchKernelGGL(kde_eval_kernel, dim3(n_sets * n_choices), dim3(kEvalThreads), 0, stream, A);
+  const int64_t nch = (max_samples + SSM_KDE_CHUNK - 1) / SSM_KDE_CHUNK;
+  if (nch > 65535) return fail(SSM_EINVAL, "max_samples too large");
+  EvalArgs A{pts_rt,   grp_begin, grp_count, samples,  kde_count,         choice_total, bandwidth, kde_start,
+             offsets,  workspace, out_ll,    n_choices, (int32_t)(nch > 0 ? nch : 1), max_t, lb, eps};
+  if (nch > 0) {
+    hipLaunchKernelGGL(kde_eval_partial_kernel, dim3(n_sets * n_choices, (unsigned)nch), dim3(kEvalThreads), 0,
+                       stream, A);
+    SSM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(kde_eval_finish_kernel, dim3(n_sets * n_choices), dim3(kEvalThreads), 0, stream, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

@@ -46,10 +46,12 @@ constexpr int kDeltaRows = 128 * 3 + 16;  // dZ1, dZ2, dZ3, dZ4
 
 __device__ __forceinline__ int chain_feature(int b, int g, int j) { return 16 * (2 * b + (j >> 2)) + 4 * g + (j & 3); }
 
-__device__ __forceinline__ float tanh_fast(float x) {
-  // tanh x = 1 - 2 / (exp(2x) + 1); exp via v_exp_f32 (2^y)
-  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// Hidden-layer weights and biases are packed pre-multiplied by 2 log2(e), so the
+// accumulator holds zs = 2 log2(e) z and tanh z = 1 - 2 / (2^zs + 1):
+// v_exp_f32, add, v_rcp_f32, fma.
+constexpr float kTanhScale = 2.8853900817779268f;
+__device__ __forceinline__ float tanh_fast(float zs) {
+  return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(zs) + 1.0f), 1.0f);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -110,7 +112,7 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
     if (!bwd) {
       // A[m = out][k]: k from raw inputs (layer 1) or the previous layer's tiles
       const int kin = (layer == 1) ? (8 * g + j) : chain_feature(b, g, j);
-      v = w_at(P, layer, m, kin);
+      v = w_at(P, layer, m, kin) * (layer < 4 ? kTanhScale : 1.0f);
     } else {
       // A[m = in][k = out]: k from this layer's output tiles
       v = w_at(P, layer, chain_feature(b, g, j), m);
@@ -119,9 +121,9 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
   }
   if (e < kNBias) {
     float v = 0.f;
-    if (e < pB2) v = e < kH1 ? P[oB1 + e] : 0.f;
-    else if (e < pB3) v = (e - pB2) < kH2 ? P[oB2 + e - pB2] : 0.f;
-    else if (e < pB4) v = (e - pB3) < kH3 ? P[oB3 + e - pB3] : 0.f;
+    if (e < pB2) v = e < kH1 ? kTanhScale * P[oB1 + e] : 0.f;
+    else if (e < pB3) v = (e - pB2) < kH2 ? kTanhScale * P[oB2 + e - pB2] : 0.f;
+    else if (e < pB4) v = (e - pB3) < kH3 ? kTanhScale * P[oB3 + e - pB3] : 0.f;
     else v = (e - pB4) < kOut ? P[oB4 + e - pB4] : 0.f;
     bias[e] = v;
   }
@@ -130,15 +132,30 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 // ---- forward (inference) --------------------------------------------------------------
 // x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. Two row tiles (32 rows) per wave.
 // xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
-__global__ void __launch_bounds__(256) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
-                                                          const __bf16* __restrict__ frags,
+// The forward fragments (78 KB of bf16) are staged once per block in LDS, so the
+// per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
+constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
+
+__global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
+                                                          const __bf16* __restrict__ gfrags,
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ xnorm, float* __restrict__ y) {
+  __shared__ __attribute__((aligned(16))) __bf16 lfrag[kFwdFrag];
+  {
+    const int4* src = reinterpret_cast<const int4*>(gfrags);
+    int4* dst = reinterpret_cast<int4*>(lfrag);
+    for (int i = threadIdx.x; i < kFwdFrag / 8; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t r0 = wave * 32; r0 < n; r0 += nwaves * 32) {
-    frags = opaque(frags);
+    // an offset the compiler cannot see through keeps the LDS fragment reads inside
+    // the row loop (hoisting them would need hundreds of registers)
+    int lo = 0;
+    asm volatile("" : "+v"(lo));
+    const __bf16* frags = lfrag + lo;
     bf16x8 B[2][4];
     f32x4 acc[2][8];
     // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
@@ -508,11 +525,12 @@ int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, cons
   if (!x || !frags || !bias || !y) return fail(SSM_EINVAL, "NULL buffer");
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
+  // 512-thread blocks, two resident per CU (2 x 78 KB of LDS weights)
   const int64_t waves_needed = (n + 31) / 32;
-  int64_t blocks = (waves_needed + 3) / 4;
-  const int64_t cap = (int64_t)cus * 16;
+  int64_t blocks = (waves_needed + 7) / 8;
+  const int64_t cap = (int64_t)cus * 2;
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, frags, bias, xnorm, y);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(512), 0, s, x, n, frags, bias, xnorm, y);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

@@ -28,6 +28,7 @@ SET_STRIDE = 64
 P_T0, P_DEADLINE, P_DEADLINE_TMP, P_A, P_B1, P_B2, P_Z, P_V, P_SQRT_ST, P_G, P_BINH = range(11)
 P_VJ, P_ZJ, P_SSTJ = 16, 32, 48
 EM_WORKSPACE_WORDS = 4
+KDE_CHUNK = 8192
 
 class EmProblem(C.Structure):
     _fields_ = [
@@ -90,7 +91,8 @@ _SIGS = {
     ),
     "ssm_kde_eval": (
         C.c_int32,
-        [C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float, _VP, _VP],
+        [C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float,
+         C.c_int64, _VP, _VP, _VP],
     ),
     "ssm_lan_assemble": (
         C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]

@@ -71,7 +71,7 @@ def time_launch(job, lp: LaunchParams, reps: int = 3) -> float:
     return float(np.min(times))
 
 
-def tune(job, candidates=None, use_cache: bool = True, verbose: bool = False) -> LaunchParams:
+def tune(job, candidates=None, use_cache: bool = True, verbose: bool = False, reps: int = 3) -> LaunchParams:
     """Pick the fastest launch shape for `job` (an em.EmJob, already on the device)."""
     key = _key(job)
     if use_cache:
@@ -87,7 +87,7 @@ def tune(job, candidates=None, use_cache: bool = True, verbose: bool = False) ->
     D.torch().cuda.synchronize()
     best, best_ms = cands[0], float("inf")
     for lp in cands:
-        ms = time_launch(job, lp)
+        ms = time_launch(job, lp, reps)
         if verbose:
             print(f"[autotune] block={lp.block_size} per_cu={lp.blocks_per_cu} refill={lp.refill_min}: {ms:.3f} ms",
                   flush=True)

@@ -63,7 +63,8 @@ class BatchedSimulation:
 
     def __init__(self, model: str, theta: np.ndarray, n_samples, *, delta_t: float = 0.001, max_t: float = 20.0,
                  seed: int = 0, smooth_unif: bool = False, choice_dtype: str = "int8", launch="auto",
-                 deadline=None, replay_normals=None, replay_uniforms=None, use_table: bool = False):
+                 deadline=None, replay_normals=None, replay_uniforms=None, use_table: bool = False,
+                 tune_sample: float | None = 8e6):
         kind, code, P, rec, bname, bparams = records_for(model, theta, delta_t, max_t, deadline)
         n_sets = rec.shape[0]
         ns = np.broadcast_to(np.asarray(n_samples, np.int64), (n_sets,))
@@ -83,16 +84,20 @@ class BatchedSimulation:
                             replay_normals=replay_normals, replay_uniforms=replay_uniforms)
         self._graph = None
         if launch == "auto":
-            self.launch_params = self._autotune(kind, code, P, rec, delta_t, max_t, seed, smooth_unif, table)
+            if tune_sample is None:  # tune on the batch itself
+                self.launch_params = autotune.tune(self.job)
+            else:
+                self.launch_params = self._autotune(kind, code, P, rec, delta_t, max_t, seed, smooth_unif, table,
+                                                    tune_sample)
         else:
             self.launch_params = launch or LaunchParams()
 
-    def _autotune(self, kind, code, P, rec, delta_t, max_t, seed, smooth_unif, table):
+    def _autotune(self, kind, code, P, rec, delta_t, max_t, seed, smooth_unif, table, tune_sample):
         # tune on a bounded sample: up to 1024 evenly spaced sets (keeps the parameter mix),
-        # at most 8e6 trajectories
+        # at most `tune_sample` trajectories
         idx = np.arange(0, self.n_sets, max(1, self.n_sets // 1024))
         ns = np.diff(self.offsets)[idx]
-        scale = min(1.0, 8e6 / max(1, ns.sum()))
+        scale = min(1.0, tune_sample / max(1, ns.sum()))
         ns = np.maximum(1, (ns * scale).astype(np.int64))
         offs = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
         sub = make_job(kind, code, P, rec[idx], offs, delta_t=delta_t, max_t=max_t, seed=seed,

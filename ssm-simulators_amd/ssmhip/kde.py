@@ -52,6 +52,8 @@ class KDEBank:
         if not 1 <= self.C <= L.MAX_PARTICLES:
             raise ValueError("between 1 and 16 possible choices are supported")
         self.offsets = offsets
+        offs_h = offsets.cpu().numpy()
+        self.max_samples = int(np.max(np.diff(offs_h))) if offs_h.size > 1 else 0
         self.d_choice_values = D.to_dev(np.asarray(self.choice_values, np.int32))
         total = int(rts.shape[0])
         G = self.n_sets * self.C
@@ -96,10 +98,13 @@ class KDEBank:
         """Log-likelihood of every point of `pts` (same indexing as pts.rt)."""
         t = D.require_gpu()
         out = D.empty(tuple(pts.rt.shape), t.float32)
+        nch = max(1, -(-self.max_samples // L.KDE_CHUNK))
+        ws = D.empty((int(pts.rt.shape[0]) * nch,), t.float32)
         rc = L.load().ssm_kde_eval(self.n_sets, self.C, D.ptr(pts.grp_begin), D.ptr(pts.grp_count), D.ptr(pts.rt),
                                    D.ptr(self.samples), D.ptr(self.kde_count), D.ptr(self.choice_total),
                                    D.ptr(self.bandwidth), D.ptr(self.kde_start), D.ptr(self.offsets),
-                                   float(max_t), float(lb), float(eps), D.ptr(out), D.stream_ptr(stream))
+                                   float(max_t), float(lb), float(eps), self.max_samples, D.ptr(ws), D.ptr(out),
+                                   D.stream_ptr(stream))
         L.check(rc, "ssm_kde_eval")
         return out
 

@@ -345,13 +345,16 @@ class _SubBank:
         self.bandwidth = bank.bandwidth[g].contiguous()
         self.kde_start = bank.kde_start[g].contiguous()
         self.offsets = bank.offsets[k:k + 2].contiguous()
+        self.max_samples = bank.max_samples
 
     def evaluate(self, pts, max_t=20.0, lb=-66.774, eps=10e-5):
         t = D.torch()
         out = D.empty(tuple(pts.rt.shape), t.float32)
+        nch = max(1, -(-self.max_samples // L.KDE_CHUNK))
+        ws = D.empty((int(pts.rt.shape[0]) * nch,), t.float32)
         rc = L.load().ssm_kde_eval(1, self.C, D.ptr(pts.grp_begin), D.ptr(pts.grp_count), D.ptr(pts.rt),
                                    D.ptr(self.bank.samples), D.ptr(self.kde_count), D.ptr(self.choice_total),
                                    D.ptr(self.bandwidth), D.ptr(self.kde_start), D.ptr(self.offsets), float(max_t),
-                                   float(lb), float(eps), D.ptr(out), D.stream_ptr())
+                                   float(lb), float(eps), self.max_samples, D.ptr(ws), D.ptr(out), D.stream_ptr())
         L.check(rc, "ssm_kde_eval")
         return out
