@@ -206,8 +206,9 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
  * params: FP32 master weights, flat nn.Linear layout W1[100][7], b1, W2, b2,
  * W3[120][100], b3, W4[1][120], b4 (ssm_mlp_sizes gives every length).
  * ssm_mlp_pack converts params to bf16 MFMA fragments + padded biases.
- * xnorm_dev (nullable) = shift[7], scale[7]: the network sees (x - shift) * scale
- *   (input standardisation); dx is returned with respect to the raw x.
+ * xnorm_dev (nullable) = shift[7], scale[7], log_rt: the network sees
+ *   (x' - shift) * scale where x' is x with input 5 (rt) replaced by
+ *   log(max(rt, 1e-4)) when log_rt != 0; dx is returned w.r.t. the raw x.
  * ssm_mlp_forward: y[n] = net(x[n][7]).
  * ssm_mlp_backward: forward + backward-data for n rows; mode 0 takes the
  *   upstream gradient dL/dy in target_or_grad, mode 1 the Huber targets

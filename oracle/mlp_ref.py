@@ -41,12 +41,14 @@ def bf16_round(a: np.ndarray) -> np.ndarray:
 
 
 def standardise(x: np.ndarray, xnorm) -> np.ndarray:
-    """(x - shift) * scale in float32, as the kernels apply it before the first layer."""
-    x = np.asarray(x, np.float32)
+    """(x' - shift) * scale in float32, x' = x with rt -> log(max(rt, 1e-4)) when xnorm[14] != 0."""
+    x = np.array(x, np.float32)
     if xnorm is None:
         return x
     xn = np.asarray(xnorm, np.float32)
-    return ((x - xn[:7]).astype(np.float32) * xn[7:]).astype(np.float32)
+    if xn.shape[0] > 14 and xn[14] != 0:
+        x[:, 5] = np.log(np.maximum(x[:, 5], np.float32(1e-4)))
+    return ((x - xn[:7]).astype(np.float32) * xn[7:14]).astype(np.float32)
 
 
 def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None):
@@ -93,7 +95,11 @@ def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False, x
             dx = dh
     flat = np.concatenate([np.concatenate([g[0].reshape(-1), g[1]]) for g in grads])
     if xnorm is not None:
-        dx = dx * np.asarray(xnorm, np.float64)[7:]
+        xn = np.asarray(xnorm, np.float64)
+        dx = dx * xn[7:14]
+        if xn.shape[0] > 14 and xn[14] != 0:
+            rt = np.asarray(x, np.float64)[:, 5]
+            dx[:, 5] = np.where(rt > 1e-4, dx[:, 5] / np.maximum(rt, 1e-4), 0.0)
     return dx, flat
 
 

@@ -214,17 +214,41 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
         }
         bool alive = true;
+        // Steps use exec-mask branches rather than selects: with every lane usually on the
+        // same side, a branch costs scalar instructions while a select costs VALU issue slots.
+        if (__ballot(ix + 4 > A.n_steps) == 0) {
+          // common case: no lane can reach the end of t_s in this block, no index guard needed
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          // predicated step (no branch): a lane that stopped keeps its state
-          const int ixc = ix < A.n_steps ? ix : A.n_steps - 1;  // never read past a boundary table
-          const float b = boundary_at<BOUND>(ixc, dt, a, b1, b2, row);
-          const bool go = alive && (ix < A.n_steps) && (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
-          const float xn = x[0] + (vdt[0] + sst[0] * nz[i]);  // ddm_models.pyx:720
-          x[0] = go ? xn : x[0];
-          t = go ? t + dt : t;
-          ix += go ? 1 : 0;
-          alive = go;
+          for (int i = 0; i < 4; ++i) {
+            if (alive) {
+              const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+              if ((x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp)) {
+                x[0] += vdt[0] + sst[0] * nz[i];  // ddm_models.pyx:720
+                t += dt;
+                ++ix;
+              } else {
+                alive = false;
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (alive) {
+              bool go = ix < A.n_steps;  // never read past a boundary table
+              if (go) {
+                const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+                go = (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
+              }
+              if (go) {
+                x[0] += vdt[0] + sst[0] * nz[i];
+                t += dt;
+                ++ix;
+              } else {
+                alive = false;
+              }
+            }
+          }
         }
         if (!alive) state = kDone;
       } else {

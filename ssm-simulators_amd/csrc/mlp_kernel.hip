@@ -54,6 +54,15 @@ __device__ __forceinline__ float tanh_fast(float zs) {
   return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(zs) + 1.0f), 1.0f);
 }
 
+// Input transform: xnorm = shift[7], scale[7], log_rt flag. With the flag set,
+// input 5 (rt) enters as log(max(rt, 1e-4)) before the shift and scale.
+constexpr float kRtFloor = 1e-4f;
+__device__ __forceinline__ float input_value(const float* xnorm, int j, float v) {
+  if (!xnorm) return v;
+  if (j == 5 && xnorm[2 * kIn] != 0.f) v = logf(fmaxf(v, kRtFloor));
+  return (v - xnorm[j]) * xnorm[kIn + j];
+}
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -166,7 +175,7 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       if (g == 0 && row < n) {
         const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)(xnorm ? (xr[j] - xnorm[j]) * xnorm[kIn + j] : xr[j]);
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value(xnorm, j, xr[j]);
       }
       B[t][0] = v;
     }
@@ -253,8 +262,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       if (g == 0 && live) {
 #pragma unroll
         for (int jj = 0; jj < kIn; ++jj) {
-          const float xv = A.x[row * kIn + jj];
-          v[jj] = (__bf16)(A.xnorm ? (xv - A.xnorm[jj]) * A.xnorm[kIn + jj] : xv);
+          v[jj] = (__bf16)input_value(A.xnorm, jj, A.x[row * kIn + jj]);
         }
       }
       x0 = v;
@@ -399,8 +407,13 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       if (live && g < 2) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int k = 4 * g + i;  // d/dx_raw = d/dx_norm * scale
-          A.dx[row * 8 + k] = (A.xnorm && k < kIn) ? c[i] * A.xnorm[kIn + k] : c[i];
+          const int k = 4 * g + i;  // d/dx_raw = d/dx_norm * scale (* 1/rt for a log-rt input)
+          float d = (A.xnorm && k < kIn) ? c[i] * A.xnorm[kIn + k] : c[i];
+          if (A.xnorm && k == 5 && A.xnorm[2 * kIn] != 0.f) {
+            const float rt = A.x[row * kIn + 5];
+            d = rt > kRtFloor ? d / rt : 0.f;
+          }
+          A.dx[row * 8 + k] = d;
         }
       }
     }

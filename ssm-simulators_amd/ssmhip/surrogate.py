@@ -20,6 +20,7 @@ from . import _device as D
 from . import _lib as L
 
 IN, H1, H2, H3, OUT = 7, 100, 100, 120, 1
+FLOOR = -66.774  # the KDE labels' log-likelihood floor (kde_class.py:188-195)
 LAYERS = [(H1, IN), (H2, H1), (H3, H2), (OUT, H3)]
 
 
@@ -149,7 +150,8 @@ class MLPSurrogate:
         self.step = 0
         self.kern.pack(self.params)
         self.bounds = None  # (lo[7], hi[7]) of the training inputs
-        self.val_mae = None
+        self.val_mae = None  # mean |error| vs held-out KDE labels above the floor
+        self.val_mae_all = None
 
     # ---- inference -----------------------------------------------------------
     def __call__(self, x):
@@ -211,15 +213,23 @@ class MLPSurrogate:
         self.bounds = (xs.min(0).values.cpu().numpy(), xs.max(0).values.cpu().numpy())
         if val_rows is not None:
             pred = self(val_rows[:, :IN].contiguous())
-            self.val_mae = float((pred - val_rows[:, IN]).abs().mean().item())
+            err = (pred - val_rows[:, IN]).abs()
+            above = val_rows[:, IN] > FLOOR + 1.0  # rows the KDE labels above its floor
+            self.val_mae_all = float(err.mean().item())
+            self.val_mae = float(err[above].mean().item()) if bool(above.any()) else self.val_mae_all
         return history
 
-    def set_input_normalisation(self, x) -> None:
-        """Standardise the inputs: the network sees (x - mean) / std (std floored at 1e-6)."""
+    def set_input_normalisation(self, x, log_rt: bool = True) -> None:
+        """Standardise the inputs: the network sees (x' - mean) / std (std floored at 1e-6), where
+        x' has rt replaced by log(max(rt, 1e-4)) when `log_rt` (the KDE lives on log rt)."""
         t = D.torch()
-        mu = x.mean(0)
-        sd = x.std(0).clamp_min(1e-6)
-        self.kern.xnorm = t.cat([mu, 1.0 / sd]).float().contiguous()
+        xx = x.float().clone()
+        if log_rt:
+            xx[:, 5] = t.log(t.clamp_min(xx[:, 5], 1e-4))
+        mu = xx.mean(0)
+        sd = xx.std(0).clamp_min(1e-6)
+        flag = t.tensor([1.0 if log_rt else 0.0], device=x.device)
+        self.kern.xnorm = t.cat([mu, 1.0 / sd, flag]).float().contiguous()
 
     # ---- checkpoint ----------------------------------------------------------
     def state_dict(self) -> dict:
@@ -252,9 +262,12 @@ class MLPSurrogate:
                 def __init__(self):
                     super().__init__()
                     self.register_buffer("shift", xn[:IN].clone())
-                    self.register_buffer("scale", xn[IN:].clone())
+                    self.register_buffer("scale", xn[IN:2 * IN].clone())
+                    self.log_rt = bool(xn.shape[0] > 2 * IN and xn[2 * IN] != 0)
 
                 def forward(self, x):
+                    if self.log_rt:
+                        x = torch.cat([x[:, :5], torch.log(torch.clamp_min(x[:, 5:6], 1e-4)), x[:, 6:]], dim=1)
                     return (x - self.shift) * self.scale
 
             layers.append(Standardise())
@@ -280,7 +293,7 @@ class SurrogateLikelihood:
     parameter set and evaluating the GPU KDE at the row.
     """
 
-    def __init__(self, surrogate: MLPSurrogate, model: str = "angle", max_val_mae: float = 0.5,
+    def __init__(self, surrogate: MLPSurrogate, model: str = "angle", max_val_mae: float = 1.0,
                  kde_samples: int = 100_000, seed: int = 0):
         self.net, self.model, self.max_val_mae = surrogate, model, max_val_mae
         self.kde_samples, self.seed = kde_samples, seed

@@ -110,7 +110,8 @@ def test_standardised_inputs_forward_and_gradients(torch_gpu):
 
     net = MLPSurrogate(seed=6)
     x_np = _inputs(2000, seed=6)
-    net.set_input_normalisation(D.to_dev(x_np))
+    x_np[:7, 5] = [0.0, -0.5, 5e-5, 1.5e-4, 2e-4, 0.01, 19.0]  # rt below, near and above the log floor
+    net.set_input_normalisation(D.to_dev(x_np), log_rt=True)
     xn = net.kern.xnorm.cpu().numpy()
     p = net.params.cpu().numpy()
     y = net(D.to_dev(x_np)).cpu().numpy()
