@@ -200,8 +200,9 @@ def config4_leg(args):
     return rec, lab, rows
 
 
-def config5_leg(args, lab, rows, kde_evals_per_s):
-    """Surrogate: train on the config-4 rows, then 4M-row inference vs GPU KDE and the analytic density."""
+def config5_leg(args, lab, kde_evals_per_s):
+    """Surrogate: label rows of --surrogate-sets angle sets, train, then 4M-row inference against
+    the GPU KDE and accuracy against the analytic DDM density."""
     import torch
 
     from oracle import ddm_analytic
@@ -209,21 +210,28 @@ def config5_leg(args, lab, rows, kde_evals_per_s):
 
     from ssmhip.lan import LanKDELabeller, sample_config4
 
-    # more training rows from the same pipeline (angle sets outside the config-4 sample)
-    extra_sets = max(0, args.surrogate_sets - lab.n_sets)
-    parts = [rows]
+    # training rows from the same pipeline at a larger simulation budget per set (1e6 draws:
+    # the KDE labels' own error falls with the bandwidth), angle sets outside the config-4 sample
+    parts = []
     done = 0
-    while done < extra_sets:
-        k = min(2048, extra_sets - done)
-        more = LanKDELabeller("angle", sample_config4(k, seed=100 + done), n_samples=100_000, seed=200 + done,
-                              launch=lab.sim.launch_params)
+    torch.cuda.synchronize()
+    t_lab = time.perf_counter()
+    while done < args.surrogate_sets:
+        k = min(2048, args.surrogate_sets - done)
+        more = LanKDELabeller("angle", sample_config4(k, seed=100 + done), n_samples=args.surrogate_samples,
+                              seed=200 + done, launch=lab.sim.launch_params)
         parts.append(more.run())
+        del more
         done += k
     rows = torch.cat(parts, 0).contiguous()
+    del parts
+    torch.cuda.synchronize()
+    label_s = time.perf_counter() - t_lab
     n = rows.shape[0]
     perm = torch.randperm(n, device=rows.device, generator=torch.Generator(device=rows.device).manual_seed(0))
-    val = rows.index_select(0, perm[: n // 20])
-    train = rows.index_select(0, perm[n // 20:])
+    val = rows.index_select(0, perm[: n // 50])
+    train = rows.index_select(0, perm[n // 50:])
+    del rows, perm
     net = MLPSurrogate(seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -288,7 +296,8 @@ def config5_leg(args, lab, rows, kde_evals_per_s):
     kde_at = np.empty_like(kll)
     kde_at[order] = kll
     kde_mae = float(np.mean(np.abs(kde_at - ref)))
-    rec = {"train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
+    rec = {"label_sets": args.surrogate_sets, "label_samples_per_set": args.surrogate_samples,
+           "label_s": label_s, "train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
            "train_rows_per_s": train.shape[0] * args.surrogate_epochs / train_s, "huber_history": hist[-3:],
            "val_mae_vs_kde_above_floor": net.val_mae, "val_mae_vs_kde_all": net.val_mae_all, "inference_rows": N, "ms_per_inference": ms, "evals_per_s": evals,
            "over_gpu_kde": evals / kde_evals_per_s,
@@ -310,8 +319,12 @@ def main() -> int:
     ap.add_argument("--launch", type=str, default="auto", help="'auto' or block,blocks_per_cu,refill_min")
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU roof probe")
     ap.add_argument("--headline-only", action="store_true", help="skip the config-3/4/5 legs")
-    ap.add_argument("--surrogate-epochs", type=int, default=40)
-    ap.add_argument("--surrogate-sets", type=int, default=8192, help="parameter sets of KDE rows to train on")
+    ap.add_argument("--legs", type=str, default="headline,config3,config4,config5",
+                    help="comma list of legs to run (config5 needs config4); without 'headline' the "
+                         "JSON line carries no headline value (for profiling one leg)")
+    ap.add_argument("--surrogate-epochs", type=int, default=60)
+    ap.add_argument("--surrogate-sets", type=int, default=32768, help="parameter sets of KDE rows to train on")
+    ap.add_argument("--surrogate-samples", type=int, default=1_000_000, help="simulations per set for the labels")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -323,11 +336,36 @@ def main() -> int:
     import torch
 
     from ssmhip import batched, roofline
-    from ssmhip.em import LaunchParams
 
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
+    legs = {"headline"} if args.headline_only else set(args.legs.split(","))
+    unknown = legs - {"headline", "config3", "config4", "config5"}
+    if unknown or ("config5" in legs and "config4" not in legs):
+        print(f"bad --legs {args.legs!r}: legs are headline, config3, config4, config5 (config5 needs config4)",
+              file=sys.stderr)
+        return 2
+    out = {"metric": "angle-DDM trajectories/s", "value": None, "unit": "trajectories/s", "n_gpus": 1,
+           "legs": sorted(legs)}
+    if "headline" in legs:
+        out = headline_leg(args, torch, batched, roofline)
+    if "config3" in legs:
+        out["config3"] = config3_leg(args)
+    if "config4" in legs:
+        c4, lab, rows = config4_leg(args)
+        out["config4"] = c4
+        if "config5" in legs:
+            del rows
+            out["config5"] = config5_leg(args, lab, c4["kde_evals_per_s"])
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def headline_leg(args, torch, batched, roofline) -> dict:
+    """Batched angle-DDM, 4096 ragged sets, one persistent hipGraph launch per step."""
+    from ssmhip.em import LaunchParams
+
     theta, ns = batched.sample_config2(args.n_sets, seed=0)
     log(f"config2: {args.n_sets} sets, {int(ns.sum())} trajectories")
     launch = "auto" if args.launch == "auto" else LaunchParams(*[int(x) for x in args.launch.split(",")])
@@ -411,15 +449,8 @@ def main() -> int:
         cb.pop("calibration_rate", None)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
-    if not args.headline_only:
-        sim.close()
-        del sim
-        out["config3"] = config3_leg(args)
-        c4, lab, rows = config4_leg(args)
-        out["config4"] = c4
-        out["config5"] = config5_leg(args, lab, rows, c4["kde_evals_per_s"])
-    print(json.dumps(out), flush=True)
-    return 0
+    sim.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -206,9 +206,11 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
  * params: FP32 master weights, flat nn.Linear layout W1[100][7], b1, W2, b2,
  * W3[120][100], b3, W4[1][120], b4 (ssm_mlp_sizes gives every length).
  * ssm_mlp_pack converts params to bf16 MFMA fragments + padded biases.
- * xnorm_dev (nullable) = shift[7], scale[7], log_rt: the network sees
- *   (x' - shift) * scale where x' is x with input 5 (rt) replaced by
- *   log(max(rt, 1e-4)) when log_rt != 0; dx is returned w.r.t. the raw x.
+ * xnorm_dev (nullable) = shift[7], scale[7], rt_mode, t_index (float32[16]):
+ *   the network sees (x' - shift) * scale where x' is x with input 5 (rt)
+ *   replaced by log(max(u, 1e-4)), u = rt for rt_mode 1 and u = rt - x[t_index]
+ *   for rt_mode 2 (unchanged for 0; t_index is clamped to 0..6); dx is
+ *   returned w.r.t. the raw x (chain rule through u included).
  * ssm_mlp_forward: y[n] = net(x[n][7]).
  * ssm_mlp_backward: forward + backward-data for n rows; mode 0 takes the
  *   upstream gradient dL/dy in target_or_grad, mode 1 the Huber targets
@@ -217,7 +219,16 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
  *   acts[act_rows][ld] and deltas[delta_rows][ld] (ld >= n, ld % 32 == 0,
  *   columns >= n zero). Any of these outputs may be NULL.
  * ssm_mlp_wgrad: grad[n_params] = dL/dparams from acts/deltas.
- * ssm_mlp_adam: Adam step `step` (>= 1) on params with moments m, v.      */
+ * ssm_mlp_adam: Adam step `step` (>= 1) on params with moments m, v.
+ * ssm_mlp_train_steps: n_steps Adam steps on the Huber loss, step i on rows
+ *   [i*batch, (i+1)*batch) of x[n_steps*batch][7] / target[...], learning rate
+ *   lr_host[i] (host array), bias corrections for steps step0+1 ...; per step
+ *   the fused forward/backward, the weight gradient (summed over batch slices
+ *   in a fixed order), Adam and the repack of frags/bias are enqueued from
+ *   the host without a synchronisation. batch must be a multiple of 32;
+ *   workspace_dev (16-byte aligned) holds at least
+ *   ssm_mlp_train_workspace_bytes(batch) bytes (-1: invalid batch); loss_dev
+ *   (nullable) receives the per-row Huber loss [n_steps*batch].             */
 int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
 int32_t ssm_mlp_pack(const float* params_dev, void* frags_dev, float* bias_dev, void* stream);
 int32_t ssm_mlp_forward(const float* x_dev, int64_t n, const void* frags_dev, const float* bias_dev,
@@ -230,6 +241,12 @@ int32_t ssm_mlp_wgrad(const void* acts_dev, const void* deltas_dev, int64_t n, i
                       void* stream);
 int32_t ssm_mlp_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, int32_t step, float lr,
                      float beta1, float beta2, float eps, float weight_decay, void* stream);
+int64_t ssm_mlp_train_workspace_bytes(int64_t batch);
+int32_t ssm_mlp_train_steps(const float* x_dev, const float* target_dev, int64_t batch, int32_t n_steps,
+                            int32_t step0, const float* lr_host, float beta1, float beta2, float eps,
+                            float weight_decay, float delta, float* params_dev, float* m_dev, float* v_dev,
+                            void* frags_dev, float* bias_dev, const float* xnorm_dev, void* workspace_dev,
+                            int64_t workspace_bytes, float* loss_dev, void* stream);
 /* One v_mfma_f32_16x16x32_bf16 with per-lane operands a[64][8], b[64][8]
  * (bf16) and result d[64][4]: pins the operand / result lane layout.        */
 int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream);

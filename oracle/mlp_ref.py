@@ -40,14 +40,29 @@ def bf16_round(a: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def rt_feature(x: np.ndarray, xnorm) -> np.ndarray | None:
+    """u with input 5 = log(max(u, 1e-4)) (float32): u = rt (mode 1) or rt - x[t index]
+    (mode 2); None for mode 0."""
+    xn = np.asarray(xnorm, np.float32)
+    mode = float(xn[14]) if xn.shape[0] > 14 else 0.0
+    if mode == 0.0:
+        return None
+    u = np.array(x, np.float32)[:, 5]
+    if mode == 2.0:
+        u = (u - np.array(x, np.float32)[:, int(xn[15])]).astype(np.float32)
+    return u
+
+
 def standardise(x: np.ndarray, xnorm) -> np.ndarray:
-    """(x' - shift) * scale in float32, x' = x with rt -> log(max(rt, 1e-4)) when xnorm[14] != 0."""
+    """(x' - shift) * scale in float32; x' = x with input 5 replaced by log(max(u, 1e-4)),
+    u from rt_feature, when xnorm[14] (the rt mode) is 1 or 2."""
     x = np.array(x, np.float32)
     if xnorm is None:
         return x
     xn = np.asarray(xnorm, np.float32)
-    if xn.shape[0] > 14 and xn[14] != 0:
-        x[:, 5] = np.log(np.maximum(x[:, 5], np.float32(1e-4)))
+    u = rt_feature(x, xnorm)
+    if u is not None:
+        x[:, 5] = np.log(np.maximum(u, np.float32(1e-4)))
     return ((x - xn[:7]).astype(np.float32) * xn[7:14]).astype(np.float32)
 
 
@@ -97,9 +112,12 @@ def backward(p: np.ndarray, x: np.ndarray, gy: np.ndarray, bf16: bool = False, x
     if xnorm is not None:
         xn = np.asarray(xnorm, np.float64)
         dx = dx * xn[7:14]
-        if xn.shape[0] > 14 and xn[14] != 0:
-            rt = np.asarray(x, np.float64)[:, 5]
-            dx[:, 5] = np.where(rt > 1e-4, dx[:, 5] / np.maximum(rt, 1e-4), 0.0)
+        u = rt_feature(x, xnorm)
+        if u is not None:
+            u = u.astype(np.float64)
+            dx[:, 5] = np.where(u > 1e-4, dx[:, 5] / np.maximum(u, 1e-4), 0.0)
+            if xn[14] == 2:
+                dx[:, int(xn[15])] -= dx[:, 5]
     return dx, flat
 
 

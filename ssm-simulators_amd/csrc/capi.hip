@@ -76,6 +76,11 @@ int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, cons
 int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s);
 int32_t mlp_wgrad_enqueue(const __bf16* acts, const __bf16* deltas, int64_t n, int64_t ld, float* grad,
                           hipStream_t s);
+int64_t mlp_train_workspace_bytes(int64_t batch);
+int32_t mlp_train_steps_enqueue(const float* x, const float* target, int64_t batch, int32_t n_steps, int32_t step0,
+                                const float* lr, float b1, float b2, float eps, float wd, float delta, float* P,
+                                float* M, float* V, __bf16* frags, float* bias, const float* xnorm, void* ws,
+                                int64_t ws_bytes, float* loss, hipStream_t s);
 int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t step, float lr, float b1, float b2,
                          float eps, float wd, hipStream_t s);
 int32_t mfma_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s);
@@ -249,6 +254,18 @@ int32_t ssm_mlp_adam(float* params_dev, const float* grad_dev, float* m_dev, flo
                      float beta1, float beta2, float eps, float weight_decay, void* stream) {
   return mlp_adam_enqueue(params_dev, grad_dev, m_dev, v_dev, step, lr, beta1, beta2, eps, weight_decay,
                           (hipStream_t)stream);
+}
+
+int64_t ssm_mlp_train_workspace_bytes(int64_t batch) { return mlp_train_workspace_bytes(batch); }
+
+int32_t ssm_mlp_train_steps(const float* x_dev, const float* target_dev, int64_t batch, int32_t n_steps,
+                            int32_t step0, const float* lr_host, float beta1, float beta2, float eps,
+                            float weight_decay, float delta, float* params_dev, float* m_dev, float* v_dev,
+                            void* frags_dev, float* bias_dev, const float* xnorm_dev, void* workspace_dev,
+                            int64_t workspace_bytes, float* loss_dev, void* stream) {
+  return mlp_train_steps_enqueue(x_dev, target_dev, batch, n_steps, step0, lr_host, beta1, beta2, eps,
+                                 weight_decay, delta, params_dev, m_dev, v_dev, (__bf16*)frags_dev, bias_dev,
+                                 xnorm_dev, workspace_dev, workspace_bytes, loss_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream) {

@@ -54,12 +54,21 @@ __device__ __forceinline__ float tanh_fast(float zs) {
   return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(zs) + 1.0f), 1.0f);
 }
 
-// Input transform: xnorm = shift[7], scale[7], log_rt flag. With the flag set,
-// input 5 (rt) enters as log(max(rt, 1e-4)) before the shift and scale.
+// Input transform: xnorm = shift[7], scale[7], rt mode, t index. Mode 1: input 5 (rt)
+// enters as log(max(rt, 1e-4)); mode 2: as log(max(rt - x[t index], 1e-4)), the log of
+// the time since the non-decision time; then every input as (v - shift) * scale.
 constexpr float kRtFloor = 1e-4f;
-__device__ __forceinline__ float input_value(const float* xnorm, int j, float v) {
+__device__ __forceinline__ int t_index(const float* xnorm) {
+  return min(max((int)xnorm[2 * kIn + 1], 0), kIn - 1);
+}
+__device__ __forceinline__ float input_value(const float* xnorm, int j, const float* xr) {
+  float v = xr[j];
   if (!xnorm) return v;
-  if (j == 5 && xnorm[2 * kIn] != 0.f) v = logf(fmaxf(v, kRtFloor));
+  if (j == 5) {
+    const float mode = xnorm[2 * kIn];
+    if (mode == 2.f) v -= xr[t_index(xnorm)];
+    if (mode != 0.f) v = logf(fmaxf(v, kRtFloor));
+  }
   return (v - xnorm[j]) * xnorm[kIn + j];
 }
 
@@ -175,7 +184,7 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       if (g == 0 && row < n) {
         const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value(xnorm, j, xr[j]);
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value(xnorm, j, xr);
       }
       B[t][0] = v;
     }
@@ -262,7 +271,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       if (g == 0 && live) {
 #pragma unroll
         for (int jj = 0; jj < kIn; ++jj) {
-          v[jj] = (__bf16)input_value(A.xnorm, jj, A.x[row * kIn + jj]);
+          v[jj] = (__bf16)input_value(A.xnorm, jj, A.x + row * kIn);
         }
       }
       x0 = v;
@@ -403,16 +412,25 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
       f32x4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int b = 0; b < 4; ++b) c = mfma(load_frag(A.frags, fB1, b, lane), B[b], c);
-      // lane holds dx[row][4g + i]; inputs are features 0..6 (groups 0 and 1)
+      // lane holds dx[row][4g + i]; inputs are features 0..6 (groups 0 and 1).
+      // d/dx_raw = d/dx_norm * scale, times d(feature)/d(rt) = 1/u for input 5 with a log
+      // transform (u = rt, or rt - t in mode 2, zero below the floor); in mode 2 the t input
+      // also receives -1/u of it, taken from the row's group-1 lane.
+      const float mode = A.xnorm ? A.xnorm[2 * kIn] : 0.f;
+      float d5 = 0.f;
+      if (mode != 0.f && live) {
+        float u = A.x[row * kIn + 5];
+        if (mode == 2.f) u -= A.x[row * kIn + t_index(A.xnorm)];
+        d5 = u > kRtFloor ? c[1] * A.xnorm[kIn + 5] / u : 0.f;
+      }
+      const float d5_row = __shfl(d5, 16 + r, 64);
       if (live && g < 2) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int k = 4 * g + i;  // d/dx_raw = d/dx_norm * scale (* 1/rt for a log-rt input)
+          const int k = 4 * g + i;
           float d = (A.xnorm && k < kIn) ? c[i] * A.xnorm[kIn + k] : c[i];
-          if (A.xnorm && k == 5 && A.xnorm[2 * kIn] != 0.f) {
-            const float rt = A.x[row * kIn + 5];
-            d = rt > kRtFloor ? d / rt : 0.f;
-          }
+          if (k == 5 && mode != 0.f) d = d5;
+          if (mode == 2.f && k == t_index(A.xnorm)) d -= d5_row;
           A.dx[row * 8 + k] = d;
         }
       }
@@ -421,12 +439,17 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
 }
 
 // ---- weight gradients: dW_L = dZ_L^T H_{L-1} over the batch -------------------------------
-// Tasks: L1 8x1 tiles, L2 8x8, L3 8x8, L4 1x8 (out tile a, in tile c); one wave per task.
-// grad: [kNParams] fp32 (nn.Linear layout), written (not accumulated).
+// Tasks: L1 8x1 tiles, L2 8x8, L3 8x8, L4 1x8 (out tile a, in tile c); one wave per task and
+// batch slice: blockIdx.y = s takes rows [s * chunk, min(n, (s + 1) * chunk)) (chunk a multiple
+// of 64) and writes its partial sums to grad + s * kNParams (nn.Linear layout, written, not
+// accumulated); the Adam kernel adds the slices in order, so the sum is deterministic.
 __global__ void __launch_bounds__(64) mlp_wgrad_kernel(const __bf16* __restrict__ acts,
                                                       const __bf16* __restrict__ deltas, int64_t n, int64_t ld,
-                                                      float* __restrict__ grad) {
+                                                      int64_t chunk, float* __restrict__ grad) {
   int task = blockIdx.x;
+  grad += (int64_t)blockIdx.y * kNParams;
+  const int64_t kb = (int64_t)blockIdx.y * chunk;
+  const int64_t ke = kb + chunk < n ? kb + chunk : n;
   int layer, a, c;
   if (task < 8) { layer = 1; a = task; c = 0; }
   else if (task < 72) { layer = 2; a = (task - 8) / 8; c = (task - 8) % 8; }
@@ -440,8 +463,8 @@ __global__ void __launch_bounds__(64) mlp_wgrad_kernel(const __bf16* __restrict_
   const __bf16* hh = acts + (int64_t)(in_ok ? arow : 0) * ld;
   f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
   float bsum = 0.f;
-  int64_t k = 0;
-  for (; k + 64 <= n; k += 64) {
+  int64_t k = kb;
+  for (; k + 64 <= ke; k += 64) {
     bf16x8 av0 = *reinterpret_cast<const bf16x8*>(dz + k + 8 * g);
     bf16x8 av1 = *reinterpret_cast<const bf16x8*>(dz + k + 32 + 8 * g);
     bf16x8 bv0 = {}, bv1 = {};
@@ -456,7 +479,7 @@ __global__ void __launch_bounds__(64) mlp_wgrad_kernel(const __bf16* __restrict_
       for (int j = 0; j < 8; ++j) bsum += (float)av0[j] + (float)av1[j];
     }
   }
-  for (; k < n; k += 32) {  // tail (ld is padded to a multiple of 32 with zeros)
+  for (; k < ke; k += 32) {  // tail of the last slice (ld is padded to a multiple of 32 with zeros)
     bf16x8 av = *reinterpret_cast<const bf16x8*>(dz + k + 8 * g);
     bf16x8 bv = {};
     if (in_ok) bv = *reinterpret_cast<const bf16x8*>(hh + k + 8 * g);
@@ -488,12 +511,15 @@ __global__ void __launch_bounds__(64) mlp_wgrad_kernel(const __bf16* __restrict_
 }
 
 // ---- Adam on the FP32 master weights ---------------------------------------------------------
-__global__ void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
-                                float* __restrict__ V, int n, float lr, float b1, float b2, float eps, float bc1,
-                                float bc2, float wd) {
+// G holds nsplit gradient slices of n values each, summed here in slice order.
+__global__ void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ G, int nsplit,
+                                float* __restrict__ M, float* __restrict__ V, int n, float lr, float b1, float b2,
+                                float eps, float bc1, float bc2, float wd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float g = G[i] + wd * P[i];
+  float gs = G[i];
+  for (int s = 1; s < nsplit; ++s) gs += G[(int64_t)s * n + i];
+  const float g = gs + wd * P[i];
   const float m = b1 * M[i] + (1.f - b1) * g;
   const float v = b2 * V[i] + (1.f - b2) * g * g;
   M[i] = m;
@@ -555,11 +581,12 @@ int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s) {
   if ((A.acts || A.deltas) && (A.ld < A.n || A.ld % 32)) return fail(SSM_EINVAL, "ld must be >= n and a multiple of 32");
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
+  // one-wave blocks (16 rows each) so that a training batch of a few thousand rows spreads
+  // over every CU; at most 16 waves per CU, the rest by grid stride
   const int64_t waves_needed = (A.n + 15) / 16;
-  int64_t blocks = (waves_needed + 3) / 4;
   const int64_t cap = (int64_t)cus * 16;
-  if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A);
+  const int64_t blocks = waves_needed < cap ? waves_needed : cap;
+  hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)blocks), dim3(64), 0, s, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }
@@ -568,7 +595,7 @@ int32_t mlp_wgrad_enqueue(const __bf16* acts, const __bf16* deltas, int64_t n, i
                           hipStream_t s) {
   if (n < 1 || ld < n || ld % 32) return fail(SSM_EINVAL, "need n >= 1 and ld >= n, ld % 32 == 0");
   if (!acts || !deltas || !grad) return fail(SSM_EINVAL, "NULL buffer");
-  hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(144), dim3(64), 0, s, acts, deltas, n, ld, grad);
+  hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(144, 1), dim3(64), 0, s, acts, deltas, n, ld, ld, grad);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }
@@ -578,9 +605,52 @@ int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t s
   if (!P || !G || !M || !V) return fail(SSM_EINVAL, "NULL buffer");
   if (step < 1) return fail(SSM_EINVAL, "step counts from 1");
   const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
-  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, G, M, V, kNParams, lr, b1,
-                     b2, eps, bc1, bc2, wd);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, G, 1, M, V, kNParams, lr,
+                     b1, b2, eps, bc1, bc2, wd);
   SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+
+// ---- a run of Adam steps on consecutive batches, enqueued from the host ----------------------
+int64_t mlp_train_split(int64_t batch) {
+  int64_t sp = batch / 1024;
+  return sp < 1 ? 1 : sp > 16 ? 16 : sp;
+}
+int64_t mlp_train_workspace_bytes(int64_t batch) {
+  if (batch < 32 || batch % 32) return -1;
+  return (int64_t)(kActRows + kDeltaRows) * batch * 2 + mlp_train_split(batch) * kNParams * 4;
+}
+int32_t mlp_train_steps_enqueue(const float* x, const float* target, int64_t batch, int32_t n_steps, int32_t step0,
+                                const float* lr, float b1, float b2, float eps, float wd, float delta, float* P,
+                                float* M, float* V, __bf16* frags, float* bias, const float* xnorm, void* ws,
+                                int64_t ws_bytes, float* loss, hipStream_t s) {
+  if (batch < 32 || batch % 32) return fail(SSM_EINVAL, "batch must be a positive multiple of 32");
+  if (n_steps < 0 || step0 < 0) return fail(SSM_EINVAL, "n_steps and step0 must be >= 0");
+  if (n_steps == 0) return SSM_OK;
+  if (!x || !target || !lr || !P || !M || !V || !frags || !bias || !ws) return fail(SSM_EINVAL, "NULL buffer");
+  if (ws_bytes < mlp_train_workspace_bytes(batch)) return fail(SSM_EINVAL, "workspace too small");
+  if ((uintptr_t)ws % 16) return fail(SSM_EINVAL, "workspace must be 16-byte aligned");
+  __bf16* acts = (__bf16*)ws;
+  __bf16* deltas = acts + (int64_t)kActRows * batch;
+  float* grad = (float*)(deltas + (int64_t)kDeltaRows * batch);
+  const int64_t split = mlp_train_split(batch);
+  const int64_t chunk = ((batch + split - 1) / split + 63) / 64 * 64;
+  for (int32_t i = 0; i < n_steps; ++i) {
+    const int64_t r0 = (int64_t)i * batch;
+    TrainArgs A{x + r0 * kIn, target + r0, frags, bias, xnorm, acts, deltas, nullptr, nullptr,
+                loss ? loss + r0 : nullptr, batch, batch, 1, delta, 1.0f / (float)batch};
+    int32_t rc = mlp_train_enqueue(A, s);
+    if (rc != SSM_OK) return rc;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(144, (unsigned)split), dim3(64), 0, s, acts, deltas, batch, batch,
+                       chunk, grad);
+    const int32_t step = step0 + i + 1;
+    const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, grad, (int)split, M, V,
+                       kNParams, lr[i], b1, b2, eps, bc1, bc2, wd);
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((kNFrag + 255) / 256), dim3(256), 0, s, P, frags, bias);
+    SSM_HIP(hipGetLastError());
+  }
   return SSM_OK;
 }
 

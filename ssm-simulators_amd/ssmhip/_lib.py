@@ -109,6 +109,12 @@ _SIGS = {
     "ssm_mlp_adam": (
         C.c_int32, [_VP, _VP, _VP, _VP, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _VP]
     ),
+    "ssm_mlp_train_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "ssm_mlp_train_steps": (
+        C.c_int32,
+        [_VP, _VP, C.c_int64, C.c_int32, C.c_int32, _VP, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+         _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int64, _VP, _VP],
+    ),
     "ssm_mfma_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (

@@ -10,8 +10,8 @@ does per set inside a process pool:
 Rows are [theta..., rt, choice, loglik] float32 (lan_data = rows[:, :-1],
 lan_labels = rows[:, -1], simulation_pipeline.py:153-167). The mixture is
 n_kde KDE resamples + n_unif uniform rows (rt ~ U(1e-4, max_t), random
-choice); the reference's default third component (negative RTs labelled
--66.77497) is supported via n_neg.
+choice) + n_neg negative-RT rows labelled -66.77497, 800 / 100 / 100 by default
+(the reference's [0.8, 0.1, 0.1] of 1000 rows per set).
 """
 
 from __future__ import annotations
@@ -51,7 +51,7 @@ class LanKDELabeller:
     """KDE-labelled LAN rows for a batch of parameter sets, entirely on one GPU."""
 
     def __init__(self, model: str, theta: np.ndarray, *, n_samples: int = 100_000, n_kde: int = 800,
-                 n_unif: int = 200, n_neg: int = 0, delta_t: float = 0.001, max_t: float = 20.0, seed: int = 0,
+                 n_unif: int = 100, n_neg: int = 100, delta_t: float = 0.001, max_t: float = 20.0, seed: int = 0,
                  smooth_unif: bool = True, shuffle: bool = True, launch="auto", lb: float = -66.774):
         self.model = model
         self.theta = np.ascontiguousarray(theta, np.float32)
@@ -91,7 +91,8 @@ class LanKDELabeller:
         m = pts.m
         W = self.theta.shape[1] + 3
         rows = D.empty((self.n_sets * m, W), t.float32)
-        rc = L.load().ssm_lan_assemble(self.n_sets, self.theta.shape[1], m, int(self.shuffle), self.seed + 2,
+        shuffle_here = self.shuffle and not self.n_neg  # with negative rows the shuffle follows their append
+        rc = L.load().ssm_lan_assemble(self.n_sets, self.theta.shape[1], m, int(shuffle_here), self.seed + 2,
                                        D.ptr(self.d_theta), D.ptr(pts.rt), D.ptr(pts.choice), D.ptr(ll),
                                        D.ptr(self.d_choice_values), D.ptr(rows), D.stream_ptr())
         L.check(rc, "ssm_lan_assemble")
@@ -106,16 +107,25 @@ class LanKDELabeller:
         return rows
 
     def _append_negative(self, rows):
-        # negative-RT rows: rt ~ U(-1, 1e-4), random choice, label -66.77497 (resample_mixture_strategy.py:202-209)
+        """Add n_neg negative-RT rows per set: rt ~ U(-1, 1e-4), random choice, label -66.77497
+        (resample_mixture_strategy.py:202-209), then shuffle each set's rows
+        (resample_mixture_strategy.py:211-212)."""
         t = D.torch()
-        rng = np.random.default_rng(self.seed + 3)
-        n = self.n_sets * self.n_neg
-        neg = np.zeros((n, rows.shape[1]), np.float32)
-        neg[:, : self.theta.shape[1]] = np.repeat(self.theta, self.n_neg, axis=0)
-        neg[:, -3] = rng.uniform(-1.0, 0.0001, n)
-        neg[:, -2] = rng.choice(self.choice_values, n)
-        neg[:, -1] = NEG_RT_LL
-        return t.cat([rows, D.to_dev(neg)], dim=0)
+        gen = t.Generator(device=rows.device)
+        gen.manual_seed(self.seed + 3)
+        S, P, W = self.n_sets, self.theta.shape[1], rows.shape[1]
+        m = rows.shape[0] // S
+        neg = t.empty((S, self.n_neg, W), dtype=t.float32, device=rows.device)
+        neg[:, :, :P] = self.d_theta[:, None, :]
+        neg[:, :, P] = -1.0 + 1.0001 * t.rand((S, self.n_neg), generator=gen, device=rows.device)
+        pick = t.randint(0, len(self.choice_values), (S, self.n_neg), generator=gen, device=rows.device)
+        neg[:, :, P + 1] = self.d_choice_values[pick].float()
+        neg[:, :, P + 2] = NEG_RT_LL
+        out = t.cat([rows.view(S, m, W), neg], dim=1)
+        if self.shuffle:
+            order = t.rand((S, m + self.n_neg), generator=gen, device=rows.device).argsort(dim=1)
+            out = t.gather(out, 1, order[:, :, None].expand(-1, -1, W))
+        return out.reshape(S * (m + self.n_neg), W).contiguous()
 
     def output_dict(self, rows) -> dict:
         """The per-key arrays TrainingDataGenerator concatenates (lan_mlp.py:533-549)."""

@@ -20,6 +20,7 @@ from . import _device as D
 from . import _lib as L
 
 IN, H1, H2, H3, OUT = 7, 100, 100, 120, 1
+RT_MODES = {"raw": 0, "log": 1, "log_minus_t": 2}  # the rt input transform (xnorm[14])
 FLOOR = -66.774  # the KDE labels' log-likelihood floor (kde_class.py:188-195)
 LAYERS = [(H1, IN), (H2, H1), (H3, H2), (OUT, H3)]
 
@@ -61,7 +62,7 @@ class _Kernels:
         self.sz = sizes()
         self.frags = D.empty((self.sz["n_frag"],), t.bfloat16)
         self.bias = D.empty((self.sz["n_bias"],), t.float32)
-        self.xnorm = None  # device float32[14]: shift[7], scale[7], or None
+        self.xnorm = None  # device float32[16]: shift[7], scale[7], rt mode, t index; or None
 
     def pack(self, params) -> None:
         L.check(L.load().ssm_mlp_pack(D.ptr(params), D.ptr(self.frags), D.ptr(self.bias), D.stream_ptr()),
@@ -138,8 +139,12 @@ def mlp_apply(x, params, kern=None):
 class MLPSurrogate:
     """FP32 master weights + Adam state on the device; BF16-MFMA compute."""
 
-    def __init__(self, seed: int = 0, params: np.ndarray | None = None):
+    def __init__(self, seed: int = 0, params: np.ndarray | None = None, rt_mode: str = "log_minus_t",
+                 t_index: int = 3):
         t = D.require_gpu()
+        if rt_mode not in RT_MODES:
+            raise ValueError(f"rt_mode must be one of {sorted(RT_MODES)}")
+        self.rt_mode, self.t_index = rt_mode, int(t_index)
         self.kern = _Kernels()
         p = init_params(seed) if params is None else np.asarray(params, np.float32)
         if p.shape[0] != self.kern.sz["n_params"]:
@@ -186,27 +191,34 @@ class MLPSurrogate:
         rows = rows.contiguous()
         n = rows.shape[0]
         if self.kern.xnorm is None:
-            self.set_input_normalisation(rows[:, :IN])
+            self.set_input_normalisation(rows[:, :IN], self.rt_mode, self.t_index)
+        if batch % 32:
+            raise ValueError("batch must be a multiple of 32")
         steps_per_epoch = max(1, n // batch)
+        if n < batch:
+            raise ValueError(f"need at least one batch of {batch} rows, got {n}")
         total = epochs * steps_per_epoch
         gen = t.Generator(device=rows.device)
         gen.manual_seed(seed)
-        k = 0
+        lib = L.load()
+        ws_bytes = int(lib.ssm_mlp_train_workspace_bytes(batch))
+        ws = D.empty((ws_bytes // 4 + 4,), t.float32)
+        loss = D.empty((steps_per_epoch * batch,), t.float32)
         history = []
         for ep in range(epochs):
-            perm = t.randperm(n, device=rows.device, generator=gen)
+            perm = t.randperm(n, device=rows.device, generator=gen)[: steps_per_epoch * batch]
             shuffled = rows.index_select(0, perm)
             x_all = shuffled[:, :IN].contiguous()
             y_all = shuffled[:, IN].contiguous()
-            ep_loss = t.zeros((), device=rows.device)
-            for s in range(steps_per_epoch):
-                frac = k / max(1, total - 1)
-                lr_k = lr * (lr_final / lr) ** frac  # exponential decay
-                lo = s * batch
-                loss = self.train_step(x_all[lo:lo + batch], y_all[lo:lo + batch], lr=lr_k, delta=delta)
-                ep_loss += loss.mean()
-                k += 1
-            history.append(float(ep_loss.item()) / steps_per_epoch)
+            k = np.arange(ep * steps_per_epoch, (ep + 1) * steps_per_epoch, dtype=np.float64)
+            lrs = (lr * (lr_final / lr) ** (k / max(1, total - 1))).astype(np.float32)  # exponential decay
+            L.check(lib.ssm_mlp_train_steps(D.ptr(x_all), D.ptr(y_all), batch, steps_per_epoch, self.step,
+                                            lrs.ctypes.data, 0.9, 0.999, 1e-8, 0.0, float(delta), D.ptr(self.params),
+                                            D.ptr(self.m), D.ptr(self.v), D.ptr(self.kern.frags),
+                                            D.ptr(self.kern.bias), D.ptr(self.kern.xnorm), D.ptr(ws), ws_bytes,
+                                            D.ptr(loss), D.stream_ptr()), "ssm_mlp_train_steps")
+            self.step += steps_per_epoch
+            history.append(float(loss.sum().item()) / (steps_per_epoch * batch))  # mean per-row Huber loss
             if log:
                 log(f"epoch {ep}: huber {history[-1]:.5f}")
         xs = rows[:, :IN]
@@ -219,17 +231,26 @@ class MLPSurrogate:
             self.val_mae = float(err[above].mean().item()) if bool(above.any()) else self.val_mae_all
         return history
 
-    def set_input_normalisation(self, x, log_rt: bool = True) -> None:
-        """Standardise the inputs: the network sees (x' - mean) / std (std floored at 1e-6), where
-        x' has rt replaced by log(max(rt, 1e-4)) when `log_rt` (the KDE lives on log rt)."""
+    def set_input_normalisation(self, x, rt_mode: str = "log_minus_t", t_index: int = 3) -> None:
+        """Standardise the inputs: the network sees (x' - mean) / std (std floored at 1e-6).
+        x' is x with rt replaced by log(max(u, 1e-4)), u = rt ("log": the KDE labels live on
+        log rt) or u = rt - x[t_index] ("log_minus_t": the time since the non-decision time,
+        where the density has its onset); "raw" leaves rt as it is."""
         t = D.torch()
+        if rt_mode not in RT_MODES:
+            raise ValueError(f"rt_mode must be one of {sorted(RT_MODES)}")
+        if not 0 <= int(t_index) < 5:
+            raise ValueError("t_index must name one of the 5 parameter inputs")
         xx = x.float().clone()
-        if log_rt:
+        if rt_mode == "log_minus_t":
+            xx[:, 5] = xx[:, 5] - xx[:, int(t_index)]
+        if rt_mode != "raw":
             xx[:, 5] = t.log(t.clamp_min(xx[:, 5], 1e-4))
         mu = xx.mean(0)
         sd = xx.std(0).clamp_min(1e-6)
-        flag = t.tensor([1.0 if log_rt else 0.0], device=x.device)
-        self.kern.xnorm = t.cat([mu, 1.0 / sd, flag]).float().contiguous()
+        tail = t.tensor([float(RT_MODES[rt_mode]), float(t_index)], device=x.device)
+        self.kern.xnorm = t.cat([mu, 1.0 / sd, tail]).float().contiguous()
+        self.rt_mode, self.t_index = rt_mode, int(t_index)
 
     # ---- checkpoint ----------------------------------------------------------
     def state_dict(self) -> dict:
@@ -263,11 +284,13 @@ class MLPSurrogate:
                     super().__init__()
                     self.register_buffer("shift", xn[:IN].clone())
                     self.register_buffer("scale", xn[IN:2 * IN].clone())
-                    self.log_rt = bool(xn.shape[0] > 2 * IN and xn[2 * IN] != 0)
+                    self.mode = int(xn[2 * IN]) if xn.shape[0] > 2 * IN else 0
+                    self.ti = int(xn[2 * IN + 1]) if xn.shape[0] > 2 * IN + 1 else 0
 
                 def forward(self, x):
-                    if self.log_rt:
-                        x = torch.cat([x[:, :5], torch.log(torch.clamp_min(x[:, 5:6], 1e-4)), x[:, 6:]], dim=1)
+                    if self.mode:
+                        u = x[:, 5:6] - x[:, self.ti:self.ti + 1] if self.mode == 2 else x[:, 5:6]
+                        x = torch.cat([x[:, :5], torch.log(torch.clamp_min(u, 1e-4)), x[:, 6:]], dim=1)
                     return (x - self.shift) * self.scale
 
             layers.append(Standardise())

@@ -161,14 +161,20 @@ def test_lan_rows_small_batch_against_oracle(torch_gpu):
     for k in range(6):
         blk = rows[k * 1000:(k + 1) * 1000]
         assert np.allclose(blk[:, :5], theta[k][None, :])
+        # the reference mixture: 100 negative-RT rows per set with the fixed label
+        neg = blk[:, 5] < 1e-4
+        assert neg.sum() == 100
+        assert np.all(blk[neg, 7] == np.float32(-66.77497)) and np.all(blk[neg, 5] >= -1.0)
+        assert set(np.unique(blk[neg, 6])) <= {-1.0, 1.0}
+        pos = blk[~neg]
         fit = kde_ref.KDEFit(sim_rts[offs[k]:offs[k + 1]], sim_ch[offs[k]:offs[k + 1]], [-1, 1])
-        ref = fit.eval(blk[:, 5].astype(np.float64), blk[:, 6])
+        ref = fit.eval(pos[:, 5].astype(np.float64), pos[:, 6])
         above = ref > LB + 1
-        assert np.max(np.abs(blk[above, 7] - ref[above])) < 2e-3
-        assert np.all(blk[:, 7] >= LB)
+        assert np.max(np.abs(pos[above, 7] - ref[above])) < 2e-3
+        assert np.all(pos[:, 7] >= LB)  # the KDE floor; the negative-RT label lies just below it
         assert set(np.unique(blk[:, 6])) <= {-1.0, 1.0}
-        # 200 uniform rows in [1e-4, 20] plus 800 KDE rows
-        assert np.sum(blk[:, 5] > 0) == 1000
+        # 100 uniform rows in [1e-4, 20] plus 800 KDE rows
+        assert np.sum(blk[:, 5] > 0) == 900 and np.all(pos[:, 5] <= 20.0)
 
 
 def test_lan_rows_shuffle_is_a_permutation(torch_gpu):

@@ -104,22 +104,35 @@ def test_autograd_backward_matches_oracle_and_torch(torch_gpu):
         assert _rel(dp[sl], tg[sl]) < 6e-2, name
 
 
-def test_standardised_inputs_forward_and_gradients(torch_gpu):
+@pytest.mark.parametrize("rt_mode", ["log", "log_minus_t"])
+def test_standardised_inputs_forward_and_gradients(torch_gpu, rt_mode):
     from ssmhip import _device as D
     from ssmhip.surrogate import MLPSurrogate
 
     net = MLPSurrogate(seed=6)
     x_np = _inputs(2000, seed=6)
     x_np[:7, 5] = [0.0, -0.5, 5e-5, 1.5e-4, 2e-4, 0.01, 19.0]  # rt below, near and above the log floor
-    net.set_input_normalisation(D.to_dev(x_np), log_rt=True)
+    x_np[7:12, 5] = x_np[7:12, 3] + np.array([-0.1, 0.0, 5e-5, 3e-4, 0.2], np.float32)  # around rt = t
+    net.set_input_normalisation(D.to_dev(x_np), rt_mode=rt_mode, t_index=3)
     xn = net.kern.xnorm.cpu().numpy()
     p = net.params.cpu().numpy()
     y = net(D.to_dev(x_np)).cpu().numpy()
     yb, _ = mlp_ref.forward(p, x_np, bf16=True, xnorm=xn)
     assert np.max(np.abs(y - yb)) < 5e-3
+    yt = net.torch_reference().cuda()(D.to_dev(x_np))[:, 0].detach().cpu().numpy()
+    assert _rel(y, yt) < 2e-2
     g = net.input_gradient(D.to_dev(x_np)).cpu().numpy()
     gb, _ = mlp_ref.backward(p, x_np, np.ones(2000), bf16=True, xnorm=xn)
-    assert _rel(g, gb) < 1e-2
+    assert _rel(g[:, :7], gb) < 1e-2
+    # the chain rule through the rt transform against central differences of the float64 net
+    rows = np.arange(12, 28)
+    h = 1e-3
+    for j in (3, 5):
+        e = np.zeros_like(x_np[rows], dtype=np.float64)
+        e[:, j] = h
+        fp, _ = mlp_ref.forward(p, (x_np[rows] + e).astype(np.float32), xnorm=xn)
+        fm, _ = mlp_ref.forward(p, (x_np[rows] - e).astype(np.float32), xnorm=xn)
+        assert _rel(g[rows, j], (fp - fm) / (2 * h)) < 3e-2, j
 
 
 def test_theta_gradient_matches_finite_differences(torch_gpu):
@@ -158,3 +171,42 @@ def test_training_reduces_huber_loss_and_checkpoint_roundtrip(torch_gpu):
     other = MLPSurrogate(seed=99)
     other.load_state_dict(sd)
     assert torch.equal(other(rows[:100, :7].contiguous()), net(rows[:100, :7].contiguous()))
+
+
+def test_train_steps_match_single_steps(torch_gpu):
+    """The host-enqueued run of Adam steps (weight gradient summed over batch slices) against
+    the same steps taken one call at a time (one-slice weight gradient)."""
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+    from ssmhip.surrogate import MLPSurrogate
+
+    torch = torch_gpu
+    batch, steps = 4096, 6
+    x = _inputs(batch * steps, seed=8)
+    y = (np.cos(x[:, 0]) - 0.2 * x[:, 1] * x[:, 6]).astype(np.float32)
+    dx, dy = D.to_dev(x), D.to_dev(y)
+    lrs = np.geomspace(3e-3, 1e-3, steps).astype(np.float32)
+    a = MLPSurrogate(seed=8)
+    for i in range(steps):
+        a.train_step(dx[i * batch:(i + 1) * batch], dy[i * batch:(i + 1) * batch], lr=float(lrs[i]))
+    b = MLPSurrogate(seed=8)
+    lib = L.load()
+    ws_bytes = int(lib.ssm_mlp_train_workspace_bytes(batch))
+    assert int(lib.ssm_mlp_train_workspace_bytes(100)) == -1
+    ws = D.empty((ws_bytes // 4 + 4,), torch.float32)
+    loss = D.empty((batch * steps,), torch.float32)
+    L.check(lib.ssm_mlp_train_steps(D.ptr(dx), D.ptr(dy), batch, steps, 0, lrs.ctypes.data, 0.9, 0.999, 1e-8, 0.0,
+                                    1.0, D.ptr(b.params), D.ptr(b.m), D.ptr(b.v), D.ptr(b.kern.frags),
+                                    D.ptr(b.kern.bias), None, D.ptr(ws), ws_bytes, D.ptr(loss), D.stream_ptr()))
+    torch.cuda.synchronize()
+    pa, pb = a.params.cpu().numpy(), b.params.cpu().numpy()
+    p0 = MLPSurrogate(seed=8).params.cpu().numpy()
+    assert _rel(pb - p0, pa - p0) < 1e-3  # the updates agree to float32 summation order
+    ya = a(dx[:1000]).cpu().numpy()
+    yb = b(dx[:1000]).cpu().numpy()
+    assert np.max(np.abs(ya - yb)) < 1e-3
+    assert np.isfinite(loss.cpu().numpy()).all()
+    with pytest.raises(ValueError):
+        L.check(lib.ssm_mlp_train_steps(D.ptr(dx), D.ptr(dy), 100, 1, 0, lrs.ctypes.data, 0.9, 0.999, 1e-8, 0.0,
+                                        1.0, D.ptr(b.params), D.ptr(b.m), D.ptr(b.v), D.ptr(b.kern.frags),
+                                        D.ptr(b.kern.bias), None, D.ptr(ws), ws_bytes, None, D.stream_ptr()))

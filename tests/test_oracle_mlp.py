@@ -1,6 +1,7 @@
 """Pins of the MLP oracle (CPU): PyTorch autograd of the same network and finite differences."""
 
 import numpy as np
+import pytest
 import torch
 
 from oracle import mlp_ref
@@ -41,6 +42,28 @@ def test_forward_and_backward_match_torch_autograd():
     assert np.max(np.abs(xt.grad.numpy() - dx)) < 1e-12
     tg = np.concatenate([np.concatenate([m.weight.grad.numpy().reshape(-1), m.bias.grad.numpy()]) for m in lin])
     assert np.max(np.abs(tg - dp)) < 1e-10
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_rt_transform_forward_and_chain_rule_match_torch_autograd(mode):
+    p = _params(5)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0.0, 2.0, size=(64, 7)).astype(np.float32)
+    x[:8, 5] = x[:8, 3] + np.array([-0.5, -1e-3, 0.0, 5e-5, 2e-4, 1e-2, 0.3, 3.0], np.float32)
+    xn = np.concatenate([rng.normal(size=7), rng.uniform(0.5, 2.0, 7), [mode, 3]]).astype(np.float32)
+    gy = rng.normal(size=64)
+    y, _ = mlp_ref.forward(p, x, xnorm=xn)
+    dx, _ = mlp_ref.backward(p, x, gy, xnorm=xn)
+    net, _ = _torch_net(p)
+    xt = torch.from_numpy(x.astype(np.float64)).requires_grad_(True)
+    u = xt[:, 5] - xt[:, 3] if mode == 2 else xt[:, 5]
+    feat = torch.log(torch.clamp_min(u, 1e-4))
+    xp = torch.cat([xt[:, :5], feat[:, None], xt[:, 6:]], dim=1)
+    sh, sc = torch.from_numpy(xn[:7].astype(np.float64)), torch.from_numpy(xn[7:14].astype(np.float64))
+    yt = net((xp - sh) * sc)[:, 0]
+    assert np.max(np.abs(yt.detach().numpy() - y)) < 1e-5  # float32 standardisation in the oracle
+    (yt * torch.from_numpy(gy)).sum().backward()
+    assert np.max(np.abs(xt.grad.numpy() - dx)) < 1e-3 * max(1.0, np.abs(dx).max())
 
 
 def test_backward_matches_central_differences():
