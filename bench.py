@@ -362,6 +362,19 @@ def main() -> int:
     return 0
 
 
+def em_traffic_bytes():
+    """HBM bytes per headline launch from the committed counter summary (two rocprofv3 --pmc runs)."""
+    path = REPO / "profiles" / "em_angle_batched4096_counters.md"
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def headline_leg(args, torch, batched, roofline) -> dict:
     """Batched angle-DDM, 4096 ragged sets, one persistent hipGraph launch per step."""
     from ssmhip.em import LaunchParams
@@ -431,7 +444,10 @@ def headline_leg(args, torch, batched, roofline) -> dict:
             "peak": None if ideal is None else ideal / 1e9,
             "unit": "G EM-steps/s",
             "frac": None if ideal is None else steps_per_s / ideal,
-            "traffic": None,
+            "traffic": em_traffic_bytes(),
+            "traffic_basis": "HBM bytes per launch: rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a "
+                             "1 GiB copy in the same runs), profiles/em_angle_batched4096_counters.md; algorithmic "
+                             f"bytes {int(traj) * 5 + int(args.n_sets) * 264}",
             "peak_basis": "measured: branch-free Philox4x32-10 + Box-Muller + angle-step loop at full occupancy "
                           "(VALU issue bound; no MFMA, ~5 B of HBM traffic per trajectory)",
         },
