@@ -93,65 +93,129 @@ struct FitArgs {
   int32_t choice_bytes;
 };
 
+// One block per set, three passes, every load and store coalesced:
+//  1. count per choice (draws with that choice, and those not omitted) by block strides;
+//  2. compact log(rt) per choice in sample order: each 1024-sample tile places its kept
+//     samples by wave ballots plus a prefix over the tile's waves, behind a running
+//     cursor per choice (double-buffered counts, two barriers per tile);
+//  3. mean and population variance of each choice's contiguous run, summed in a fixed
+//     order (deterministic), then the Silverman bandwidth.
 __global__ void __launch_bounds__(kFitThreads) kde_fit_kernel(FitArgs A) {
-  __shared__ float redf[kFitThreads / 64 + 1];
-  __shared__ int redi[kFitThreads / 64 + 1];
+  constexpr int NW = kFitThreads / 64;
+  constexpr int MC = SSM_MAX_PARTICLES;
+  __shared__ float redf[NW + 1];
+  __shared__ int redi[NW + 1];
+  __shared__ int wave_cnt[2][NW][MC];
+  __shared__ int run[MC];
   const int set = blockIdx.x;
   const int64_t lo = A.offsets[set], hi = A.offsets[set + 1];
-  const int64_t n = hi - lo;
-  const int64_t chunk = (n + kFitThreads - 1) / kFitThreads;
-  const int64_t b = lo + (int64_t)threadIdx.x * chunk;
-  const int64_t e = (b + chunk < hi) ? b + chunk : hi;
-  int base = 0;
-  for (int c = 0; c < A.n_choices; ++c) {
-    const int cv = A.choice_values[c];
-    int mine = 0, mine_all = 0;
-    for (int64_t i = b; i < e; ++i) {
-      if (load_choice(A.choices, A.choice_bytes, i) == cv) {
-        ++mine_all;
-        if (A.rts[i] != kOmission) ++mine;
-      }
+  const int C = A.n_choices;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int cv[MC];
+#pragma unroll
+  for (int c = 0; c < MC; ++c) cv[c] = c < C ? A.choice_values[c] : 0x7fffffff;
+  auto choice_index = [&](int ch) {
+    int k = -1;
+#pragma unroll
+    for (int c = MC - 1; c >= 0; --c)
+      if (c < C && ch == cv[c]) k = c;  // first match
+    return k;
+  };
+  // pass 1
+  int kept[MC], all[MC];
+#pragma unroll
+  for (int c = 0; c < MC; ++c) kept[c] = all[c] = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kFitThreads) {
+    const int k = choice_index(load_choice(A.choices, A.choice_bytes, i));
+    const int keep = A.rts[i] != kOmission ? 1 : 0;
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      all[c] += (k == c) ? 1 : 0;
+      kept[c] += (k == c) ? keep : 0;
     }
-    int tot = 0, tot_all = 0;
-    const int off = block_scan_excl<kFitThreads>(mine, redi, &tot);
-    block_scan_excl<kFitThreads>(mine_all, redi, &tot_all);
-    // compact log(rt) in sample order (deterministic) and accumulate the mean
+  }
+  int tot[MC], tot_all[MC], base[MC];
+  int acc = 0;
+#pragma unroll
+  for (int c = 0; c < MC; ++c) {
+    tot[c] = tot_all[c] = 0;
+    if (c < C) {
+      block_scan_excl<kFitThreads>(kept[c], redi, &tot[c]);
+      block_scan_excl<kFitThreads>(all[c], redi, &tot_all[c]);
+    }
+    base[c] = acc;
+    acc += tot[c];
+  }
+  // pass 2
+  if (threadIdx.x < MC) run[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+  const int64_t ntiles = (hi - lo + kFitThreads - 1) / kFitThreads;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int par = (int)(t & 1);
+    const int64_t i = lo + t * kFitThreads + threadIdx.x;
+    int k = -1;
+    float r = 0.f;
+    if (i < hi) {
+      r = A.rts[i];
+      if (r != kOmission) k = choice_index(load_choice(A.choices, A.choice_bytes, i));
+    }
+    int inwave = 0;
+    for (int c = 0; c < C; ++c) {
+      const uint64_t m = __ballot(k == c);
+      if (k == c) inwave = __popcll(m & below);
+      if (l == 0) wave_cnt[par][w][c] = __popcll(m);
+    }
+    __syncthreads();
+    if (k >= 0) {
+      int pos = run[k] + inwave;
+      for (int v = 0; v < w; ++v) pos += wave_cnt[par][v][k];
+      int b = 0;
+#pragma unroll
+      for (int c = 0; c < MC; ++c) b = (k == c) ? base[c] : b;
+      A.samples[lo + b + pos] = logf(r);
+    }
+    __syncthreads();
+    if (threadIdx.x < C) {
+      int sum = 0;
+      for (int v = 0; v < NW; ++v) sum += wave_cnt[par][v][threadIdx.x];
+      run[threadIdx.x] += sum;
+    }
+  }
+  __syncthreads();
+  // pass 3
+#pragma unroll 1
+  for (int c = 0; c < C; ++c) {
+    int tc = 0, tac = 0, bc = 0;
+#pragma unroll
+    for (int q = 0; q < MC; ++q) {
+      if (q == c) { tc = tot[q]; tac = tot_all[q]; bc = base[q]; }
+    }
+    const float* sp = A.samples + lo + bc;
     float s = 0.f;
-    int w = 0;
-    for (int64_t i = b; i < e; ++i) {
-      if (load_choice(A.choices, A.choice_bytes, i) == cv) {
-        const float r = A.rts[i];
-        if (r != kOmission) {
-          const float lr = logf(r);
-          A.samples[lo + base + off + w] = lr;
-          s += lr;
-          ++w;
-        }
-      }
+    for (int j = threadIdx.x; j < tc; j += kFitThreads) s += sp[j];
+    const float mean = block_sum<kFitThreads>(s, redf) / (float)(tc > 0 ? tc : 1);
+    float q2 = 0.f;
+    for (int j = threadIdx.x; j < tc; j += kFitThreads) {
+      const float d = sp[j] - mean;
+      q2 += d * d;
     }
-    const float mean = block_sum<kFitThreads>(s, redf) / (float)(tot > 0 ? tot : 1);
-    float q = 0.f;
-    for (int j = 0; j < w; ++j) {
-      const float d = A.samples[lo + base + off + j] - mean;
-      q += d * d;
-    }
-    const float ss = block_sum<kFitThreads>(q, redf);
+    const float ss = block_sum<kFitThreads>(q2, redf);
     if (threadIdx.x == 0) {
-      const int g = set * A.n_choices + c;
+      const int g = set * C + c;
       float h = 0.f;
-      if (tot > 1) {
-        float sd = sqrtf(ss / (float)tot);  // np.std: population
+      if (tc > 1) {
+        float sd = sqrtf(ss / (float)tc);  // np.std: population
         if (sd < 1e-3f) sd = 1e-3f;
-        h = powf(4.0f / (3.0f * (float)tot), 0.2f) * sd;
-      } else if (tot == 1) {
+        h = powf(4.0f / (3.0f * (float)tc), 0.2f) * sd;
+      } else if (tc == 1) {
         h = powf(4.0f / 3.0f, 0.2f) * 10.0f;
       }
-      A.kde_count[g] = tot;
-      A.choice_total[g] = tot_all;
+      A.kde_count[g] = tc;
+      A.choice_total[g] = tac;
       A.bandwidth[g] = h;
-      A.kde_start[g] = lo + base;
+      A.kde_start[g] = lo + bc;
     }
-    base += tot;
   }
 }
 

@@ -185,8 +185,10 @@ class MLPSurrogate:
         return loss
 
     def fit(self, rows, *, epochs=10, batch=8192, lr=1e-3, lr_final=1e-4, delta=1.0, seed=0, val_rows=None,
-            log=None):
-        """Train on LAN rows [N, 8] (device tensor: 7 inputs + loglik label)."""
+            label_lower_bound=None, log=None):
+        """Train on LAN rows [N, 8] (device tensor: 7 inputs + loglik label). With
+        `label_lower_bound` the training targets are clipped from below (e.g. log(1e-7)): the
+        floor rows (-66.77) then no longer pull the fit away from the likelihood's bulk."""
         t = D.torch()
         rows = rows.contiguous()
         n = rows.shape[0]
@@ -210,6 +212,8 @@ class MLPSurrogate:
             shuffled = rows.index_select(0, perm)
             x_all = shuffled[:, :IN].contiguous()
             y_all = shuffled[:, IN].contiguous()
+            if label_lower_bound is not None:
+                y_all.clamp_(min=float(label_lower_bound))
             k = np.arange(ep * steps_per_epoch, (ep + 1) * steps_per_epoch, dtype=np.float64)
             lrs = (lr * (lr_final / lr) ** (k / max(1, total - 1))).astype(np.float32)  # exponential decay
             L.check(lib.ssm_mlp_train_steps(D.ptr(x_all), D.ptr(y_all), batch, steps_per_epoch, self.step,
