@@ -322,7 +322,7 @@ def main() -> int:
     ap.add_argument("--legs", type=str, default="headline,config3,config4,config5",
                     help="comma list of legs to run (config5 needs config4); without 'headline' the "
                          "JSON line carries no headline value (for profiling one leg)")
-    ap.add_argument("--surrogate-epochs", type=int, default=60)
+    ap.add_argument("--surrogate-epochs", type=int, default=120)
     ap.add_argument("--surrogate-sets", type=int, default=32768, help="parameter sets of KDE rows to train on")
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000, help="simulations per set for the labels")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -409,7 +409,10 @@ def headline_leg(args, torch, batched, roofline) -> dict:
     steps_per_s = steps_per_launch / (ms / 1e3)
     # roof: the same Philox + Box–Muller + angle-step loop, branch-free, at full occupancy
     ideal = None if args.no_probe else roofline.probe_rate("em_block", iters=2048)
-    log(f"{ms:.3f} ms/launch, {value:.4e} traj/s, {steps_per_launch / (ms / 1e3):.4e} EM steps/s (wall {wall:.2f} s)")
+    slots = sim.lane_slots()
+    util = steps_per_launch / slots if slots else None
+    log(f"{ms:.3f} ms/launch, {value:.4e} traj/s, {steps_per_launch / (ms / 1e3):.4e} EM steps/s "
+        f"(wall {wall:.2f} s, lane utilisation {util})")
 
     # sanity: choices are +-1 and rts finite or the omission sentinel
     ch = sim.choices.cpu().numpy()
@@ -436,7 +439,9 @@ def headline_leg(args, torch, batched, roofline) -> dict:
             "trajectories": int(traj),
             "em_steps_per_launch": int(steps_per_launch),
             "em_steps_per_s": steps_per_launch / (ms / 1e3),
-            "launch": {"block": lp.block_size, "blocks_per_cu": lp.blocks_per_cu, "refill_min": lp.refill_min},
+            "launch": {"block": lp.block_size, "blocks_per_cu": lp.blocks_per_cu, "refill_min": lp.refill_min,
+                       "queue_block": lp.queue_block},
+            "lane_utilisation": util,
         },
         "roofline": {
             "bound": "valu",

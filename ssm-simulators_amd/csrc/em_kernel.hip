@@ -3,8 +3,10 @@
 // One trajectory per lane. Each wavefront pulls work from a global queue over
 // the flattened (set, sample) index space; lanes whose walk has ended hold
 // their result until enough lanes of the wave are idle, then the wave writes
-// those results in one store instruction and refills the idle lanes with one
-// atomic for the whole wave.  The walk follows the sequential loops of
+// those results in one store instruction and refills the idle lanes from a
+// wave-local index range. The range is topped up by one atomic on the global
+// head for up to `queue_block` indices (fewer as the queue drains), so the
+// single head address sees one atomic per several refills.  The walk follows the sequential loops of
 //   ddm_flexbound  src/cssm/ddm_models.pyx:696-744
 //   race_model     src/cssm/race_models.pyx:417-464
 //   lca            src/cssm/race_models.pyx:825-879
@@ -28,7 +30,7 @@ struct EmArgs {
   const float* replay_uniforms;
   float* rts;
   void* choices;
-  unsigned long long* ws;  // [0] queue head, [1] executed steps
+  unsigned long long* ws;  // [0] queue head, [1] executed steps, [2] lane-step slots issued
   int64_t total;
   int64_t replay_stride;
   int32_t n_sets;
@@ -38,6 +40,7 @@ struct EmArgs {
   int32_t out_layout;
   int32_t choice_bytes;
   int32_t refill_min;
+  int32_t queue_block;  // largest index range a wave claims from the global queue at once
   float dt;
   uint32_t k0, k1;
 };
@@ -92,7 +95,12 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   float a = 0.f, b1 = 0.f, b2 = 0.f, t0 = 0.f, deadline = 0.f, dl_tmp = 0.f, leak = 0.f, inh = 0.f;
   const float* row = nullptr;
   unsigned long long my_steps = 0;
-  bool exhausted = false;  // wave-uniform
+  unsigned long long iters = 0;  // wave-uniform: step iterations issued
+  bool exhausted = false;        // wave-uniform: no work left for this wave
+  bool queue_done = false;       // wave-uniform: the global queue is drained
+  int64_t q_lo = 0, q_hi = 0;    // wave-uniform local range [q_lo, q_hi) of unclaimed indices
+  int q_k = 0;                   // wave-uniform: set of q_lo (valid while q_lo < q_hi)
+  const int64_t waves_total = ((int64_t)gridDim.x * blockDim.x) >> 6;
 
 #pragma unroll
   for (int j = 0; j < MAXP; ++j) { x[j] = 0.f; vdt[j] = 0.f; sst[j] = 0.f; vraw[j] = 0.f; }
@@ -145,18 +153,54 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
       if (!exhausted) {
         const uint64_t want = __ballot(state == kEmpty);
         const int cnt = __popcll(want);
-        const int leader = __ffsll((unsigned long long)want) - 1;
-        unsigned long long base = 0;
-        if (lane_id() == leader) base = atomicAdd(A.ws, (unsigned long long)cnt);
-        base = readlane_u64(base, leader);
-        if (base + (unsigned long long)cnt >= (unsigned long long)A.total) exhausted = true;
-        // set of the wave's first new index (uniform search), then a short per-lane walk forward
-        const int kbase = (base < (unsigned long long)A.total) ? find_set(A.offsets, A.n_sets, (int64_t)base) : 0;
+        // the first r ranked lanes take the wave's local range, the rest a fresh block
+        const int64_t lo_old = q_lo;
+        const int k_old = q_k;
+        const int64_t r = q_hi - q_lo;
+        const int64_t from_local = r < cnt ? r : cnt;
+        int64_t nb_base = 0, nb_end = 0;
+        int nb_k = 0;
+        if (r < cnt && !queue_done) {
+          // block size: queue_block while the queue is long, shrinking to one refill near its end
+          int64_t blk = (int64_t)A.queue_block;
+          const int64_t left = A.total - q_hi;
+          if (left / (waves_total * 8) < blk) blk = left / (waves_total * 8);
+          if (blk < cnt - from_local) blk = cnt - from_local;
+          const int leader = __ffsll((unsigned long long)want) - 1;
+          unsigned long long b = 0;
+          if (lane_id() == leader) b = atomicAdd(A.ws, (unsigned long long)blk);
+          nb_base = (int64_t)readlane_u64(b, leader);
+          if (nb_base + blk >= A.total) queue_done = true;
+          nb_end = nb_base + blk < A.total ? nb_base + blk : A.total;
+          if (nb_base < nb_end) nb_k = find_set(A.offsets, A.n_sets, nb_base);
+        }
+        const int64_t need_new = cnt - from_local;
+        if (r >= cnt) {
+          q_lo = lo_old + cnt;
+        } else if (nb_base < nb_end) {
+          q_lo = nb_base + need_new < nb_end ? nb_base + need_new : nb_end;
+          q_hi = nb_end;
+          q_k = nb_k;
+        } else {
+          q_lo = q_hi;
+        }
+        if (q_lo < q_hi)
+          while (q_k < A.n_sets - 1 && A.offsets[q_k + 1] <= q_lo) ++q_k;  // keep q_k = set of q_lo
+        if (q_lo >= q_hi && queue_done) exhausted = true;
         if (state == kEmpty) {
-          const int64_t gi = (int64_t)base + lanes_below(want);
+          const int j = lanes_below(want);
+          int64_t gi = A.total;
+          int kk = 0;
+          if (j < from_local) {
+            gi = lo_old + j;
+            kk = k_old;
+          } else if (nb_base + (j - from_local) < nb_end) {
+            gi = nb_base + (j - from_local);
+            kk = nb_k;
+          }
           if (gi < A.total) {
             g = gi;
-            k = kbase;
+            k = kk;
             while (k < A.n_sets - 1 && A.offsets[k + 1] <= gi) ++k;
             n = (int)(gi - A.offsets[k]);
             const float* sp = A.set_params + (int64_t)k * SSM_SET_STRIDE;
@@ -202,6 +246,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     }
 
     // ---------------- advance running lanes ----------------
+    ++iters;
     if (state == kRunning) {
       if constexpr (MODEL == SSM_MODEL_DDM) {
         // four steps per Philox block; every walk starts block-aligned
@@ -315,10 +360,13 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     }
   }
 
-  // one atomic per wavefront for the executed-step counter
+  // one atomic per wavefront for each counter
   unsigned long long s = my_steps;
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane_id() == 0 && s) atomicAdd(A.ws + 1, s);
+  if (lane_id() == 0) {
+    if (s) atomicAdd(A.ws + 1, s);
+    atomicAdd(A.ws + 2, iters * (MODEL == SSM_MODEL_DDM ? 256ull : 64ull));
+  }
 }
 
 __global__ void ws_reset_kernel(unsigned long long* ws) {
@@ -385,7 +433,7 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
 int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const float* set_params,
                    const int64_t* offsets, const float* table, const float* rn, const float* ru,
                    float* rts, void* choices, uint64_t* ws, hipStream_t stream) {
-  int block = 256, per_cu = 8, refill = 16;
+  int block = 256, per_cu = 8, refill = 8;
   if (L) {
     if (L->block_size) block = L->block_size;
     if (L->blocks_per_cu) per_cu = L->blocks_per_cu;
@@ -394,6 +442,8 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   if (block < 64 || block > 1024 || block % 64) return fail(SSM_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
   if (per_cu < 1 || per_cu > 64) return fail(SSM_EINVAL, "blocks_per_cu must be in [1, 64]");
   if (refill < 1 || refill > 64) return fail(SSM_EINVAL, "refill_min must be in [1, 64]");
+  if (L && (L->queue_block < 0 || L->queue_block > 65536))
+    return fail(SSM_EINVAL, "queue_block must be in [0, 65536]");
   hipLaunchKernelGGL(ws_reset_kernel, dim3(1), dim3(64), 0, stream, (unsigned long long*)ws);
   SSM_HIP(hipGetLastError());
   if (P->n_total == 0) return SSM_OK;
@@ -416,6 +466,7 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   A.out_layout = P->out_layout;
   A.choice_bytes = P->choice_bytes;
   A.refill_min = refill;
+  A.queue_block = (L && L->queue_block) ? L->queue_block : 64;
   A.dt = P->dt;
   A.k0 = (uint32_t)P->seed;
   A.k1 = (uint32_t)(P->seed >> 32);

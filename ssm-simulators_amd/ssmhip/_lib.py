@@ -53,7 +53,7 @@ class LaunchConfig(C.Structure):
         ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("refill_min", C.c_int32),
-        ("pad0", C.c_int32),
+        ("queue_block", C.c_int32),
     ]
 
 

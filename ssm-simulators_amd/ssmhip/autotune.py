@@ -1,8 +1,9 @@
 """Launch-shape autotuner for the persistent EM kernel.
 
-Chooses (block_size, blocks_per_cu, refill_min) — threads per block, resident
-blocks per CU in the persistent grid, and how many idle lanes a wavefront
-tolerates before it refills them from the work queue — by timing candidate
+Chooses (block_size, blocks_per_cu, refill_min, queue_block) — threads per
+block, resident blocks per CU in the persistent grid, how many idle lanes a
+wavefront tolerates before it refills them, and the largest index range a
+wavefront claims from the global queue per atomic — by timing candidate
 shapes on a job the caller supplies (BatchedSimulation passes a bounded
 sample of its own sets, so the sample keeps the workload's mix of short and
 long walks). Results are cached per (device, model, boundary, particles) in memory
@@ -21,8 +22,8 @@ from . import _device as D
 from .em import LaunchParams
 
 DEFAULT_CANDIDATES = [
-    (256, 8, 8), (256, 8, 16), (256, 8, 32), (512, 4, 4), (512, 4, 8), (512, 4, 16), (512, 4, 32),
-    (1024, 2, 8), (1024, 2, 16), (128, 16, 8), (512, 8, 8), (256, 16, 8),
+    (256, 8, 8, 64), (256, 8, 4, 64), (256, 8, 2, 64), (256, 8, 4, 32), (256, 8, 8, 32), (256, 8, 8, 128),
+    (256, 8, 16, 64), (512, 4, 8, 64), (512, 4, 4, 64), (128, 16, 8, 64), (1024, 2, 8, 64), (256, 8, 4, 128),
 ]
 
 _memo: dict[str, LaunchParams] = {}
@@ -36,7 +37,7 @@ def _key(job) -> str:
     t = D.torch()
     name = t.cuda.get_device_name() if t.cuda.is_available() else "cpu"
     p = job.prob
-    return f"{name}|m{p.model}|b{p.boundary}|p{p.n_particles}|r{int(p.replay_stride > 0)}"
+    return f"{name}|m{p.model}|b{p.boundary}|p{p.n_particles}|r{int(p.replay_stride > 0)}|q"
 
 
 def _load_file() -> dict:
@@ -48,7 +49,7 @@ def _load_file() -> dict:
 
 def _store(key: str, lp: LaunchParams) -> None:
     data = _load_file()
-    data[key] = [lp.block_size, lp.blocks_per_cu, lp.refill_min]
+    data[key] = [lp.block_size, lp.blocks_per_cu, lp.refill_min, lp.queue_block]
     try:
         path = _cache_path()
         path.parent.mkdir(parents=True, exist_ok=True)

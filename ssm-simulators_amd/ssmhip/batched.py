@@ -148,6 +148,9 @@ class BatchedSimulation:
     def steps(self) -> int:
         return self.job.steps()
 
+    def lane_slots(self) -> int:
+        return self.job.lane_slots()
+
     def per_set(self, k: int):
         lo, hi = int(self.offsets[k]), int(self.offsets[k + 1])
         return self.rts[lo:hi].cpu().numpy(), self.choices[lo:hi].cpu().numpy()

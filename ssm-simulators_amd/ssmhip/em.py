@@ -113,10 +113,11 @@ def accumulator_records(v, z, t, s, deadline, max_t, delta_t, code, bparams, g=N
 class LaunchParams:
     block_size: int = 256
     blocks_per_cu: int = 8
-    refill_min: int = 16
+    refill_min: int = 8
+    queue_block: int = 64  # indices a wave claims per global atomic; 0 = the library default (64)
 
     def as_struct(self) -> L.LaunchConfig:
-        return L.LaunchConfig(self.block_size, self.blocks_per_cu, self.refill_min, 0)
+        return L.LaunchConfig(self.block_size, self.blocks_per_cu, self.refill_min, self.queue_block)
 
 
 @dataclass
@@ -153,6 +154,11 @@ class EmJob:
     def steps(self) -> int:
         """EM steps executed by the last completed launch (device counter)."""
         return int(self.workspace[1].item())
+
+    def lane_slots(self) -> int:
+        """Lane-step slots issued in the last launch (64 lanes x the steps of each wave
+        iteration); steps() / lane_slots() is the fraction of lanes doing useful steps."""
+        return int(self.workspace[2].item())
 
 
 def make_job(

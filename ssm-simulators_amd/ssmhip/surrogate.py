@@ -21,6 +21,7 @@ from . import _lib as L
 
 IN, H1, H2, H3, OUT = 7, 100, 100, 120, 1
 RT_MODES = {"raw": 0, "log": 1, "log_minus_t": 2}  # the rt input transform (xnorm[14])
+LABEL_LOWER_BOUND = math.log(1e-7)  # training-target floor of fit()
 FLOOR = -66.774  # the KDE labels' log-likelihood floor (kde_class.py:188-195)
 LAYERS = [(H1, IN), (H2, H1), (H3, H2), (OUT, H3)]
 
@@ -185,10 +186,11 @@ class MLPSurrogate:
         return loss
 
     def fit(self, rows, *, epochs=10, batch=8192, lr=1e-3, lr_final=1e-4, delta=1.0, seed=0, val_rows=None,
-            label_lower_bound=None, log=None):
-        """Train on LAN rows [N, 8] (device tensor: 7 inputs + loglik label). With
-        `label_lower_bound` the training targets are clipped from below (e.g. log(1e-7)): the
-        floor rows (-66.77) then no longer pull the fit away from the likelihood's bulk."""
+            label_lower_bound=LABEL_LOWER_BOUND, log=None):
+        """Train on LAN rows [N, 8] (device tensor: 7 inputs + loglik label). The training
+        targets are clipped from below at `label_lower_bound` (log(1e-7) by default, None: no
+        clipping): the floor rows (-66.77) then no longer pull the fit away from the
+        likelihood's bulk, where its accuracy matters."""
         t = D.torch()
         rows = rows.contiguous()
         n = rows.shape[0]

@@ -173,7 +173,7 @@ def test_replay_is_independent_of_launch_shape(torch_gpu):
     normals = rng.standard_normal((K * N, L_)).astype(F32)
     rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, L.BOUNDARY_ANGLE, bp)
     outs = []
-    for lp in [em.LaunchParams(64, 1, 1), em.LaunchParams(256, 8, 16), em.LaunchParams(1024, 2, 64)]:
+    for lp in [em.LaunchParams(64, 1, 1, 1), em.LaunchParams(256, 8, 16, 64), em.LaunchParams(1024, 2, 64, 4096)]:
         job = em.make_job(L.MODEL_DDM, L.BOUNDARY_ANGLE, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0,
                           seed=1, smooth_unif=False, replay_normals=normals)
         job.launch(lp)
@@ -181,6 +181,43 @@ def test_replay_is_independent_of_launch_shape(torch_gpu):
         outs.append((job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy()))
     for r, c in outs[1:]:
         assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1])
+
+
+@pytest.mark.parametrize("model", ["ddm", "lca"])
+def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, model):
+    """Counter-based normals: the same seed gives the same trajectories whichever wave, lane,
+    local queue range or refill point a trajectory lands on; ragged and empty sets included."""
+    import torch
+
+    L, em = _ssm()
+    sizes = np.array([0, 1, 3, 700, 0, 64, 65, 1000, 2, 129], np.int64)
+    K = sizes.size
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    rng = np.random.default_rng(11)
+    if model == "ddm":
+        v, a, z, t, bp = _ddm_case("angle", K, 12)
+        rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, L.BOUNDARY_ANGLE, bp)
+        kind, bound, P = L.MODEL_DDM, L.BOUNDARY_ANGLE, 1
+    else:
+        P = 3
+        v = rng.uniform(0.2, 1.5, (K, P)).astype(F32)
+        z = rng.uniform(0.0, 0.3, (K, P)).astype(F32)
+        rec = em.accumulator_records(v, z, rng.uniform(0.1, 0.5, K).astype(F32), np.ones((K, P), F32),
+                                     np.full(K, 999, F32), 20.0, DT, L.BOUNDARY_CONSTANT,
+                                     {"a": rng.uniform(1.0, 2.5, K).astype(F32)},
+                                     g=rng.uniform(0, 0.3, K).astype(F32), b=rng.uniform(0, 0.3, K).astype(F32))
+        kind, bound = L.MODEL_LCA, L.BOUNDARY_CONSTANT
+    outs = []
+    for lp in [em.LaunchParams(64, 1, 1, 1), em.LaunchParams(64, 1, 1, 64), em.LaunchParams(256, 8, 8, 64),
+               em.LaunchParams(512, 4, 3, 7), em.LaunchParams(1024, 2, 64, 4096)]:
+        job = em.make_job(kind, bound, P, rec, offs, delta_t=DT, max_t=20.0, seed=5, smooth_unif=True)
+        job.launch(lp)
+        torch.cuda.synchronize()
+        n = int(offs[-1])
+        outs.append((job.rts[:n].cpu().numpy(), job.choices[:n].cpu().numpy(), job.steps()))
+    for r, c, st in outs[1:]:
+        assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1]) and st == outs[0][2]
+    assert np.all(outs[0][0] > 0)
 
 
 # ----------------------------------------------------------------------------- native RNG statistics
