@@ -106,11 +106,14 @@ def config3_leg(args):
     out = {}
     for model, kind in (("lca_4", 2), ("race_4_weibull", 1)):
         theta = sample_config3(model)
-        sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto")
+        sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto", tune_sample=None)
         ms = time_graph(sim, max(2, args.steps), 1)
         steps = sim.steps()
+        lp = sim.launch_params
         rec = {"trajectories": sim.n_total, "ms_per_launch": ms, "trajectories_per_s": sim.n_total / (ms / 1e3),
-               "em_steps_per_s": steps / (ms / 1e3)}
+               "em_steps_per_s": steps / (ms / 1e3),
+               "launch": [lp.block_size, lp.blocks_per_cu, lp.refill_min, lp.queue_block],
+               "lane_utilisation": steps / max(1, sim.lane_slots())}
         if not args.no_cpu:
             idx = np.arange(0, theta.shape[0], 32)
             n_cpu = 20000
@@ -152,7 +155,8 @@ def config4_leg(args):
     from ssmhip.lan import LanKDELabeller, sample_config4
 
     theta = sample_config4(1024, seed=1)
-    lab = LanKDELabeller("angle", theta, n_samples=100_000, seed=4, launch="auto")
+    # mixture [0.8, 0.2, 0.0] of 1000 rows: 800 KDE resamples + 200 uniform rows, no negative rows
+    lab = LanKDELabeller("angle", theta, n_samples=100_000, n_kde=800, n_unif=200, n_neg=0, seed=4, launch="auto")
     lab.run(timed=True)  # warm-up
     torch.cuda.synchronize()
     reps = max(2, min(args.steps, 5))
@@ -219,6 +223,7 @@ def config5_leg(args, lab, kde_evals_per_s):
     while done < args.surrogate_sets:
         k = min(2048, args.surrogate_sets - done)
         more = LanKDELabeller("angle", sample_config4(k, seed=100 + done), n_samples=args.surrogate_samples,
+                              n_kde=800, n_unif=200, n_neg=0,
                               seed=200 + done, launch=lab.sim.launch_params)
         parts.append(more.run())
         del more
@@ -414,9 +419,22 @@ def headline_leg(args, torch, batched, roofline) -> dict:
     log(f"{ms:.3f} ms/launch, {value:.4e} traj/s, {steps_per_launch / (ms / 1e3):.4e} EM steps/s "
         f"(wall {wall:.2f} s, lane utilisation {util})")
 
+    # hand-over to the host (what a caller of the array-returning drop-ins also pays): the
+    # device-to-host copy of rts + choices into pinned buffers, timed after the launches
+    h_rts = torch.empty(sim.rts.shape, dtype=sim.rts.dtype, pin_memory=True)
+    h_ch = torch.empty(sim.choices.shape, dtype=sim.choices.dtype, pin_memory=True)
+    torch.cuda.synchronize()
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    h_rts.copy_(sim.rts, non_blocking=True)
+    h_ch.copy_(sim.choices, non_blocking=True)
+    c1.record(stream)
+    c1.synchronize()
+    d2h_ms = c0.elapsed_time(c1)
+    d2h_bytes = h_rts.numel() * h_rts.element_size() + h_ch.numel() * h_ch.element_size()
     # sanity: choices are +-1 and rts finite or the omission sentinel
-    ch = sim.choices.cpu().numpy()
-    rts = sim.rts.cpu().numpy()
+    ch = h_ch.numpy()
+    rts = h_rts.numpy()
     assert np.isin(ch, (-1, 0, 1)).all() and np.isfinite(rts).all()
 
     out = {
@@ -442,6 +460,8 @@ def headline_leg(args, torch, batched, roofline) -> dict:
             "launch": {"block": lp.block_size, "blocks_per_cu": lp.blocks_per_cu, "refill_min": lp.refill_min,
                        "queue_block": lp.queue_block},
             "lane_utilisation": util,
+            "host_handover": {"d2h_ms": d2h_ms, "bytes": int(d2h_bytes), "GB_per_s": d2h_bytes / d2h_ms / 1e6,
+                              "trajectories_per_s_incl_d2h": traj / ((ms + d2h_ms) / 1e3)},
         },
         "roofline": {
             "bound": "valu",

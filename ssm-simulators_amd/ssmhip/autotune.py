@@ -24,6 +24,7 @@ from .em import LaunchParams
 DEFAULT_CANDIDATES = [
     (256, 8, 8, 64), (256, 8, 4, 64), (256, 8, 2, 64), (256, 8, 4, 32), (256, 8, 8, 32), (256, 8, 8, 128),
     (256, 8, 16, 64), (512, 4, 8, 64), (512, 4, 4, 64), (128, 16, 8, 64), (1024, 2, 8, 64), (256, 8, 4, 128),
+    (256, 8, 32, 64), (512, 4, 16, 64), (1024, 2, 16, 64), (512, 4, 32, 64),
 ]
 
 _memo: dict[str, LaunchParams] = {}
