@@ -156,7 +156,8 @@ def config4_leg(args):
 
     theta = sample_config4(1024, seed=1)
     # mixture [0.8, 0.2, 0.0] of 1000 rows: 800 KDE resamples + 200 uniform rows, no negative rows
-    lab = LanKDELabeller("angle", theta, n_samples=100_000, n_kde=800, n_unif=200, n_neg=0, seed=4, launch="auto")
+    lab = LanKDELabeller("angle", theta, n_samples=100_000, n_kde=800, n_unif=200, n_neg=0, seed=4, launch="auto",
+                         tune_sample=None)
     lab.run(timed=True)  # warm-up
     torch.cuda.synchronize()
     reps = max(2, min(args.steps, 5))
@@ -327,7 +328,7 @@ def main() -> int:
     ap.add_argument("--legs", type=str, default="headline,config3,config4,config5",
                     help="comma list of legs to run (config5 needs config4); without 'headline' the "
                          "JSON line carries no headline value (for profiling one leg)")
-    ap.add_argument("--surrogate-epochs", type=int, default=120)
+    ap.add_argument("--surrogate-epochs", type=int, default=150)
     ap.add_argument("--surrogate-sets", type=int, default=32768, help="parameter sets of KDE rows to train on")
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000, help="simulations per set for the labels")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

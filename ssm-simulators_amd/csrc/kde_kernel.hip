@@ -342,9 +342,13 @@ constexpr int kTile = 2048;
 // Pass 1: block (group g, chunk c) sums the kernel over samples
 // [c * SSM_KDE_CHUNK, (c + 1) * SSM_KDE_CHUNK) of group g for every point of g.
 // Splitting groups along their samples balances the very unequal
-// (points x samples) work of the groups across the chip.
+// (points x samples) work of the groups across the chip. A group with few points
+// (<= 512, <= 256) splits the block's threads into 2 or 4 teams that cover the same
+// points against different quarters/halves of each sample tile, so every lane holds
+// live points; the teams' sums are added in LDS in a fixed order.
 __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs A) {
   __shared__ __attribute__((aligned(16))) float tile[kTile];
+  __shared__ float red[3 * kEvalThreads];
   const int g = blockIdx.x;
   const int chunk = blockIdx.y;
   const int cnt = A.grp_count[g];
@@ -354,39 +358,65 @@ __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs
   if (cnt <= 0 || nc == 0 || !(h > 0.f) || s0 >= nc) return;
   const int s1 = (s0 + SSM_KDE_CHUNK < nc) ? s0 + SSM_KDE_CHUNK : nc;
   const int64_t pb = A.grp_begin[g];
-  const float nc2 = -1.4426950408889634f / (2.0f * h * h);  // -log2(e) / (2 h^2)
   const float* smp = A.samples + A.kde_start[g];
-  for (int pbase = 0; pbase < cnt; pbase += kEvalThreads * kPtsPerThread) {
+  // Points and samples enter centred on the group's first sample and scaled by
+  // sqrt(log2(e) / 2) / h, so a pair costs d = x - s, e = 64 - d*d (one fma), 2^e, add:
+  // 2^(64 - log2(e) (x - s)^2 / (2 h^2)) = 2^64 exp(-(x - s)^2 / (2 h^2)).
+  const float sc = 0.84932180028801904f / h;
+  const float ctr = smp[0];
+  const int teams = cnt <= kEvalThreads ? 4 : (cnt <= 2 * kEvalThreads ? 2 : 1);
+  const int tpt = kEvalThreads / teams;  // threads per team
+  const int team = threadIdx.x / tpt, lt = threadIdx.x % tpt;
+  const int seg = kTile / teams;         // samples of each tile per team (a multiple of 4)
+  for (int pbase = 0; pbase < cnt; pbase += tpt * kPtsPerThread) {
     float x[kPtsPerThread], acc[kPtsPerThread];
 #pragma unroll
     for (int q = 0; q < kPtsPerThread; ++q) {
-      const int pi = pbase + q * kEvalThreads + threadIdx.x;
+      const int pi = pbase + q * tpt + lt;
       const float rt = pi < cnt ? A.pts_rt[pb + pi] : 1.f;
-      x[q] = logf(fmaxf(rt, A.eps));
+      x[q] = (logf(fmaxf(rt, A.eps)) - ctr) * sc;
       acc[q] = 0.f;
     }
     for (int t0 = s0; t0 < s1; t0 += kTile) {
       const int tn = (s1 - t0) < kTile ? (s1 - t0) : kTile;
       __syncthreads();
-      for (int i = threadIdx.x; i < kTile; i += kEvalThreads) tile[i] = i < tn ? smp[t0 + i] : 1e30f;
+      for (int i = threadIdx.x; i < kTile; i += kEvalThreads) tile[i] = i < tn ? (smp[t0 + i] - ctr) * sc : 1e30f;
       __syncthreads();
       const int tn4 = (tn + 3) & ~3;
-      for (int i = 0; i < tn4; i += 4) {
+      const int lo = team * seg;
+      const int hi = lo + seg < tn4 ? lo + seg : tn4;
+      for (int i = lo; i < hi; i += 4) {
         const float4 s4 = *reinterpret_cast<const float4*>(&tile[i]);
 #pragma unroll
         for (int q = 0; q < kPtsPerThread; ++q) {
           float d;
-          d = x[q] - s4.x; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
-          d = x[q] - s4.y; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
-          d = x[q] - s4.z; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
-          d = x[q] - s4.w; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(d * d, nc2, kScaleLog2));
+          d = x[q] - s4.x; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+          d = x[q] - s4.y; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+          d = x[q] - s4.z; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+          d = x[q] - s4.w; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
         }
       }
     }
+    if (teams > 1) {
+      __syncthreads();
+      if (team > 0) {
 #pragma unroll
-    for (int q = 0; q < kPtsPerThread; ++q) {
-      const int pi = pbase + q * kEvalThreads + threadIdx.x;
-      if (pi < cnt) A.partial[(pb + pi) * A.n_chunks + chunk] = acc[q];
+        for (int q = 0; q < kPtsPerThread; ++q) red[((team - 1) * tpt + lt) * kPtsPerThread + q] = acc[q];
+      }
+      __syncthreads();
+      if (team == 0) {
+        for (int m = 1; m < teams; ++m) {
+#pragma unroll
+          for (int q = 0; q < kPtsPerThread; ++q) acc[q] += red[((m - 1) * tpt + lt) * kPtsPerThread + q];
+        }
+      }
+    }
+    if (team == 0) {
+#pragma unroll
+      for (int q = 0; q < kPtsPerThread; ++q) {
+        const int pi = pbase + q * tpt + lt;
+        if (pi < cnt) A.partial[(pb + pi) * A.n_chunks + chunk] = acc[q];
+      }
     }
   }
 }

@@ -24,6 +24,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // real sizes
 constexpr int kIn = 7, kH1 = 100, kH2 = 100, kH3 = 120, kOut = 1;
+static_assert(kH1 <= 112 && kH2 <= 112, "the forward kernel skips output tile 7 of layers 1 and 2");
 // parameter offsets in the flat FP32 master buffer (nn.Linear layout W[out][in])
 constexpr int oW1 = 0, oB1 = oW1 + kH1 * kIn, oW2 = oB1 + kH1, oB2 = oW2 + kH2 * kH1, oW3 = oB2 + kH2,
               oB3 = oW3 + kH3 * kH2, oW4 = oB3 + kH3, oB4 = oW4 + kOut * kH3, kNParams = oB4 + kOut;
@@ -188,8 +189,15 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       }
       B[t][0] = v;
     }
+    // Output tile 7 of the 100-unit layers 1 and 2 (units 112..127) is all padding: its
+    // weights and biases are zero, so it stays 0 and tanh(0) = 0 need not be computed.
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
+      if (a == 7) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[t][a] = f32x4{0.f, 0.f, 0.f, 0.f};
+        continue;
+      }
       const bf16x8 w = load_frag(frags, fF1, a, lane);
       const f32x4 bt = bias_tile(bias, pB1, a, g);
 #pragma unroll
@@ -201,7 +209,7 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < 7; ++a)  // tile 7 of layers 1 and 2 is zero
 #pragma unroll
           for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
 #pragma unroll
@@ -211,6 +219,11 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       const int bo = (L == 2) ? pB2 : pB3;
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
+        if (L == 2 && a == 7) {
+          acc[0][a] = f32x4{0.f, 0.f, 0.f, 0.f};
+          acc[1][a] = f32x4{0.f, 0.f, 0.f, 0.f};
+          continue;
+        }
         const f32x4 bt = bias_tile(bias, bo, a, g);
         f32x4 c0 = bt, c1 = bt;
 #pragma unroll

@@ -35,20 +35,28 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
 }
 
 __global__ void __launch_bounds__(256) probe_kde_pair(int64_t iters, float* out) {
+  // the evaluation kernel's inner step: 4 points x 4 samples per iteration, each pair
+  // d = x - s, 2^(fma(-d, d, 64)), add; the 4 sample updates stand in for its LDS read
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const float x = 0.01f * (float)(gid & 255);
-  const float c = -0.7213475f, K = 64.f;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
-  float s = 0.25f;
-  for (int64_t it = 0; it < iters; ++it) {
-    const float d0 = x - s, d1 = x - (s + 0.1f), d2 = x - (s + 0.2f), d3 = x - (s + 0.3f);
-    acc0 += __builtin_amdgcn_exp2f(__builtin_fmaf(d0 * d0, c, K));
-    acc1 += __builtin_amdgcn_exp2f(__builtin_fmaf(d1 * d1, c, K));
-    acc2 += __builtin_amdgcn_exp2f(__builtin_fmaf(d2 * d2, c, K));
-    acc3 += __builtin_amdgcn_exp2f(__builtin_fmaf(d3 * d3, c, K));
-    s += 1e-6f;
+  float x[4], acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = 0.01f * (float)((gid + q) & 255);
+    acc[q] = 0.f;
   }
-  out[gid] = acc0 + acc1 + acc2 + acc3;
+  float s0 = 0.25f, s1 = 0.35f, s2 = 0.45f, s3 = 0.55f;
+  for (int64_t it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float d;
+      d = x[q] - s0; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, 64.f));
+      d = x[q] - s1; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, 64.f));
+      d = x[q] - s2; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, 64.f));
+      d = x[q] - s3; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, 64.f));
+    }
+    s0 += 1e-6f; s1 += 1e-6f; s2 += 1e-6f; s3 += 1e-6f;
+  }
+  out[gid] = acc[0] + acc[1] + acc[2] + acc[3];
 }
 
 template <int KIND>

@@ -52,7 +52,8 @@ class LanKDELabeller:
 
     def __init__(self, model: str, theta: np.ndarray, *, n_samples: int = 100_000, n_kde: int = 800,
                  n_unif: int = 100, n_neg: int = 100, delta_t: float = 0.001, max_t: float = 20.0, seed: int = 0,
-                 smooth_unif: bool = True, shuffle: bool = True, launch="auto", lb: float = -66.774):
+                 smooth_unif: bool = True, shuffle: bool = True, launch="auto", lb: float = -66.774,
+                 tune_sample: float | None = 8e6):
         self.model = model
         self.theta = np.ascontiguousarray(theta, np.float32)
         self.n_sets = self.theta.shape[0]
@@ -60,7 +61,8 @@ class LanKDELabeller:
         self.max_t, self.seed, self.shuffle, self.lb = float(max_t), int(seed), bool(shuffle), float(lb)
         self.choice_values = choice_values_for(model)
         self.sim = BatchedSimulation(model, self.theta, n_samples, delta_t=delta_t, max_t=max_t, seed=seed,
-                                     smooth_unif=smooth_unif, choice_dtype="int8", launch=launch)
+                                     smooth_unif=smooth_unif, choice_dtype="int8", launch=launch,
+                                     tune_sample=tune_sample)
         self.d_theta = D.to_dev(self.theta)
         self.d_choice_values = D.to_dev(np.asarray(self.choice_values, np.int32))
         self.times = StageTimes()
