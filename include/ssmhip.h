@@ -200,6 +200,28 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
                          const float* theta_dev, const float* pts_rt_dev, const int8_t* pts_choice_dev,
                          const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream);
 
+/* ssm_aux_labels replaces SimulationPipeline._compute_auxiliary_labels
+ * (ssms/dataset_generators/pipelines/simulation_pipeline.py:258-351) as
+ * integer counts per set k over its draws [offsets[k], offsets[k+1]):
+ *   counts_dev[k][c]   draws with choice choice_values[c] (omitted rts included)
+ *   nogo_dev[k]        draws with choice != max_choice or rt == -999
+ *   hist128_dev[k][b][c], hist256_dev[k][b][c]
+ *                      np.histogram counts of the non-omitted rts of choice c
+ *                      over np.linspace(0, max_t, B + 1), B = 128 / 256
+ * The caller forms choice_p = counts / n, nogo_p = nogo / n, etc.            */
+int32_t ssm_aux_labels(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, int32_t max_choice,
+                       const float* rts_dev, const void* choices_dev, int32_t choice_bytes,
+                       const int64_t* offsets_dev, double max_t, int32_t* counts_dev, int32_t* nogo_dev,
+                       int32_t* hist128_dev, int32_t* hist256_dev, void* stream);
+
+/* ssm_theta_uniform replaces UniformParameterSampler._sample_parameter
+ * (ssms/dataset_generators/parameter_samplers/uniform_sampler.py:24-51) for
+ * independent bounds: theta_dev[k][j] = float32(lo[j] + (hi[j] - lo[j]) u) in
+ * float64, u a 53-bit uniform from Philox4x32-10 keyed by seed (lo_dev, hi_dev:
+ * float64[n_params], 1 <= n_params <= 64).                                    */
+int32_t ssm_theta_uniform(int32_t n_sets, int32_t n_params, const double* lo_dev, const double* hi_dev,
+                          uint64_t seed, float* theta_dev, void* stream);
+
 /* ---- likelihood surrogate (no counterpart in the reference) ------------------
  * MLP 7 -> 100 -> 100 -> 120 -> 1 with tanh hidden layers on BF16 MFMA with
  * FP32 accumulation; inputs are the LAN rows' (5 theta, rt, choice)

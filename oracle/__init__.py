@@ -12,6 +12,7 @@ Modules
   ddm_analytic  closed forms for the constant-bound DDM (Navarro & Fuss 2009)
   kde_ref       float64 LogKDE restatement (fit / eval / sample)
   mlp_ref       float32/float64 NumPy MLP forward/backward, Huber, Adam
+  aux_ref       auxiliary labels of the LAN data generator (counts, histograms)
   cpu_baseline  ctypes loader of the C++/OpenMP restatement (cpu_baseline.cpp)
 
 Pinning status of each function is stated in its docstring.

@@ -28,6 +28,7 @@ SOURCES = {
     "em_kernel.hip": ["-ffp-contract=off"],
     "lba_kernel.hip": ["-ffp-contract=off"],
     "kde_kernel.hip": [],
+    "aux_kernel.hip": [],
     # MFMA results in VGPRs (no accumulator-register copies before tanh)
     "mlp_kernel.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "probe_kernel.hip": [],

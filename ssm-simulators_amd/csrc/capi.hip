@@ -65,6 +65,12 @@ int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_b
                          const int32_t* choice_total, const float* bandwidth, const int64_t* kde_start,
                          const int64_t* offsets, float max_t, float lb, float eps, int64_t max_samples,
                          float* workspace, float* out_ll, hipStream_t stream);
+int32_t aux_labels_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev,
+                           int32_t max_choice, const float* rts, const void* choices, int32_t choice_bytes,
+                           const int64_t* offsets, double max_t, int32_t* counts, int32_t* nogo, int32_t* hist128,
+                           int32_t* hist256, hipStream_t s);
+int32_t theta_uniform_enqueue(int32_t n_sets, int32_t n_params, const double* lo, const double* hi, uint64_t seed,
+                              float* theta, hipStream_t s);
 int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
                              const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
                              const int32_t* choice_values, float* rows, hipStream_t stream);
@@ -219,6 +225,20 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shu
                          const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream) {
   return lan_assemble_enqueue(n_sets, n_theta, m, shuffle, seed, theta_dev, pts_rt_dev, pts_choice_dev, ll_dev,
                               choice_values_dev, rows_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_aux_labels(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, int32_t max_choice,
+                       const float* rts_dev, const void* choices_dev, int32_t choice_bytes,
+                       const int64_t* offsets_dev, double max_t, int32_t* counts_dev, int32_t* nogo_dev,
+                       int32_t* hist128_dev, int32_t* hist256_dev, void* stream) {
+  return aux_labels_enqueue(n_sets, n_choices, choice_values_dev, max_choice, rts_dev, choices_dev, choice_bytes,
+                            offsets_dev, max_t, counts_dev, nogo_dev, hist128_dev, hist256_dev,
+                            (hipStream_t)stream);
+}
+
+int32_t ssm_theta_uniform(int32_t n_sets, int32_t n_params, const double* lo_dev, const double* hi_dev,
+                          uint64_t seed, float* theta_dev, void* stream) {
+  return theta_uniform_enqueue(n_sets, n_params, lo_dev, hi_dev, seed, theta_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows,

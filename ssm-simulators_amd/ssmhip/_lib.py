@@ -115,6 +115,11 @@ _SIGS = {
         [_VP, _VP, C.c_int64, C.c_int32, C.c_int32, _VP, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
          _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int64, _VP, _VP],
     ),
+    "ssm_aux_labels": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, _VP, C.c_int32, _VP, _VP, C.c_int32, _VP, C.c_double, _VP, _VP, _VP, _VP, _VP],
+    ),
+    "ssm_theta_uniform": (C.c_int32, [C.c_int32, C.c_int32, _VP, _VP, C.c_uint64, _VP, _VP]),
     "ssm_mfma_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (

@@ -129,10 +129,75 @@ class LanKDELabeller:
             out = t.gather(out, 1, order[:, :, None].expand(-1, -1, W))
         return out.reshape(S * (m + self.n_neg), W).contiguous()
 
-    def output_dict(self, rows) -> dict:
-        """The per-key arrays TrainingDataGenerator concatenates (lan_mlp.py:533-549)."""
+    def aux_counts(self) -> dict:
+        """Integer counts behind the auxiliary labels of every set, from the last simulation
+        (SimulationPipeline._compute_auxiliary_labels, simulation_pipeline.py:258-351):
+        per-choice draws [S, C], no-go draws [S], RT histograms [S, 128, C] and [S, 256, C]."""
+        t = D.require_gpu()
+        S, C = self.n_sets, len(self.choice_values)
+        counts = D.empty((S, C), t.int32)
+        nogo = D.empty((S,), t.int32)
+        h128 = D.empty((S, 128, C), t.int32)
+        h256 = D.empty((S, 256, C), t.int32)
+        rc = L.load().ssm_aux_labels(S, C, D.ptr(self.d_choice_values), int(max(self.choice_values)),
+                                     D.ptr(self.sim.rts), D.ptr(self.sim.choices), self.sim.job.prob.choice_bytes,
+                                     D.ptr(self.sim.job.offsets), float(self.max_t), D.ptr(counts), D.ptr(nogo),
+                                     D.ptr(h128), D.ptr(h256), D.stream_ptr())
+        L.check(rc, "ssm_aux_labels")
+        return {"counts": counts, "nogo": nogo, "binned_128": h128, "binned_256": h256}
+
+    def aux_labels(self) -> dict:
+        """The auxiliary label arrays of all sets, concatenated as TrainingDataGenerator does
+        (float32; lan_mlp.py:533-541): cpn, cpn_no_omission, opn, gonogo labels and the
+        binned histograms. Probabilities are formed in float64 from exact counts, as the
+        reference's np.sum(...) / n_samples, then cast."""
+        a = {k: v.cpu().numpy() for k, v in self.aux_counts().items()}
+        n = np.asarray(self.sim.offsets[1:] - self.sim.offsets[:-1], np.int64)
+        nf = np.maximum(n, 1).astype(np.float64)[:, None]
+        choice_p = a["counts"].astype(np.float64) / nf
+        # choices never take the omission sentinel (only rts do), so no draw is removed
+        choice_p_no_omission = choice_p.copy()
+        omission_p = np.zeros((self.n_sets, 1))
+        nogo_p = a["nogo"].astype(np.float64)[:, None] / nf
+        if len(self.choice_values) == 2:
+            cpn = choice_p[:, 1:2]
+            cpn_no = choice_p_no_omission[:, 1:2]
+        else:
+            cpn, cpn_no = choice_p, choice_p_no_omission
+        th = self.theta.astype(np.float32)
+        f32 = lambda x: np.asarray(x).astype(np.float32)  # noqa: E731
+        return {"cpn_data": th, "cpn_labels": f32(cpn), "cpn_no_omission_data": th,
+                "cpn_no_omission_labels": f32(cpn_no), "opn_data": th, "opn_labels": f32(omission_p),
+                "gonogo_data": th, "gonogo_labels": f32(nogo_p), "binned_128": f32(a["binned_128"]),
+                "binned_256": f32(a["binned_256"]), "theta": th}
+
+    def output_dict(self, rows, aux: bool = True) -> dict:
+        """The per-key arrays TrainingDataGenerator returns for all sets (lan_mlp.py:533-549):
+        lan_data / lan_labels from `rows`, and with `aux` the auxiliary labels of the same
+        simulations (call before the next simulate)."""
         r = rows.cpu().numpy()
-        return {"lan_data": r[:, :-1], "lan_labels": r[:, -1], "theta": self.theta}
+        out = {"lan_data": r[:, :-1], "lan_labels": r[:, -1], "theta": self.theta.astype(np.float32)}
+        if aux:
+            out.update(self.aux_labels())
+        return out
+
+
+def sample_theta(n_sets: int, lower, upper, seed: int = 0) -> np.ndarray:
+    """Parameter sets drawn on the device, independent uniform bounds per parameter
+    (UniformParameterSampler, parameter_samplers/uniform_sampler.py:24-51; float64 draw,
+    float32 result). Returns float32 [n_sets, n_params] on the host."""
+    t = D.require_gpu()
+    lo = np.asarray(lower, np.float64).reshape(-1)
+    hi = np.asarray(upper, np.float64).reshape(-1)
+    if lo.shape != hi.shape or lo.size < 1:
+        raise ValueError("lower and upper must have the same, non-zero length")
+    if np.any(hi < lo):
+        raise ValueError("every upper bound must be >= its lower bound")
+    out = D.empty((int(n_sets), lo.size), t.float32)
+    d_lo, d_hi = D.to_dev(lo), D.to_dev(hi)
+    L.check(L.load().ssm_theta_uniform(int(n_sets), lo.size, D.ptr(d_lo), D.ptr(d_hi), int(seed) & (2**64 - 1),
+                                       D.ptr(out), D.stream_ptr()), "ssm_theta_uniform")
+    return out.cpu().numpy()
 
 
 def sample_config4(n_sets: int = 1024, seed: int = 1):
