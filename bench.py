@@ -211,6 +211,7 @@ def config5_leg(args, lab, kde_evals_per_s):
     import torch
 
     from oracle import ddm_analytic
+    from ssmhip import roofline
     from ssmhip.surrogate import MLPSurrogate
 
     from ssmhip.lan import LanKDELabeller, sample_config4
@@ -302,7 +303,13 @@ def config5_leg(args, lab, kde_evals_per_s):
     kde_at = np.empty_like(kll)
     kde_at[order] = kll
     kde_mae = float(np.mean(np.abs(kde_at - ref)))
-    rec = {"label_sets": args.surrogate_sets, "label_samples_per_set": args.surrogate_samples,
+    # roof: the surrogate's tanh (exp2 + rcp on the transcendental unit) at full occupancy;
+    # the forward computes 352 tanh per row (112 + 112 + 128 with padding), MFMA runs beside it
+    tanh_rate = None if args.no_probe else roofline.probe_rate("tanh", iters=2048)
+    rec_roof = None if tanh_rate is None else {
+        "bound": "transcendental (tanh)", "achieved": evals / 1e9, "peak": tanh_rate / 352 / 1e9,
+        "unit": "G rows/s", "frac": evals / (tanh_rate / 352), "tanh_per_s": tanh_rate}
+    rec = {"inference_roofline": rec_roof, "label_sets": args.surrogate_sets, "label_samples_per_set": args.surrogate_samples,
            "label_s": label_s, "train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
            "train_rows_per_s": train.shape[0] * args.surrogate_epochs / train_s, "huber_history": hist[-3:],
            "val_mae_vs_kde_above_floor": net.val_mae, "val_mae_vs_kde_all": net.val_mae_all, "inference_rows": N, "ms_per_inference": ms, "evals_per_s": evals,

@@ -278,9 +278,10 @@ int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void*
  * Branch-free loops at the occupancy of `n_threads` lanes (multiple of 256),
  * `iters` iterations each; out_dev float32[n_threads]. kind 0: the EM block
  * (Philox4x32-10 + 2 Box–Muller pairs + 4 angle-DDM steps); kind 1: the KDE
- * pair (sub, mul, fma, exp2, add) x4; kinds 2..8: 8 chains of one instruction
- * (v_mad_u64_u32, v_xor_b32, v_fma_f32, v_log_f32, v_exp_f32, v_sin_f32,
- * v_sqrt_f32). Time it with events on `stream`.                            */
+ * pair (sub, fma, exp2, add), 4 points x 4 samples; kinds 2..8: 8 chains of
+ * one instruction (v_mad_u64_u32, v_xor_b32, v_fma_f32, v_log_f32, v_exp_f32,
+ * v_sin_f32, v_sqrt_f32); kind 9: 8 chains of the surrogate's tanh
+ * (exp2, add, rcp, fma). Time it with events on `stream`.                  */
 int32_t ssm_valu_probe(int32_t kind, int64_t iters, int64_t n_threads, float* out_dev, void* stream);
 
 #ifdef __cplusplus

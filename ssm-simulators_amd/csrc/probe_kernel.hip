@@ -2,10 +2,11 @@
 // measured against. Each probe runs a branch-free loop at full occupancy:
 //   kind 0  EM block: Philox4x32-10 + 2 Box–Muller pairs + 4 angle-DDM steps
 //           (the simulator's inner loop without walk termination or refill)
-//   kind 1  KDE pair: sub, mul, fma, exp2, add (the KDE evaluation inner loop)
+//   kind 1  KDE pair: sub, fma, exp2, add (the KDE evaluation inner loop)
 //   kind 2..8  single instructions, 8 independent chains per lane:
 //           2 v_mad_u64_u32, 3 v_xor_b32, 4 v_fma_f32, 5 v_log_f32,
 //           6 v_exp_f32, 7 v_sin_f32, 8 v_sqrt_f32
+//   kind 9  the surrogate's tanh: exp2, add, rcp, fma (+ one fma), 8 chains per lane
 // Every lane writes one float so nothing is optimised away.
 #include <hip/hip_runtime.h>
 
@@ -85,6 +86,11 @@ __global__ void __launch_bounds__(256) probe_single(int64_t iters, float* out) {
         f[j] = __builtin_amdgcn_exp2f(f[j]) * 0.25f;
       } else if constexpr (KIND == 7) {
         f[j] = __builtin_amdgcn_sinf(f[j]) + 0.5f;
+      } else if constexpr (KIND == 9) {
+        // the surrogate's tanh on a pre-scaled argument: 1 - 2 / (2^z + 1), with one fma
+        // standing in for the bf16 repack of the result
+        const float z = __builtin_fmaf(f[j], 0.5f, 0.25f);
+        f[j] = __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(z) + 1.0f), 1.0f);
       } else {
         f[j] = __builtin_amdgcn_sqrtf(f[j]) + 0.5f;
       }
@@ -110,6 +116,7 @@ int32_t probe_enqueue(int32_t kind, int64_t iters, int64_t n_threads, float* out
     case 6: hipLaunchKernelGGL(probe_single<6>, grid, block, 0, stream, iters, out); break;
     case 7: hipLaunchKernelGGL(probe_single<7>, grid, block, 0, stream, iters, out); break;
     case 8: hipLaunchKernelGGL(probe_single<8>, grid, block, 0, stream, iters, out); break;
+    case 9: hipLaunchKernelGGL(probe_single<9>, grid, block, 0, stream, iters, out); break;
     default: return fail(SSM_EINVAL, "unknown probe kind");
   }
   SSM_HIP(hipGetLastError());

@@ -182,6 +182,29 @@ class LanKDELabeller:
         return out
 
 
+def save_training_data(data: dict, output_folder, pickle_protocol: int = 4, generator_config: dict | None = None,
+                       model_config: dict | None = None):
+    """Write a training-data dict to `output_folder/training_data_<uuid1 hex>.pickle` as
+    TrainingDataGenerator._save_training_data does (lan_mlp.py:375-396); with the configs
+    given, they are stored under "generator_config" / "model_config" as the reference adds
+    them (lan_mlp.py:551-557). Returns the path."""
+    import pickle
+    import uuid
+    from pathlib import Path
+
+    folder = Path(output_folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    out = dict(data)
+    if generator_config is not None:
+        out["generator_config"] = generator_config
+    if model_config is not None:
+        out["model_config"] = model_config
+    path = folder / f"training_data_{uuid.uuid1().hex}.pickle"
+    with path.open("wb") as f:
+        pickle.dump(out, f, protocol=int(pickle_protocol))
+    return path
+
+
 def sample_theta(n_sets: int, lower, upper, seed: int = 0) -> np.ndarray:
     """Parameter sets drawn on the device, independent uniform bounds per parameter
     (UniformParameterSampler, parameter_samplers/uniform_sampler.py:24-51; float64 draw,

@@ -1,4 +1,4 @@
-"""Measured compute roofs for the VALU-bound kernels (EM walk, KDE pairs).
+"""Measured compute roofs for the VALU-bound kernels (EM walk, KDE pairs, surrogate tanh).
 
 The EM and KDE kernels do no MFMA work and move a few bytes per thousand
 instructions, so their roof is VALU issue. The probes in probe_kernel.hip run
@@ -12,9 +12,9 @@ from . import _device as D
 from . import _lib as L
 
 KINDS = {"em_block": 0, "kde_pair": 1, "mad_u64_u32": 2, "xor_b32": 3, "fma_f32": 4, "log_f32": 5, "exp_f32": 6,
-         "sin_f32": 7, "sqrt_f32": 8}
+         "sin_f32": 7, "sqrt_f32": 8, "tanh": 9}
 # work units per lane-iteration: EM steps, KDE pairs, or instructions
-UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8}
+UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8, 9: 8}
 
 # MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz: full-rate 32-bit VALU lane-ops/s
 PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9

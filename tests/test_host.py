@@ -97,3 +97,22 @@ def test_resample_mixture_strategy_with_a_stub_estimator():
     assert out.shape == (1000, 7) and out.dtype == F32
     assert np.sum(out[:, -1] == F32(-66.77497)) == 100
     assert np.all(out[:, 0] == 1.0)
+
+
+def test_training_data_file_format(tmp_path):
+    import pickle
+    import re
+
+    from ssmhip.lan import save_training_data
+
+    data = {"lan_data": np.zeros((4, 7), np.float32), "lan_labels": np.ones(4, np.float32),
+            "binned_128": np.zeros((1, 128, 2), np.float32)}
+    path = save_training_data(data, tmp_path / "out", generator_config={"output": {"pickle_protocol": 4}},
+                              model_config={"name": "angle"})
+    assert re.fullmatch(r"training_data_[0-9a-f]{32}\.pickle", path.name)
+    with path.open("rb") as f:
+        assert f.read(2) == b"\x80\x04"  # protocol 4
+    with path.open("rb") as f:
+        back = pickle.load(f)  # a file this test wrote
+    assert set(back) == {"lan_data", "lan_labels", "binned_128", "generator_config", "model_config"}
+    assert np.array_equal(back["lan_data"], data["lan_data"]) and back["model_config"]["name"] == "angle"
