@@ -451,6 +451,188 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(TrainArgs A) {
   }
 }
 
+// ---- the same forward + loss + backward-data, four waves per 16-row tile -------------
+// Wave w of a block computes output tiles 2w and 2w+1 of every 128-feature layer. Packed
+// as an MFMA operand, those two tiles are exactly K block w of the next layer's input
+// (pack_pair keeps each lane's own values), so after each layer the four waves swap their
+// blocks through LDS and continue. Every output element is accumulated in the same order
+// as in mlp_train_kernel, so y, loss, acts and deltas are bit-identical to it; a batch of
+// a few thousand rows then fills the chip four times more densely, with a quarter of the
+// registers per wave. No input gradient (dx must be NULL).
+__global__ void __launch_bounds__(256) mlp_train_team_kernel(TrainArgs A) {
+  __shared__ bf16x8 xch[2][4][64];  // double-buffered operand exchange: [buffer][K block][lane]
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int w = threadIdx.x >> 6;
+  const int a0 = 2 * w, a1 = 2 * w + 1;
+  const int64_t ntiles = (A.n + 15) / 16;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    A.frags = opaque(A.frags);
+    const int64_t row = tile * 16 + r;
+    const bool live = row < A.n;
+    bf16x8 x0 = {};
+    if (g == 0 && live) {
+#pragma unroll
+      for (int jj = 0; jj < kIn; ++jj) x0[jj] = (__bf16)input_value(A.xnorm, jj, A.x + row * kIn);
+    }
+    if (w == 0 && A.acts && g == 0 && live) {
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) A.acts[(int64_t)jj * A.ld + row] = x0[jj];
+    }
+    bf16x8 B[4];
+    // layer 1
+    f32x4 c0 = mfma(load_frag(A.frags, fF1, a0, lane), x0, bias_tile(A.bias, pB1, a0, g));
+    f32x4 c1 = mfma(load_frag(A.frags, fF1, a1, lane), x0, bias_tile(A.bias, pB1, a1, g));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      c0[i] = tanh_fast(c0[i]);
+      c1[i] = tanh_fast(c1[i]);
+    }
+    const bf16x8 h1w = pack_pair(c0, c1);
+    xch[0][w][lane] = h1w;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = xch[0][b][lane];
+    // layer 2
+    c0 = bias_tile(A.bias, pB2, a0, g);
+    c1 = bias_tile(A.bias, pB2, a1, g);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      c0 = mfma(load_frag(A.frags, fF2, a0 * 4 + b, lane), B[b], c0);
+      c1 = mfma(load_frag(A.frags, fF2, a1 * 4 + b, lane), B[b], c1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      c0[i] = tanh_fast(c0[i]);
+      c1[i] = tanh_fast(c1[i]);
+    }
+    const bf16x8 h2w = pack_pair(c0, c1);
+    xch[1][w][lane] = h2w;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = xch[1][b][lane];
+    // layer 3
+    c0 = bias_tile(A.bias, pB3, a0, g);
+    c1 = bias_tile(A.bias, pB3, a1, g);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      c0 = mfma(load_frag(A.frags, fF3, a0 * 4 + b, lane), B[b], c0);
+      c1 = mfma(load_frag(A.frags, fF3, a1 * 4 + b, lane), B[b], c1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      c0[i] = tanh_fast(c0[i]);
+      c1[i] = tanh_fast(c1[i]);
+    }
+    const bf16x8 h3w = pack_pair(c0, c1);
+    xch[0][w][lane] = h3w;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = xch[0][b][lane];
+    // output layer (every wave forms y and dy of the tile's rows itself)
+    f32x4 yo = bias_tile(A.bias, pB4, 0, g);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) yo = mfma(load_frag(A.frags, fF4, b, lane), B[b], yo);
+    float dy = 0.f;
+    if (g == 0 && live) {
+      const float yv = yo[0];
+      if (w == 0 && A.y) A.y[row] = yv;
+      if (A.mode == 0) {
+        dy = A.target_or_grad[row];
+      } else {
+        const float e = yv - A.target_or_grad[row];
+        const float ae = fabsf(e);
+        if (w == 0 && A.loss) A.loss[row] = ae <= A.delta ? 0.5f * e * e : A.delta * (ae - 0.5f * A.delta);
+        dy = fminf(fmaxf(e, -A.delta), A.delta) * A.scale;
+      }
+    }
+    if (A.acts && live) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * (a0 + q) + 4 * g + i;
+          A.acts[(int64_t)(8 + f) * A.ld + row] = h1w[4 * q + i];
+          A.acts[(int64_t)(8 + 128 + f) * A.ld + row] = h2w[4 * q + i];
+          A.acts[(int64_t)(8 + 256 + f) * A.ld + row] = h3w[4 * q + i];
+        }
+    }
+    const f32x4 dz4 = {g == 0 ? dy : 0.f, 0.f, 0.f, 0.f};
+    if (w == 0 && A.deltas && live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) store_fm(A.deltas, A.ld, 384 + 4 * g + i, row, dz4[i]);
+    }
+    // dz3 = (W4^T dz4) (1 - h3^2)
+    const bf16x8 bd4 = pack_pair(dz4, f32x4{0.f, 0.f, 0.f, 0.f});
+    c0 = mfma(load_frag(A.frags, fB4, a0, lane), bd4, f32x4{0.f, 0.f, 0.f, 0.f});
+    c1 = mfma(load_frag(A.frags, fB4, a1, lane), bd4, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float p0 = (float)h3w[i], p1 = (float)h3w[4 + i];
+      c0[i] *= 1.0f - p0 * p0;
+      c1[i] *= 1.0f - p1 * p1;
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        store_fm(A.deltas, A.ld, 256 + 16 * a0 + 4 * g + i, row, c0[i]);
+        store_fm(A.deltas, A.ld, 256 + 16 * a1 + 4 * g + i, row, c1[i]);
+      }
+    }
+    xch[1][w][lane] = pack_pair(c0, c1);
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = xch[1][b][lane];
+    // dz2 = (W3^T dz3) (1 - h2^2)
+    c0 = f32x4{0.f, 0.f, 0.f, 0.f};
+    c1 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      c0 = mfma(load_frag(A.frags, fB3, a0 * 4 + b, lane), B[b], c0);
+      c1 = mfma(load_frag(A.frags, fB3, a1 * 4 + b, lane), B[b], c1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float p0 = (float)h2w[i], p1 = (float)h2w[4 + i];
+      c0[i] = c0[i] * (1.0f - p0 * p0);
+      c1[i] = c1[i] * (1.0f - p1 * p1);
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        store_fm(A.deltas, A.ld, 128 + 16 * a0 + 4 * g + i, row, c0[i]);
+        store_fm(A.deltas, A.ld, 128 + 16 * a1 + 4 * g + i, row, c1[i]);
+      }
+    }
+    xch[0][w][lane] = pack_pair(c0, c1);
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = xch[0][b][lane];
+    // dz1 = (W2^T dz2) (1 - h1^2)
+    c0 = f32x4{0.f, 0.f, 0.f, 0.f};
+    c1 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      c0 = mfma(load_frag(A.frags, fB2, a0 * 4 + b, lane), B[b], c0);
+      c1 = mfma(load_frag(A.frags, fB2, a1 * 4 + b, lane), B[b], c1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float p0 = (float)h1w[i], p1 = (float)h1w[4 + i];
+      c0[i] = c0[i] * (1.0f - p0 * p0);
+      c1[i] = c1[i] * (1.0f - p1 * p1);
+    }
+    if (A.deltas && live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        store_fm(A.deltas, A.ld, 16 * a0 + 4 * g + i, row, c0[i]);
+        store_fm(A.deltas, A.ld, 16 * a1 + 4 * g + i, row, c1[i]);
+      }
+    }
+    // the next tile's first exchange writes buffer 0, which this tile last read before
+    // its final barrier: no further barrier needed here
+  }
+}
+
 // ---- weight gradients: dW_L = dZ_L^T H_{L-1} over the batch -------------------------------
 // Tasks: L1 8x1 tiles, L2 8x8, L3 8x8, L4 1x8 (out tile a, in tile c); one wave per task and
 // batch slice: blockIdx.y = s takes rows [s * chunk, min(n, (s + 1) * chunk)) (chunk a multiple
@@ -594,12 +776,19 @@ int32_t mlp_train_enqueue(const TrainArgs& A, hipStream_t s) {
   if ((A.acts || A.deltas) && (A.ld < A.n || A.ld % 32)) return fail(SSM_EINVAL, "ld must be >= n and a multiple of 32");
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
-  // one-wave blocks (16 rows each) so that a training batch of a few thousand rows spreads
-  // over every CU; at most 16 waves per CU, the rest by grid stride
-  const int64_t waves_needed = (A.n + 15) / 16;
-  const int64_t cap = (int64_t)cus * 16;
-  const int64_t blocks = waves_needed < cap ? waves_needed : cap;
-  hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)blocks), dim3(64), 0, s, A);
+  const int64_t tiles = (A.n + 15) / 16;
+  if (!A.dx) {
+    // four waves per 16-row tile, up to 8 blocks per CU, the rest by grid stride
+    const int64_t cap = (int64_t)cus * 8;
+    const int64_t blocks = tiles < cap ? tiles : cap;
+    hipLaunchKernelGGL(mlp_train_team_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A);
+  } else {
+    // one-wave blocks (16 rows each) so that a batch of a few thousand rows spreads over
+    // every CU; at most 16 waves per CU, the rest by grid stride
+    const int64_t cap = (int64_t)cus * 16;
+    const int64_t blocks = tiles < cap ? tiles : cap;
+    hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)blocks), dim3(64), 0, s, A);
+  }
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

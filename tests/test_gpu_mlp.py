@@ -210,3 +210,37 @@ def test_train_steps_match_single_steps(torch_gpu):
         L.check(lib.ssm_mlp_train_steps(D.ptr(dx), D.ptr(dy), 100, 1, 0, lrs.ctypes.data, 0.9, 0.999, 1e-8, 0.0,
                                         1.0, D.ptr(b.params), D.ptr(b.m), D.ptr(b.v), D.ptr(b.kern.frags),
                                         D.ptr(b.kern.bias), None, D.ptr(ws), ws_bytes, None, D.stream_ptr()))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_team_kernel_matches_single_wave_kernel_bit_for_bit(torch_gpu, mode):
+    """Without dx the library runs four waves per 16-row tile; with dx one wave per tile.
+    Both accumulate every element in the same order, so all shared outputs agree exactly."""
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+    from ssmhip.surrogate import MLPSurrogate
+
+    torch = torch_gpu
+    net = MLPSurrogate(seed=3)
+    n, ld = 1000, 1024
+    x_np = _inputs(n, seed=9)
+    dxin = D.to_dev(x_np)
+    net.set_input_normalisation(dxin)
+    tgt = D.to_dev(np.random.default_rng(4).normal(size=n).astype(np.float32))
+    sz = net.kern.sz
+    lib = L.load()
+    outs = []
+    for want_dx in (False, True):
+        acts = D.zeros((sz["act_rows"], ld), torch.bfloat16)
+        deltas = D.zeros((sz["delta_rows"], ld), torch.bfloat16)
+        y = D.zeros((n,), torch.float32)
+        loss = D.zeros((n,), torch.float32)
+        dx = D.zeros((n, 8), torch.float32) if want_dx else None
+        L.check(lib.ssm_mlp_backward(D.ptr(dxin), D.ptr(tgt), n, mode, 1.0, 1.0 / n, D.ptr(net.kern.frags),
+                                     D.ptr(net.kern.bias), D.ptr(net.kern.xnorm), D.ptr(acts), D.ptr(deltas), ld,
+                                     D.ptr(dx), D.ptr(y), D.ptr(loss), D.stream_ptr()))
+        torch.cuda.synchronize()
+        outs.append([t.float().cpu().numpy() for t in (y, loss, acts, deltas)])
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    assert np.abs(outs[0][3]).sum() > 0
