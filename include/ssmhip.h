@@ -247,8 +247,10 @@ int32_t ssm_theta_uniform(int32_t n_sets, int32_t n_params, const double* lo_dev
  *   [i*batch, (i+1)*batch) of x[n_steps*batch][7] / target[...], learning rate
  *   lr_host[i] (host array), bias corrections for steps step0+1 ...; per step
  *   the fused forward/backward, the weight gradient (summed over batch slices
- *   in a fixed order), Adam and the repack of frags/bias are enqueued from
- *   the host without a synchronisation. batch must be a multiple of 32;
+ *   in a fixed order) and Adam fused with the repack of frags/bias are
+ *   enqueued from the host without a synchronisation. frags_dev/bias_dev
+ *   must hold ssm_mlp_pack(params) on entry (the steps rewrite only the
+ *   entries of real parameters; padding stays zero). batch % 32 == 0;
  *   workspace_dev (16-byte aligned) holds at least
  *   ssm_mlp_train_workspace_bytes(batch) bytes (-1: invalid batch); loss_dev
  *   (nullable) receives the per-row Huber loss [n_steps*batch].             */

@@ -722,6 +722,69 @@ __global__ void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__
   P[i] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
 }
 
+// Adam step fused with the repack: each parameter, once updated, is written straight to
+// its forward and backward fragment slots (and its padded bias), the inverse of
+// mlp_pack_kernel's mapping, with the same scaling and bf16 rounding. Padding entries are
+// never touched (they stay zero from the initial pack).
+__device__ __forceinline__ void decode_chain(int f, int& b, int& g, int& j) {
+  // inverse of chain_feature: f = 16 (2b + j/4) + 4g + j%4
+  const int t = f >> 4, w4 = f & 15;
+  b = t >> 1;
+  g = w4 >> 2;
+  j = ((t & 1) << 2) | (w4 & 3);
+}
+
+__global__ void mlp_adam_pack_kernel(float* __restrict__ P, const float* __restrict__ G, int nsplit,
+                                     float* __restrict__ M, float* __restrict__ V, float lr, float b1, float b2,
+                                     float eps, float bc1, float bc2, float wd, __bf16* __restrict__ frags,
+                                     float* __restrict__ bias) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kNParams) return;
+  float gs = G[i];
+  for (int s = 1; s < nsplit; ++s) gs += G[(int64_t)s * kNParams + i];
+  const float gr = gs + wd * P[i];
+  const float m = b1 * M[i] + (1.f - b1) * gr;
+  const float v = b2 * V[i] + (1.f - b2) * gr * gr;
+  M[i] = m;
+  V[i] = v;
+  const float p = P[i] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+  P[i] = p;
+  int layer, out, in = -1;
+  if (i < oB1) { layer = 1; out = i / kIn; in = i % kIn; }
+  else if (i < oW2) { layer = 1; out = i - oB1; }
+  else if (i < oB2) { layer = 2; out = (i - oW2) / kH1; in = (i - oW2) % kH1; }
+  else if (i < oW3) { layer = 2; out = i - oB2; }
+  else if (i < oB3) { layer = 3; out = (i - oW3) / kH2; in = (i - oW3) % kH2; }
+  else if (i < oW4) { layer = 3; out = i - oB3; }
+  else if (i < oB4) { layer = 4; out = (i - oW4) / kH3; in = (i - oW4) % kH3; }
+  else { layer = 4; out = i - oB4; }
+  const float sc = layer < 4 ? kTanhScale : 1.0f;
+  if (in < 0) {
+    const int pb = layer == 1 ? pB1 : layer == 2 ? pB2 : layer == 3 ? pB3 : pB4;
+    bias[pb + out] = sc * p;
+    return;
+  }
+  // forward fragment: A[m = out][k = in]
+  {
+    const int base = layer == 1 ? fF1 : layer == 2 ? fF2 : layer == 3 ? fF3 : fF4;
+    const int KB = layer == 1 ? 1 : 4;
+    int b, g, j;
+    if (layer == 1) { b = 0; g = in >> 3; j = in & 7; }
+    else decode_chain(in, b, g, j);
+    const int a = out >> 4, lane = 16 * g + (out & 15);
+    frags[base + ((a * KB + b) * 64 + lane) * 8 + j] = (__bf16)(p * sc);
+  }
+  // backward fragment: A[m = in][k = out]
+  {
+    const int base = layer == 1 ? fB1 : layer == 2 ? fB2 : layer == 3 ? fB3 : fB4;
+    const int KB = layer == 4 ? 1 : 4;
+    int b, g, j;
+    decode_chain(out, b, g, j);
+    const int a = in >> 4, lane = 16 * g + (in & 15);
+    frags[base + ((a * KB + b) * 64 + lane) * 8 + j] = (__bf16)p;
+  }
+}
+
 // one MFMA with per-lane operands, to pin the operand / result lane layout
 __global__ void mfma_probe_kernel(const bf16x8* a, const bf16x8* b, f32x4* d) {
   const int l = threadIdx.x;
@@ -816,7 +879,7 @@ int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t s
 
 // ---- a run of Adam steps on consecutive batches, enqueued from the host ----------------------
 int64_t mlp_train_split(int64_t batch) {
-  int64_t sp = batch / 1024;
+  int64_t sp = batch / 512;
   return sp < 1 ? 1 : sp > 16 ? 16 : sp;
 }
 int64_t mlp_train_workspace_bytes(int64_t batch) {
@@ -848,9 +911,8 @@ int32_t mlp_train_steps_enqueue(const float* x, const float* target, int64_t bat
                        chunk, grad);
     const int32_t step = step0 + i + 1;
     const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, grad, (int)split, M, V,
-                       kNParams, lr[i], b1, b2, eps, bc1, bc2, wd);
-    hipLaunchKernelGGL(mlp_pack_kernel, dim3((kNFrag + 255) / 256), dim3(256), 0, s, P, frags, bias);
+    hipLaunchKernelGGL(mlp_adam_pack_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, P, grad, (int)split, M,
+                       V, lr[i], b1, b2, eps, bc1, bc2, wd, frags, bias);
     SSM_HIP(hipGetLastError());
   }
   return SSM_OK;

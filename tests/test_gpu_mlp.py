@@ -206,6 +206,12 @@ def test_train_steps_match_single_steps(torch_gpu):
     yb = b(dx[:1000]).cpu().numpy()
     assert np.max(np.abs(ya - yb)) < 1e-3
     assert np.isfinite(loss.cpu().numpy()).all()
+    # the fused Adam + repack leaves exactly the packing of the updated parameters
+    fr = torch.zeros_like(b.kern.frags)
+    bi = torch.zeros_like(b.kern.bias)
+    L.check(lib.ssm_mlp_pack(D.ptr(b.params), D.ptr(fr), D.ptr(bi), D.stream_ptr()))
+    torch.cuda.synchronize()
+    assert torch.equal(fr, b.kern.frags) and torch.equal(bi, b.kern.bias)
     with pytest.raises(ValueError):
         L.check(lib.ssm_mlp_train_steps(D.ptr(dx), D.ptr(dy), 100, 1, 0, lrs.ctypes.data, 0.9, 0.999, 1e-8, 0.0,
                                         1.0, D.ptr(b.params), D.ptr(b.m), D.ptr(b.v), D.ptr(b.kern.frags),
