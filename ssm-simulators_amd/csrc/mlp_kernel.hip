@@ -740,8 +740,14 @@ __global__ void mlp_adam_pack_kernel(float* __restrict__ P, const float* __restr
                                      float* __restrict__ bias) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kNParams) return;
-  float gs = G[i];
-  for (int s = 1; s < nsplit; ++s) gs += G[(int64_t)s * kNParams + i];
+  // issue every slice's load before summing them in slice order
+  float part[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) part[s] = s < nsplit ? G[(int64_t)s * kNParams + i] : 0.f;
+  float gs = part[0];
+#pragma unroll
+  for (int s = 1; s < 16; ++s)
+    if (s < nsplit) gs += part[s];
   const float gr = gs + wd * P[i];
   const float m = b1 * M[i] + (1.f - b1) * gr;
   const float v = b2 * V[i] + (1.f - b2) * gr * gr;
