@@ -68,7 +68,7 @@ def cpu_baseline_leg(theta, ns, target_seconds: float = 15.0):
     }
 
 
-def sample_config3(model: str, n_sets: int = 1024, seed: int = 3):
+def sample_accumulator_sets(model: str, n_sets: int = 1024, seed: int = 3):
     """Config-3 parameter sets: LCA-4 (v_i in [0,2.5], a in [1,3], z_i in [0,0.5], g,b in [-1,1], t in [0,2])
     or race-4 with a Weibull-collapsing bound (race_4 bounds, alpha in [0.3,5], beta in [0.3,7])."""
     rng = np.random.default_rng(seed)
@@ -97,7 +97,7 @@ def time_graph(sim, steps: int, warmup: int):
     return s.elapsed_time(e) / steps
 
 
-def config3_leg(args):
+def accumulator_leg(args):
     """LCA-4 and race-4/Weibull, 1024 sets x 20,000 samples each, against the CPU restatement."""
     from oracle import cpu_baseline
     from oracle import em_numpy as E
@@ -105,7 +105,7 @@ def config3_leg(args):
 
     out = {}
     for model, kind in (("lca_4", 2), ("race_4_weibull", 1)):
-        theta = sample_config3(model)
+        theta = sample_accumulator_sets(model)
         sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto", tune_sample=None)
         ms = time_graph(sim, max(2, args.steps), 1)
         steps = sim.steps()
@@ -140,21 +140,21 @@ def config3_leg(args):
                                    "sample": f"{idx.size} of 1024 sets (every 32nd) x {n_cpu} samples in {sec:.2f} s"}
             rec["gpu_over_cpu"] = rec["trajectories_per_s"] / rec["cpu_baseline"]["value"]
         out[model] = rec
-        log(f"config3 {model}: {rec['trajectories_per_s']:.3e} traj/s"
+        log(f"accumulators {model}: {rec['trajectories_per_s']:.3e} traj/s"
             + (f", {rec['gpu_over_cpu']:.0f}x CPU" if "gpu_over_cpu" in rec else ""))
     return out
 
 
-def config4_leg(args):
+def lan_leg(args):
     """KDE-labelled LAN rows: 1024 angle sets x 1e5 simulations -> 800 KDE + 200 uniform rows per set."""
     import torch
 
     from oracle import cpu_baseline, kde_ref
     from oracle import em_numpy as E
     from ssmhip import roofline
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
 
-    theta = sample_config4(1024, seed=1)
+    theta = sample_angle_uniform(1024, seed=1)
     # mixture [0.8, 0.2, 0.0] of 1000 rows: 800 KDE resamples + 200 uniform rows, no negative rows
     lab = LanKDELabeller("angle", theta, n_samples=100_000, n_kde=800, n_unif=200, n_neg=0, seed=4, launch="auto",
                          tune_sample=None)
@@ -201,11 +201,11 @@ def config4_leg(args):
                                "sample": f"{nsets} sets: 1e5 simulations + 1000 exact float64 KDE evaluations each, "
                                          f"{sec:.2f} s per set"}
         rec["gpu_over_cpu"] = rec["rows_per_s"] / rec["cpu_baseline"]["value"]
-    log(f"config4: {rec['rows_per_s']:.3e} rows/s, KDE {rec['kde_evals_per_s']:.3e} evals/s")
+    log(f"lan rows: {rec['rows_per_s']:.3e} rows/s, KDE {rec['kde_evals_per_s']:.3e} evals/s")
     return rec, lab, rows
 
 
-def config5_leg(args, lab, kde_evals_per_s):
+def surrogate_leg(args, lab, kde_evals_per_s):
     """Surrogate: label rows of --surrogate-sets angle sets, train, then 4M-row inference against
     the GPU KDE and accuracy against the analytic DDM density."""
     import torch
@@ -214,17 +214,17 @@ def config5_leg(args, lab, kde_evals_per_s):
     from ssmhip import roofline
     from ssmhip.surrogate import MLPSurrogate
 
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
 
     # training rows from the same pipeline at a larger simulation budget per set (1e6 draws:
-    # the KDE labels' own error falls with the bandwidth), angle sets outside the config-4 sample
+    # the KDE labels' own error falls with the bandwidth), angle sets outside the LAN-rows sample
     parts = []
     done = 0
     torch.cuda.synchronize()
     t_lab = time.perf_counter()
     while done < args.surrogate_sets:
         k = min(2048, args.surrogate_sets - done)
-        more = LanKDELabeller("angle", sample_config4(k, seed=100 + done), n_samples=args.surrogate_samples,
+        more = LanKDELabeller("angle", sample_angle_uniform(k, seed=100 + done), n_samples=args.surrogate_samples,
                               n_kde=800, n_unif=200, n_neg=0,
                               seed=200 + done, launch=lab.sim.launch_params)
         parts.append(more.run())
@@ -316,7 +316,7 @@ def config5_leg(args, lab, kde_evals_per_s):
            "over_gpu_kde": evals / kde_evals_per_s,
            "mfma_tflops_algorithmic": flops / 1e12, "mfma_frac_of_bf16_peak": flops / 2.5e15,
            "mae_vs_analytic_theta0": mae, "kde_mae_vs_analytic_theta0": kde_mae, "analytic_rows": int(ref.size)}
-    log(f"config5: {evals:.3e} evals/s ({rec['over_gpu_kde']:.0f}x GPU KDE), MAE vs analytic {mae:.3f} nats "
+    log(f"surrogate: {evals:.3e} evals/s ({rec['over_gpu_kde']:.0f}x GPU KDE), MAE vs analytic {mae:.3f} nats "
         f"(KDE labels: {kde_mae:.3f})")
     return rec
 
@@ -331,9 +331,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--launch", type=str, default="auto", help="'auto' or block,blocks_per_cu,refill_min")
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU roof probe")
-    ap.add_argument("--headline-only", action="store_true", help="skip the config-3/4/5 legs")
-    ap.add_argument("--legs", type=str, default="headline,config3,config4,config5",
-                    help="comma list of legs to run (config5 needs config4); without 'headline' the "
+    ap.add_argument("--headline-only", action="store_true", help="run the headline leg only")
+    ap.add_argument("--legs", type=str, default="headline,accumulators,lan,surrogate",
+                    help="comma list of legs to run (surrogate needs lan); without 'headline' the "
                          "JSON line carries no headline value (for profiling one leg)")
     ap.add_argument("--surrogate-epochs", type=int, default=150)
     ap.add_argument("--surrogate-sets", type=int, default=32768, help="parameter sets of KDE rows to train on")
@@ -354,23 +354,23 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     legs = {"headline"} if args.headline_only else set(args.legs.split(","))
-    unknown = legs - {"headline", "config3", "config4", "config5"}
-    if unknown or ("config5" in legs and "config4" not in legs):
-        print(f"bad --legs {args.legs!r}: legs are headline, config3, config4, config5 (config5 needs config4)",
+    unknown = legs - {"headline", "accumulators", "lan", "surrogate"}
+    if unknown or ("surrogate" in legs and "lan" not in legs):
+        print(f"bad --legs {args.legs!r}: legs are headline, accumulators, lan, surrogate (surrogate needs lan)",
               file=sys.stderr)
         return 2
     out = {"metric": "angle-DDM trajectories/s", "value": None, "unit": "trajectories/s", "n_gpus": 1,
            "legs": sorted(legs)}
     if "headline" in legs:
         out = headline_leg(args, torch, batched, roofline)
-    if "config3" in legs:
-        out["config3"] = config3_leg(args)
-    if "config4" in legs:
-        c4, lab, rows = config4_leg(args)
-        out["config4"] = c4
-        if "config5" in legs:
+    if "accumulators" in legs:
+        out["accumulators"] = accumulator_leg(args)
+    if "lan" in legs:
+        c4, lab, rows = lan_leg(args)
+        out["lan_rows"] = c4
+        if "surrogate" in legs:
             del rows
-            out["config5"] = config5_leg(args, lab, c4["kde_evals_per_s"])
+            out["surrogate"] = surrogate_leg(args, lab, c4["kde_evals_per_s"])
     print(json.dumps(out), flush=True)
     return 0
 
@@ -392,8 +392,8 @@ def headline_leg(args, torch, batched, roofline) -> dict:
     """Batched angle-DDM, 4096 ragged sets, one persistent hipGraph launch per step."""
     from ssmhip.em import LaunchParams
 
-    theta, ns = batched.sample_config2(args.n_sets, seed=0)
-    log(f"config2: {args.n_sets} sets, {int(ns.sum())} trajectories")
+    theta, ns = batched.sample_angle_ragged(args.n_sets, seed=0)
+    log(f"headline: {args.n_sets} angle sets, {int(ns.sum())} trajectories")
     launch = "auto" if args.launch == "auto" else LaunchParams(*[int(x) for x in args.launch.split(",")])
     # the headline batch tunes its launch shape on itself (~3 s), not on a sample
     sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch=launch,

@@ -156,7 +156,7 @@ class BatchedSimulation:
         return self.rts[lo:hi].cpu().numpy(), self.choices[lo:hi].cpu().numpy()
 
 
-def sample_config2(n_sets: int = 4096, seed: int = 0, lo: float = 1e3, hi: float = 1e5):
+def sample_angle_ragged(n_sets: int = 4096, seed: int = 0, lo: float = 1e3, hi: float = 1e5):
     """Batched angle-DDM workload: theta ~ U(angle bounds), n_i = floor(10^U(log10 lo, log10 hi))."""
     rng = np.random.default_rng(seed)
     lower = np.array([-3.0, 0.3, 0.1, 1e-3, -0.1])

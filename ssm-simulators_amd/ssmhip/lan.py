@@ -223,7 +223,7 @@ def sample_theta(n_sets: int, lower, upper, seed: int = 0) -> np.ndarray:
     return out.cpu().numpy()
 
 
-def sample_config4(n_sets: int = 1024, seed: int = 1):
+def sample_angle_uniform(n_sets: int = 1024, seed: int = 1):
     """1024 angle-DDM parameter sets, uniform within the angle bounds."""
     rng = np.random.default_rng(seed)
     lower = np.array([-3.0, 0.3, 0.1, 1e-3, -0.1])

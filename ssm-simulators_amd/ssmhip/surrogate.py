@@ -6,7 +6,7 @@ master weights) behind a ``torch.autograd.Function``.
 The reference generates the training rows (ResampleMixtureStrategy,
 ssms/dataset_generators/strategies/resample_mixture_strategy.py:64-214) but
 has no network; the KDE path stays the fallback outside the training bounds
-and the judge of the surrogate's error (``SurrogateLikelihood``).
+and the check of the surrogate's error (``SurrogateLikelihood``).
 """
 
 from __future__ import annotations
@@ -315,7 +315,7 @@ class MLPSurrogate:
 
 
 class SurrogateLikelihood:
-    """Surrogate log-likelihood with the KDE path as fallback and judge.
+    """Surrogate log-likelihood with the KDE path as fallback and as the check of its error.
 
     Rows outside the surrogate's training input box (or every row, when the
     validation error exceeds `max_val_mae`) are labelled by simulating their

@@ -49,9 +49,9 @@ def test_aux_counts_match_oracle_exactly(torch_gpu, choice_bytes):
 
 
 def test_labeller_output_dict_matches_oracle(torch_gpu):
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
 
-    theta = sample_config4(8, seed=3)
+    theta = sample_angle_uniform(8, seed=3)
     lab = LanKDELabeller("angle", theta, n_samples=20000, launch=None, seed=9)
     rows = lab.run()
     out = lab.output_dict(rows)

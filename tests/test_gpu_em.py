@@ -244,7 +244,7 @@ def _compare_samples(r_gpu, c_gpu, r_cpu, c_cpu, choices):
             _quantile_check(x, y)
 
 
-def test_ddm_native_config1_matches_cpu_restatement_and_closed_form(torch_gpu):
+def test_ddm_native_single_set_1e5_matches_cpu_restatement_and_closed_form(torch_gpu):
     import torch
 
     L, em = _ssm()

@@ -149,9 +149,9 @@ def test_logkde_dropin_contract(torch_gpu):
 
 
 def test_lan_rows_small_batch_against_oracle(torch_gpu):
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
 
-    theta = sample_config4(6, seed=7)
+    theta = sample_angle_uniform(6, seed=7)
     lab = LanKDELabeller("angle", theta, n_samples=20000, launch=None, seed=5)
     rows = lab.run().cpu().numpy()
     assert rows.shape == (6 * 1000, 8)
@@ -178,9 +178,9 @@ def test_lan_rows_small_batch_against_oracle(torch_gpu):
 
 
 def test_lan_rows_shuffle_is_a_permutation(torch_gpu):
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
 
-    theta = sample_config4(3, seed=8)
+    theta = sample_angle_uniform(3, seed=8)
     a = LanKDELabeller("angle", theta, n_samples=5000, launch=None, seed=2, shuffle=False)
     ra = a.run().cpu().numpy()
     b = LanKDELabeller("angle", theta, n_samples=5000, launch=None, seed=2, shuffle=True)

@@ -110,7 +110,7 @@ def test_graph_replay_matches_eager_launch(torch_gpu):
 
     from ssmhip import batched
 
-    theta, ns = batched.sample_config2(64, seed=3, lo=100, hi=2000)
+    theta, ns = batched.sample_angle_ragged(64, seed=3, lo=100, hi=2000)
     sim = batched.BatchedSimulation("angle", theta, ns, seed=7, launch=None)
     sim.run()
     torch.cuda.synchronize()

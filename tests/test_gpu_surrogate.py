@@ -10,10 +10,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def trained(torch_gpu):
-    from ssmhip.lan import LanKDELabeller, sample_config4
+    from ssmhip.lan import LanKDELabeller, sample_angle_uniform
     from ssmhip.surrogate import MLPSurrogate
 
-    theta = sample_config4(512, seed=21)
+    theta = sample_angle_uniform(512, seed=21)
     lab = LanKDELabeller("angle", theta, n_samples=50_000, seed=3, launch=None)
     rows = lab.run()
     net = MLPSurrogate(seed=1)

@@ -71,7 +71,7 @@ def test_model_configs_and_deadline_variant():
 def test_batched_records_shape_checks():
     from ssmhip import batched
 
-    theta, ns = batched.sample_config2(8, seed=0)
+    theta, ns = batched.sample_angle_ragged(8, seed=0)
     assert theta.shape == (8, 5) and np.all((ns >= 1000) & (ns <= 100000))
     with pytest.raises(ValueError):
         batched.records_for("angle", theta[:, :4], 0.001, 20.0)

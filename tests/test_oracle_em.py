@@ -37,7 +37,7 @@ def test_analytic_density_integrates_to_absorption_probability():
     lo, _ = integrate.quad(lambda t: ddm_analytic.density(np.array([t]), v, a, z, -1)[0], 0, 30, limit=200)
     assert abs(up - ddm_analytic.p_upper(v, a, z)) < 1e-6
     assert abs(up + lo - 1.0) < 1e-6
-    # config 1 closed forms
+    # closed forms at v=1, a=1.5, z=0.5, t=0.2
     assert abs(ddm_analytic.p_upper(1.0, 1.5, 0.5) - 0.952574) < 1e-6
     assert abs(ddm_analytic.mean_decision_time(1.0, 1.5, 0.5) - 1.35772) < 1e-5
 
@@ -176,7 +176,7 @@ def test_cpu_restatement_replay_matches_numpy_oracle_accumulators(model):
 
 
 def test_cpu_baseline_native_matches_closed_form():
-    # MT19937 + Ziggurat path of the C++ restatement against the Wiener closed form (config 1)
+    # MT19937 + Ziggurat path of the C++ restatement against the Wiener closed form (v=1, a=1.5, z=0.5)
     v, a, z = 1.0, 1.5, 0.5
     K, N = 1, 50000
     table = E.boundary_table("constant", {"a": [a]}, K, 20.0, DT)
