@@ -136,7 +136,7 @@ int64_t cpu_em_simulate(int model, int P, int n_sets, const int64_t* offsets, co
       int choice = 0;
       if (model == 0) {
         const float vdt = v[k] * dt, s = sst[k];
-        float y = (-b0) + (z[k] * 2.0f * b0);
+        float y = (float)((double)(-b0) + (double)z[k] * 2.0 * (double)b0);  // 2.0 is a C double
         for (;;) {
           if (ix >= n_steps) break;
           const float bb = row[ix];
@@ -185,8 +185,8 @@ int64_t cpu_em_simulate(int model, int P, int n_sets, const int64_t* offsets, co
       float su = 0.f;
       if (smooth_unif) {
         const float u = replay_uniforms ? replay_uniforms[i] : rng.uni();
-        if (t == 0.f) su = u * 0.5f * dt;
-        else if (t < dlt) su = (0.5f - u) * dt;
+        if (t == 0.f) su = (float)((double)u * 0.5 * (double)dt);  // 0.5 is a C double
+        else if (t < dlt) su = (float)((0.5 - (double)u) * (double)dt);
       }
       float rt = t + t0[k] + su;
       if (rt >= deadline[k] || deadline[k] <= 0.f) rt = -999.f;

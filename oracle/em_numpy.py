@@ -75,9 +75,11 @@ def sqrt_dt32(delta_t: float) -> np.float32:
 
 
 def smooth_unif32(t: np.ndarray, dl_tmp: np.ndarray, dt: np.float32, u: np.ndarray) -> np.ndarray:
-    """compute_smooth_unif (_utils.pyx:668-695), float32."""
-    half = (u * F32(0.5)).astype(F32) * dt
-    mid = ((F32(0.5) - u).astype(F32) * dt).astype(F32)
+    """compute_smooth_unif (_utils.pyx:668-695): float arguments, but the 0.5 literals are C
+    doubles, so each branch is formed in double and rounded to float on return."""
+    u64 = np.asarray(u, F32).astype(np.float64)
+    half = (u64 * 0.5 * np.float64(dt)).astype(F32)
+    mid = ((0.5 - u64) * np.float64(dt)).astype(F32)
     return np.where(t == F32(0.0), half, np.where(t < dl_tmp, mid, F32(0.0))).astype(F32)
 
 
@@ -118,7 +120,9 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     total = K * N
     lane_set = np.repeat(np.arange(K), N)
     b0 = B[lane_set, 0]
-    y = ((-b0) + ((z[lane_set] * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
+    # :719 in C: (-1) * b0 is float, z * 2.0 * b0 double (2.0 is a double literal); the sum is
+    # double, rounded to float on assignment
+    y = ((-b0).astype(np.float64) + (z[lane_set].astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
     tt = np.zeros(total, F32)
     ix = np.zeros(total, np.int64)
     src = _Normals(normals, rng)

@@ -125,8 +125,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             u = u32_to_unit(r.x);
           }
           // compute_smooth_unif, _utils.pyx:668-695
-          if (t == 0.f) su = u * 0.5f * dt;
-          else if (t < dl_tmp) su = (0.5f - u) * dt;
+          // the 0.5 literals are C doubles: formed in double, rounded to float once
+          if (t == 0.f) su = (float)((double)u * 0.5 * (double)dt);
+          else if (t < dl_tmp) su = (float)((0.5 - (double)u) * (double)dt);
           else su = 0.f;
         }
         float rt = t + t0 + su;
@@ -217,7 +218,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               const float z = sp[SSM_P_Z];
               vdt[0] = sp[SSM_P_V] * dt;
               sst[0] = sp[SSM_P_SQRT_ST];
-              x[0] = (-b0) + (z * 2.0f * b0);  // ddm_models.pyx:719
+              // ddm_models.pyx:719: the literal 2.0 is a C double, so the sum is formed in
+              // double and rounded to float once
+              x[0] = (float)((double)(-b0) + (double)z * 2.0 * (double)b0);
             } else {
               leak = sp[SSM_P_G];
               inh = sp[SSM_P_BINH];
