@@ -39,6 +39,10 @@ extern "C" {
 #define SSM_MODEL_DDM 0   /* ddm_flexbound: bounds +-b(t), start -b0 + 2 z b0 */
 #define SSM_MODEL_RACE 1  /* race_model: N accumulators clamped at 0           */
 #define SSM_MODEL_LCA 2   /* lca: race + leak g and lateral inhibition b       */
+#define SSM_MODEL_FULL_DDM 3 /* full_ddm: DDM + per-trajectory v, z, t variability
+                                (src/cssm/ddm_models.pyx:2177-2360)            */
+#define SSM_MODEL_ORNSTEIN 4 /* ornstein_uhlenbeck: DDM with leak, step
+                                ((v - g y) dt) + sqrt_st g (ornstein_models.pyx:47-200) */
 
 #define SSM_BOUNDARY_CONSTANT 0 /* b(t) = a                                  */
 #define SSM_BOUNDARY_ANGLE 1    /* b(t) = a + t_s * slope, slope = -sin/cos  */
@@ -58,8 +62,11 @@ extern "C" {
 #define SSM_P_Z 6            /* DDM relative start z                          */
 #define SSM_P_V 7            /* DDM drift v                                   */
 #define SSM_P_SQRT_ST 8      /* DDM float32(sqrt(dt)) * s                     */
-#define SSM_P_G 9            /* LCA leak g                                    */
+#define SSM_P_G 9            /* LCA / Ornstein leak g                         */
 #define SSM_P_BINH 10        /* LCA inhibition b                              */
+#define SSM_P_SZ 11          /* full DDM start variability sz                 */
+#define SSM_P_SV 12          /* full DDM drift variability sv                 */
+#define SSM_P_ST 13          /* full DDM non-decision-time variability st     */
 #define SSM_P_VJ 16          /* race/LCA v_j, j < 16                          */
 #define SSM_P_ZJ 32          /* race/LCA z_j                                  */
 #define SSM_P_SSTJ 48        /* race/LCA float32(sqrt(dt)) * s_j              */
@@ -103,8 +110,10 @@ int32_t ssm_device_cu_count(void);
  *   table_dev        float32[n_sets][n_steps] when boundary == TABLE, else NULL
  *   replay_normals_dev float32[n_total][replay_stride] (replay mode) or NULL;
  *                    draw d of trajectory i reads [i][d % replay_stride],
- *                    d = step * n_particles + particle
- *   replay_uniforms_dev float32[n_total] (replay + smooth_unif) or NULL
+ *                    d = step * n_particles + particle (full DDM: [i][0] is
+ *                    the drift draw and step d reads [i][(1 + d) % stride])
+ *   replay_uniforms_dev float32[n_total] (replay + smooth_unif) or NULL;
+ *                    full DDM replay: float32[n_total][3] = (u_t, u_z, u_smooth)
  *   rts_dev          float32[n_total]; choices_dev int8/int32[n_total]
  *   workspace_dev    uint64[SSM_EM_WORKSPACE_WORDS] scratch owned by the caller
  *                    ([0] work-queue head, [1] executed EM steps)

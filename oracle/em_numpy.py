@@ -3,6 +3,8 @@
 Follows, operation by operation in float32:
   ddm_flexbound  src/cssm/ddm_models.pyx:696-744  (start :719, walk :721-733,
                  smooth_unif :735-739, rt :741, sign :742, deadline :744)
+  full_ddm       src/cssm/ddm_models.pyx:2284-2358 (variability of v, z, t)
+  ornstein_uhlenbeck  src/cssm/ornstein_models.pyx:156-198 (leaky drift)
   race_model     src/cssm/race_models.pyx:417-464
   lca            src/cssm/race_models.pyx:825-879
   lba_vanilla / lba_angle  src/cssm/lba_models.pyx:101-115, 212-224
@@ -105,8 +107,19 @@ def _finish(rts, t_el, t0, dl_tmp, deadline, dt, smooth, uniforms):
 
 
 def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
-                  bparams=None, smooth_unif=False, normals=None, uniforms=None, rng=None, table=None):
-    """Returns (rts float32[n_total], choices int32[n_total], steps int64[n_total]), set-major."""
+                  bparams=None, smooth_unif=False, normals=None, uniforms=None, rng=None, table=None,
+                  sz=None, sv=None, st=None, g=None):
+    """Returns (rts float32[n_total], choices int32[n_total], steps int64[n_total]), set-major.
+
+    With sz/sv/st: full_ddm (src/cssm/ddm_models.pyx:2284-2358): per trajectory the drift
+    increment (v + sv gv) dt (float), t_tmp = t + 2 (u_t - 0.5) st and the start shift
+    2 (u_z - 0.5) sz (both formed in double: 0.5 is a C double), rt = t_particle + t_tmp +
+    smooth_u, deadline_tmp from the set's t. Replay: normals[i, 0] is gv and step d reads
+    normals[i, (1 + d) % stride]; uniforms[i] = (u_t, u_z, u_smooth).
+    With g: ornstein_uhlenbeck (src/cssm/ornstein_models.pyx:156-198): float start
+    -b0 + (2z) b0 (the literal 2 is an int) and step ((v - g y) dt) + sqrt_st gm."""
+    full = sz is not None or sv is not None or st is not None
+    ou = g is not None
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     K = v.shape[0]
     N = int(n_samples)
@@ -122,7 +135,30 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     b0 = B[lane_set, 0]
     # :719 in C: (-1) * b0 is float, z * 2.0 * b0 double (2.0 is a double literal); the sum is
     # double, rounded to float on assignment
-    y = ((-b0).astype(np.float64) + (z[lane_set].astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
+    if ou:
+        y = ((-b0) + ((z[lane_set] * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
+    else:
+        y = ((-b0).astype(np.float64) + (z[lane_set].astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
+    t_lane = t[lane_set].copy()
+    vdt_lane = vdt_k[lane_set].copy()
+    draw_off = 0
+    if full:
+        sz_, sv_, st_ = (np.zeros(K, F32) if x is None else np.asarray(x, F32).reshape(-1) for x in (sz, sv, st))
+        if normals is not None:
+            gv = np.asarray(normals, F32)[:, 0]
+            uu = np.asarray(uniforms, F32).reshape(total, 3)
+            u_t, u_z = uu[:, 0], uu[:, 1]
+            draw_off = 1
+        else:
+            gv = rng.standard_normal(total, dtype=F32)
+            u_t = rng.uniform(0.0, 1.0, total).astype(F32)
+            u_z = rng.uniform(0.0, 1.0, total).astype(F32)
+        vdt_lane = ((v[lane_set] + (sv_[lane_set] * gv).astype(F32)).astype(F32) * dt).astype(F32)
+        t_lane = (t[lane_set].astype(np.float64) + (2.0 * (u_t.astype(np.float64) - 0.5))
+                  * st_[lane_set].astype(np.float64)).astype(F32)
+        y = (y.astype(np.float64) + (2.0 * (u_z.astype(np.float64) - 0.5)) * sz_[lane_set].astype(np.float64)).astype(F32)
+    if ou:
+        g_k = np.asarray(g, F32).reshape(-1)
     tt = np.zeros(total, F32)
     ix = np.zeros(total, np.int64)
     src = _Normals(normals, rng)
@@ -138,15 +174,26 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
         if not active.size:
             break
         ks = lane_set[active]
-        g = src.draw(active, ix[active])
-        inc = (vdt_k[ks] + (sqrt_st_k[ks] * g).astype(F32)).astype(F32)
+        gm = src.draw(active, ix[active] + draw_off)
+        noise = (sqrt_st_k[ks] * gm).astype(F32)
+        if ou:
+            ya = y[active]
+            drift = ((v[ks] - (g_k[ks] * ya).astype(F32)).astype(F32) * dt).astype(F32)
+            inc = (drift + noise).astype(F32)
+        else:
+            inc = (vdt_lane[active] + noise).astype(F32)
         y[active] = (y[active] + inc).astype(F32)
         tt[active] = (tt[active] + dt).astype(F32)
         ix[active] += 1
-    u = np.zeros(total, F32) if uniforms is None else np.asarray(uniforms, F32).reshape(-1)
+    if uniforms is None:
+        u = np.zeros(total, F32)
+    elif full:
+        u = np.asarray(uniforms, F32).reshape(total, 3)[:, 2]
+    else:
+        u = np.asarray(uniforms, F32).reshape(-1)
     if smooth_unif and uniforms is None:
         u = (rng.uniform(0.0, 1.0, total)).astype(F32)
-    rts = _finish(None, tt, t[lane_set], dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
+    rts = _finish(None, tt, t_lane, dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
     choices = ((y > 0).astype(np.int32) - (y < 0).astype(np.int32)).astype(np.int32)
     return rts, choices, ix
 

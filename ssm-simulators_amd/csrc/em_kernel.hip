@@ -79,9 +79,23 @@ __device__ __forceinline__ int find_set(const int64_t* __restrict__ off, int n_s
   return lo;
 }
 
+// Position increment of one step of the single-particle diffusions, in the reference's order:
+//   DDM, full DDM   (v dt) + (sqrt_st g)             ddm_models.pyx:720, :2335
+//   Ornstein        ((v - g y) dt) + (sqrt_st g)     ornstein_models.pyx:173
+template <int MODEL>
+__device__ __forceinline__ float step_increment(float y, float vdt, float v, float leak, float dt, float sst,
+                                                float nz) {
+  if constexpr (MODEL == SSM_MODEL_ORNSTEIN) return ((v - (leak * y)) * dt) + (sst * nz);
+  else return vdt + sst * nz;
+}
+
+// the single-particle diffusions that share the DDM walk
+template <int MODEL>
+constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM || MODEL == SSM_MODEL_ORNSTEIN;
+
 template <int MODEL, int BOUND, bool REPLAY, int MAXP>
 __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
-  const int P = (MODEL == SSM_MODEL_DDM) ? 1 : A.n_particles;
+  const int P = kDiffusion<MODEL> ? 1 : A.n_particles;
   const float dt = A.dt;
 
   int state = kEmpty;
@@ -119,7 +133,8 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         if (A.smooth_unif) {
           float u;
           if constexpr (REPLAY) {
-            u = A.replay_uniforms[g];
+            // full DDM rows carry (u_t, u_z, u_smooth); the others one value per trajectory
+            u = MODEL == SSM_MODEL_FULL_DDM ? A.replay_uniforms[g * 3 + 2] : A.replay_uniforms[g];
           } else {
             u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, 0u, 1u}, A.k0, A.k1);
             u = u32_to_unit(r.x);
@@ -133,7 +148,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         float rt = t + t0 + su;
         if (rt >= deadline || deadline <= 0.f) rt = -999.f;  // enforce_deadline, _utils.pyx:697-710
         int choice;
-        if constexpr (MODEL == SSM_MODEL_DDM) {
+        if constexpr (kDiffusion<MODEL>) {
           choice = (x[0] > 0.f) - (x[0] < 0.f);  // sign(), _utils.pyx:474-484
         } else {
           choice = 0;  // np.argmax: first maximum
@@ -214,13 +229,41 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             if constexpr (BOUND == SSM_BOUNDARY_WEIBULL) b2 = 1.0f / b2;
             if constexpr (BOUND == SSM_BOUNDARY_TABLE) row = A.table + (int64_t)k * A.n_steps;
             const float b0 = boundary_at<BOUND>(0, dt, a, b1, b2, row);
-            if constexpr (MODEL == SSM_MODEL_DDM) {
+            if constexpr (kDiffusion<MODEL>) {
               const float z = sp[SSM_P_Z];
               vdt[0] = sp[SSM_P_V] * dt;
               sst[0] = sp[SSM_P_SQRT_ST];
-              // ddm_models.pyx:719: the literal 2.0 is a C double, so the sum is formed in
-              // double and rounded to float once
-              x[0] = (float)((double)(-b0) + (double)z * 2.0 * (double)b0);
+              if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
+                // ornstein_models.pyx:165: the literal 2 is an int, so this stays in float
+                x[0] = (-b0) + ((z * 2.0f) * b0);
+                leak = sp[SSM_P_G];
+                vraw[0] = sp[SSM_P_V];
+              } else {
+                // ddm_models.pyx:719 / :2291: the literal 2.0 is a C double, so the sum is
+                // formed in double and rounded to float once
+                x[0] = (float)((double)(-b0) + (double)z * 2.0 * (double)b0);
+              }
+              if constexpr (MODEL == SSM_MODEL_FULL_DDM) {
+                // trial-to-trial variability, ddm_models.pyx:2294-2313: drift v + sv g,
+                // non-decision time t + 2 (u_t - 1/2) st, start + 2 (u_z - 1/2) sz; the
+                // uniform terms are formed in double (0.5 is a C double)
+                float gv, ut, uz;
+                if constexpr (REPLAY) {
+                  gv = A.replay_normals[g * A.replay_stride];
+                  ut = A.replay_uniforms[g * 3 + 0];
+                  uz = A.replay_uniforms[g * 3 + 1];
+                } else {
+                  const u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, 0u, 3u}, A.k0, A.k1);
+                  float z0, z1;
+                  box_muller(r.x, r.y, z0, z1);
+                  gv = z0;
+                  ut = u32_to_unit(r.z);
+                  uz = u32_to_unit(r.w);
+                }
+                vdt[0] = (sp[SSM_P_V] + sp[SSM_P_SV] * gv) * dt;
+                t0 = (float)((double)t0 + (2.0 * ((double)ut - 0.5)) * (double)sp[SSM_P_ST]);
+                x[0] = (float)((double)x[0] + (2.0 * ((double)uz - 0.5)) * (double)sp[SSM_P_SZ]);
+              }
             } else {
               leak = sp[SSM_P_G];
               inh = sp[SSM_P_BINH];
@@ -251,13 +294,15 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     // ---------------- advance running lanes ----------------
     ++iters;
     if (state == kRunning) {
-      if constexpr (MODEL == SSM_MODEL_DDM) {
+      if constexpr (kDiffusion<MODEL>) {
         // four steps per Philox block; every walk starts block-aligned
         float nz[4];
         if constexpr (REPLAY) {
+          // a full-DDM row starts with the drift draw, the walk's draws follow it
+          constexpr int off = MODEL == SSM_MODEL_FULL_DDM ? 1 : 0;
           const float* rb = A.replay_normals + g * A.replay_stride;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) nz[i] = rb[(int64_t)(ix + i) % A.replay_stride];
+          for (int i = 0; i < 4; ++i) nz[i] = rb[(int64_t)(off + ix + i) % A.replay_stride];
         } else {
           philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
         }
@@ -271,7 +316,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             if (alive) {
               const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
               if ((x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp)) {
-                x[0] += vdt[0] + sst[0] * nz[i];  // ddm_models.pyx:720
+                x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                 t += dt;
                 ++ix;
               } else {
@@ -289,7 +334,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                 go = (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
               }
               if (go) {
-                x[0] += vdt[0] + sst[0] * nz[i];
+                x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                 t += dt;
                 ++ix;
               } else {
@@ -381,7 +426,7 @@ using KernelFn = void (*)(EmArgs);
 
 template <int MODEL, int BOUND, bool REPLAY>
 static KernelFn pick_p(int p) {
-  if constexpr (MODEL == SSM_MODEL_DDM) {
+  if constexpr (kDiffusion<MODEL>) {
     return em_kernel<MODEL, BOUND, REPLAY, 1>;
   } else {
     if (p <= 4) return em_kernel<MODEL, BOUND, REPLAY, 4>;
@@ -402,6 +447,10 @@ static KernelFn pick_bound(int bound, int p) {
 
 static KernelFn pick_kernel(int model, int bound, int p, bool replay) {
   if (model == SSM_MODEL_DDM) return replay ? pick_bound<SSM_MODEL_DDM, true>(bound, p) : pick_bound<SSM_MODEL_DDM, false>(bound, p);
+  if (model == SSM_MODEL_FULL_DDM)
+    return replay ? pick_bound<SSM_MODEL_FULL_DDM, true>(bound, p) : pick_bound<SSM_MODEL_FULL_DDM, false>(bound, p);
+  if (model == SSM_MODEL_ORNSTEIN)
+    return replay ? pick_bound<SSM_MODEL_ORNSTEIN, true>(bound, p) : pick_bound<SSM_MODEL_ORNSTEIN, false>(bound, p);
   if (model == SSM_MODEL_RACE) return replay ? pick_bound<SSM_MODEL_RACE, true>(bound, p) : pick_bound<SSM_MODEL_RACE, false>(bound, p);
   return replay ? pick_bound<SSM_MODEL_LCA, true>(bound, p) : pick_bound<SSM_MODEL_LCA, false>(bound, p);
 }
@@ -410,9 +459,10 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
                     const float* table, const float* rn, const float* ru, float* rts, void* choices,
                     uint64_t* ws) {
   if (!P) return fail(SSM_EINVAL, "problem descriptor is NULL");
-  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_LCA) return fail(SSM_EINVAL, "unknown model");
+  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_ORNSTEIN) return fail(SSM_EINVAL, "unknown model");
   if (P->boundary < SSM_BOUNDARY_CONSTANT || P->boundary > SSM_BOUNDARY_TABLE) return fail(SSM_EINVAL, "unknown boundary");
-  if (P->model == SSM_MODEL_DDM && P->n_particles != 1) return fail(SSM_EINVAL, "DDM requires n_particles == 1");
+  const bool diffusion = P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM || P->model == SSM_MODEL_ORNSTEIN;
+  if (diffusion && P->n_particles != 1) return fail(SSM_EINVAL, "single-particle models require n_particles == 1");
   if (P->n_particles < 1 || P->n_particles > SSM_MAX_PARTICLES)
     return fail(SSM_EINVAL, "n_particles must be in [1, 16]");
   if (P->n_sets < 0 || P->n_total < 0) return fail(SSM_EINVAL, "negative sizes");
@@ -429,6 +479,8 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
     if (P->boundary == SSM_BOUNDARY_TABLE && !table) return fail(SSM_EINVAL, "TABLE boundary needs a table");
     if (P->replay_stride > 0 && !rn) return fail(SSM_EINVAL, "replay mode needs a normals buffer");
     if (P->replay_stride > 0 && P->smooth_unif && !ru) return fail(SSM_EINVAL, "replay + smooth_unif needs uniforms");
+    if (P->replay_stride > 0 && P->model == SSM_MODEL_FULL_DDM && !ru)
+      return fail(SSM_EINVAL, "full-DDM replay needs uniforms (u_t, u_z, u_smooth per trajectory)");
   }
   return SSM_OK;
 }

@@ -62,13 +62,13 @@ def _lba_angle_3_transforms():
     return [f]
 
 
-def _ddm_like(name, params, bounds, defaults, bname):
+def _ddm_like(name, params, bounds, defaults, bname, simulator=None):
     fun, bparams, _ = bf.BOUNDARY_REGISTRY[bname]
     return {
         "name": name, "params": params, "param_bounds": bounds, "boundary_name": bname, "boundary": fun,
         "boundary_params": [] if bname == "constant" else None, "n_params": len(params),
         "default_params": defaults, "nchoices": 2, "choices": [-1, 1], "n_particles": 1,
-        "simulator": cssm.ddm_flexbound, "simulation_transforms": [],
+        "simulator": simulator or cssm.ddm_flexbound, "simulation_transforms": [],
     }
 
 
@@ -111,6 +111,15 @@ def _registry() -> dict:
         "weibull": _ddm_like("weibull", ["v", "a", "z", "t", "alpha", "beta"],
                              [[-2.5, 0.3, 0.2, 1e-3, 0.31, 0.31], [2.5, 2.5, 0.8, 2.0, 4.99, 6.99]],
                              [0.0, 1.0, 0.5, 1e-3, 3.0, 3.0], "weibull_cdf"),
+        # full_ddm.py:11-33, ornstein.py:7-48
+        "full_ddm": _ddm_like("full_ddm", ["v", "a", "z", "t", "sz", "sv", "st"],
+                              [[-3.0, 0.3, 0.3, 0.25, 1e-3, 1e-3, 1e-3], [3.0, 2.5, 0.7, 2.25, 0.2, 2.0, 0.25]],
+                              [0.0, 1.0, 0.5, 0.25, 1e-3, 1e-3, 1e-3], "constant", cssm.full_ddm),
+        "ornstein": _ddm_like("ornstein", ["v", "a", "z", "g", "t"], [[-2.0, 0.3, 0.1, -1.0, 1e-3], [2.0, 3.0, 0.9, 1.0, 2]],
+                              [0.0, 1.0, 0.5, 0.0, 1e-3], "constant", cssm.ornstein_uhlenbeck),
+        "ornstein_angle": _ddm_like("ornstein_angle", ["v", "a", "z", "g", "t", "theta"],
+                                    [[-2.0, 0.3, 0.1, -1.0, 1e-3, -0.1], [2.0, 3.0, 0.9, 1.0, 2, 1.3]],
+                                    [0.0, 1.0, 0.5, 0.0, 1e-3, 0.1], "angle", cssm.ornstein_uhlenbeck),
         # race.py:14-352, lca.py:15-112
         "race_2": _race(2), "race_3": _race(3), "race_4": _race(4),
         "lca_3": _race(3, lca=True), "lca_4": _race(4, lca=True),

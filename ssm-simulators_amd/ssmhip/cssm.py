@@ -88,30 +88,33 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
     return rts, choices
 
 
-def ddm_flexbound(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20000, n_trials=1,
-                  boundary_fun=None, boundary_params={}, random_state=None, return_option="full",
-                  smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
-    """Two-bound DDM with flexible boundary ±b(t) (ddm_models.pyx:589-878)."""
+def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, n_trials, boundary_fun,
+               boundary_params, random_state, return_option, smooth_unif, n_threads, replay_normals,
+               replay_uniforms, extra, params):
     _check_threads(n_threads)
     if return_option not in ("full", "minimal"):
         raise ValueError('return_option must be either "full" or "minimal"')
     boundary_fun = boundary_fun or constant
     boundary_params = dict(boundary_params or {})
+    if a is None:  # models whose boundary level arrives only through boundary_params
+        if "a" not in boundary_params:
+            raise ValueError("boundary_params must provide 'a'")
+        a = boundary_params["a"]
     if not boundary_params and boundary_fun is constant:
         boundary_params = {"a": a}
     code = kernel_boundary_code(boundary_fun)
     max_t = float(np.float32(max_t))
     delta_t = float(np.float32(delta_t))
-    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params)
-    rts, choices = _run(L.MODEL_DDM, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t, _seed(random_state),
+    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra)
+    rts, choices = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t, _seed(random_state),
                         smooth_unif, boundary_fun, boundary_params, replay_normals, replay_uniforms)
-    minimal = _minimal_meta("ddm_flexbound", [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
+    minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
     traj = np.full((time_grid(max_t, delta_t).shape[0], 1), -999.0, dtype=np.float32)
     full = _full_meta(
         minimal,
-        {"v": v, "a": a, "z": z, "t": t, "s": s, "deadline": deadline},
+        params,
         {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads},
         boundary_fun=boundary_fun,
         boundary=_boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) if n_trials > 0 else None,
@@ -119,6 +122,40 @@ def ddm_flexbound(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20
         boundary_params=boundary_params,
     )
     return {"rts": rts, "choices": choices, "metadata": full}
+
+
+def ddm_flexbound(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20000, n_trials=1,
+                  boundary_fun=None, boundary_params={}, random_state=None, return_option="full",
+                  smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """Two-bound DDM with flexible boundary ±b(t) (ddm_models.pyx:589-878)."""
+    return _diffusion("ddm_flexbound", L.MODEL_DDM, v, a, z, t, deadline, s, max_t, delta_t, n_samples, n_trials,
+                      boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads,
+                      replay_normals, replay_uniforms, {},
+                      {"v": v, "a": a, "z": z, "t": t, "s": s, "deadline": deadline})
+
+
+def full_ddm(v, a, z, t, sz, sv, st, deadline, s, delta_t=0.001, max_t=20, n_samples=20000, n_trials=1,
+             boundary_fun=None, boundary_params={}, random_state=None, return_option="full", smooth_unif=False,
+             n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """DDM with trial-to-trial variability: drift v + sv N(0,1), start z-shift 2 (u - 1/2) sz,
+    non-decision time t + 2 (u - 1/2) st (ddm_models.pyx:2177-2498). Replay rows:
+    replay_normals[i][0] is the drift draw, the walk reads [i][1:]; replay_uniforms is
+    [n_total][3] = (u_t, u_z, u_smooth)."""
+    return _diffusion("full_ddm", L.MODEL_FULL_DDM, v, a, z, t, deadline, s, max_t, delta_t, n_samples, n_trials,
+                      boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads,
+                      replay_normals, replay_uniforms, {"sz": sz, "sv": sv, "st": st},
+                      {"v": v, "a": a, "z": z, "t": t, "sz": sz, "sv": sv, "st": st, "deadline": deadline, "s": s})
+
+
+def ornstein_uhlenbeck(v, z, g, t, deadline, s, delta_t=0.001, max_t=20, n_samples=20000, n_trials=1,
+                       boundary_fun=None, boundary_params={}, random_state=None, return_option="full",
+                       smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """Ornstein–Uhlenbeck diffusion between ±b(t): step ((v - g y) dt) + sqrt(dt) s N(0,1)
+    (ornstein_models.pyx:47-324). The boundary level comes from boundary_params (e.g. "a")."""
+    return _diffusion("ornstein_uhlenbeck", L.MODEL_ORNSTEIN, v, None, z, t, deadline, s, max_t, delta_t, n_samples,
+                      n_trials, boundary_fun, boundary_params, random_state, return_option, smooth_unif,
+                      n_threads, replay_normals, replay_uniforms, {"g": g},
+                      {"v": v, "z": z, "g": g, "t": t, "deadline": deadline, "s": s})
 
 
 def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n_trials, boundary_fun,

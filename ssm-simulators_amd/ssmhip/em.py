@@ -67,10 +67,16 @@ def fill_boundary(rec: np.ndarray, code: int, params: dict) -> None:
         rec[:, L.P_B2] = col("beta")
 
 
-def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams) -> np.ndarray:
+def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None, sv=None, st=None,
+                g=None) -> np.ndarray:
+    """Records of the single-particle diffusions: DDM; with sz/sv/st the full DDM
+    (ddm_models.pyx:2177-2360); with g the Ornstein–Uhlenbeck leak (ornstein_models.pyx:47-200)."""
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     n = v.shape[0]
     rec = new_records(n)
+    for slot, val in ((L.P_SZ, sz), (L.P_SV, sv), (L.P_ST, st), (L.P_G, g)):
+        if val is not None:
+            rec[:, slot] = np.asarray(val, F32).reshape(n, -1)[:, 0]
     rec[:, L.P_V] = v
     rec[:, L.P_Z] = z
     rec[:, L.P_T0] = t

@@ -322,3 +322,102 @@ def test_same_seed_same_output_different_seed_different(torch_gpu):
         outs.append(job.rts[:5000].cpu().numpy())
     assert np.array_equal(outs[0], outs[1])
     assert not np.array_equal(outs[0], outs[2])
+
+
+# ----------------------------------------------------------------------------- full DDM and Ornstein
+def _run_gpu_diffusion(model, v, a, z, t, bp, bname, N, normals=None, unif=None, smooth=False, seed=1, **extra):
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    K = v.shape[0]
+    fun = {"constant": BF.constant, "angle": BF.angle}[bname]
+    code = BF.kernel_boundary_code(fun)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, code, bp, **extra)
+    job = em.make_job(model, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=seed,
+                      smooth_unif=smooth, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    return job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy(), job.steps()
+
+
+def _variability(K, seed):
+    rng = np.random.default_rng(seed)
+    return {"sz": rng.uniform(1e-3, 0.2, K).astype(F32), "sv": rng.uniform(1e-3, 2.0, K).astype(F32),
+            "st": rng.uniform(1e-3, 0.25, K).astype(F32)}
+
+
+@pytest.mark.parametrize("bname", ["constant", "angle"])
+def test_full_ddm_replay_matches_oracle(torch_gpu, bname):
+    L, _ = _ssm()
+    K, N, L_ = 12, 400, 1031
+    print(f"[case] full_ddm replay {bname}", flush=True)
+    v, a, z, t, bp = _ddm_case(bname, K, 31)
+    var = _variability(K, 32)
+    rng = np.random.default_rng(4)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=(K * N, 3)).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 boundary=bname, bparams=bp, smooth_unif=True, normals=normals, uniforms=unif,
+                                 **var)
+    r1, c1, steps = _run_gpu_diffusion(L.MODEL_FULL_DDM, v, a, z, t, bp, bname, N, normals, unif, True, **var)
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - r1) <= DT)
+    assert np.mean(r0 == r1) > 0.999
+    assert steps == int(s0.sum())
+
+
+@pytest.mark.parametrize("bname", ["constant", "angle"])
+def test_ornstein_replay_matches_oracle(torch_gpu, bname):
+    L, _ = _ssm()
+    K, N, L_ = 12, 400, 1031
+    print(f"[case] ornstein replay {bname}", flush=True)
+    v, a, z, t, bp = _ddm_case(bname, K, 41)
+    g = np.random.default_rng(42).uniform(-1.0, 1.0, K).astype(F32)
+    rng = np.random.default_rng(5)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 boundary=bname, bparams=bp, smooth_unif=True, normals=normals, uniforms=unif, g=g)
+    r1, c1, steps = _run_gpu_diffusion(L.MODEL_ORNSTEIN, v, a, z, t, bp, bname, N, normals, unif, True, g=g)
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - r1) <= DT)
+    assert np.mean(r0 == r1) > 0.999
+    assert steps == int(s0.sum())
+
+
+def test_variants_reduce_to_the_ddm_natively(torch_gpu):
+    """Same seed, native RNG: the full DDM with zero variability and the Ornstein model with
+    zero leak (z = 1/2) walk the DDM's normals and give its output bit for bit."""
+    L, _ = _ssm()
+    K, N = 8, 3000
+    v, a, _, t, bp = _ddm_case("angle", K, 51)
+    z = np.full(K, 0.5, F32)
+    zeros = np.zeros(K, F32)
+    ref = _run_gpu_diffusion(L.MODEL_DDM, v, a, z, t, bp, "angle", N, smooth=True, seed=7)
+    full = _run_gpu_diffusion(L.MODEL_FULL_DDM, v, a, z, t, bp, "angle", N, smooth=True, seed=7, sz=zeros, sv=zeros,
+                              st=zeros)
+    ou = _run_gpu_diffusion(L.MODEL_ORNSTEIN, v, a, z, t, bp, "angle", N, smooth=True, seed=7, g=zeros)
+    for got in (full, ou):
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]) and got[2] == ref[2]
+
+
+@pytest.mark.parametrize("model", ["full_ddm", "ornstein"])
+def test_variants_native_match_numpy_oracle_statistically(torch_gpu, model):
+    L, _ = _ssm()
+    N = 100_000
+    v, a, z, t = (np.array([x], F32) for x in (0.6, 1.4, 0.45, 0.3))
+    bp = {"a": a}
+    extra = ({"sz": np.array([0.15], F32), "sv": np.array([0.8], F32), "st": np.array([0.2], F32)}
+             if model == "full_ddm" else {"g": np.array([0.7], F32)})
+    kind = L.MODEL_FULL_DDM if model == "full_ddm" else L.MODEL_ORNSTEIN
+    r1, c1, _ = _run_gpu_diffusion(kind, v, a, z, t, bp, "constant", N, smooth=True, seed=11, **extra)
+    r0, c0, _ = E.ddm_flexbound(v, a, z, t, [999.0], [1.0], n_samples=N, delta_t=DT, smooth_unif=True,
+                                rng=np.random.default_rng(12), **extra)
+    p0, p1 = np.mean(c0 == 1), np.mean(c1 == 1)
+    assert abs(p0 - p1) <= 3 * math.sqrt(2 * p0 * (1 - p0) / N)
+    for ch in (-1, 1):
+        x, y = r0[c0 == ch], r1[c1 == ch]
+        assert stats.ks_2samp(x, y).pvalue > 0.01
+        _quantile_check(x, y)

@@ -123,3 +123,22 @@ def test_graph_replay_matches_eager_launch(torch_gpu):
         assert np.array_equal(sim.choices.cpu().numpy(), eager[1])
         torch.zeros(1000, device="cuda").sum()  # unrelated eager work between replays
     sim.close()
+
+
+def test_full_ddm_and_ornstein_through_the_simulator(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    f = Simulator("full_ddm").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3, "sz": 0.1, "sv": 0.5, "st": 0.1},
+                                       n_samples=4000, random_state=0)
+    assert f["rts"].shape == (4000, 1) and f["metadata"]["simulator"] == "full_ddm"
+    assert f["metadata"]["model"] == "full_ddm"
+    assert np.all(f["rts"] > 0.3 - 0.1)  # t_tmp >= t - st
+    o = Simulator("ornstein").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "g": 0.5, "t": 0.3}, n_samples=4000,
+                                       random_state=0)
+    assert o["metadata"]["simulator"] == "ornstein_uhlenbeck" and o["choices"].shape == (4000, 1)
+    base = Simulator("ddm").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3}, n_samples=4000, random_state=0)
+    # a positive leak pulls the walk back towards v/g: slower and less accurate than the DDM
+    assert np.mean(o["rts"]) > np.mean(base["rts"])
+    oa = Simulator("ornstein_angle").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "g": 0.5, "t": 0.3, "theta": 0.3},
+                                              n_samples=2000, random_state=0)
+    assert np.mean(oa["rts"]) < np.mean(o["rts"])  # collapsing bounds shorten decisions

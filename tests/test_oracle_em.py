@@ -187,3 +187,76 @@ def test_cpu_baseline_native_matches_closed_form():
     p = ddm_analytic.p_upper(v, a_eff, 0.5)
     assert abs(np.mean(ch == 1) - p) < 4 * math.sqrt(p * (1 - p) / N)
     assert abs(rts.mean() - ddm_analytic.mean_decision_time(v, a_eff, 0.5)) < 4 * rts.std() / math.sqrt(N) + 0.003
+
+
+# ----------------------------------------------------------------------------- full DDM and Ornstein
+def test_full_ddm_without_variability_is_the_ddm():
+    rng = np.random.default_rng(1)
+    K, N, L = 3, 100, 20002  # rows long enough that no walk wraps around its row
+    v = rng.uniform(-1, 1, K).astype(F32)
+    a = rng.uniform(0.8, 1.5, K).astype(F32)
+    z = rng.uniform(0.3, 0.7, K).astype(F32)
+    t = rng.uniform(0.1, 0.5, K).astype(F32)
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    u3 = rng.uniform(0, 1, (K * N, 3)).astype(F32)
+    kw = dict(n_samples=N, delta_t=DT, smooth_unif=True)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), normals=nz[:, 1:],
+                                 uniforms=u3[:, 2], **kw)
+    zeros = np.zeros(K, F32)
+    r1, c1, s1 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), normals=nz, uniforms=u3,
+                                 sz=zeros, sv=zeros, st=zeros, **kw)
+    # the walk normals of the full model sit one column later (column 0 is the drift draw)
+    assert np.array_equal(c0, c1) and np.array_equal(s0, s1) and np.array_equal(r0, r1)
+
+
+def test_full_ddm_noise_free_variability_closed_forms():
+    # s = 0: each walk is deterministic given its drift draw gv and uniforms (u_t, u_z)
+    K, N = 1, 64
+    v, a, z, t, sz, sv, st = 0.5, 1.0, 0.5, 0.3, 0.2, 0.4, 0.1
+    rng = np.random.default_rng(2)
+    nz = np.zeros((N, 8), F32)
+    nz[:, 0] = rng.standard_normal(N).astype(F32)
+    u3 = rng.uniform(0, 1, (N, 3)).astype(F32)
+    rts, ch, steps = E.ddm_flexbound([v], [a], [z], [t], [999.0], [0.0], n_samples=N, delta_t=DT, normals=nz,
+                                     uniforms=u3, sz=[sz], sv=[sv], st=[st])
+    vk = v + sv * nz[:, 0].astype(np.float64)
+    y0 = 2 * (u3[:, 1].astype(np.float64) - 0.5) * sz          # start shift around 0
+    ttmp = t + 2 * (u3[:, 0].astype(np.float64) - 0.5) * st
+    target = np.where(vk > 0, a - y0, -a - y0)                   # distance to the boundary it reaches
+    k_exact = np.floor(target / (vk * DT)) + 1                  # first step beyond the boundary
+    # exact-arithmetic crossing step; float32 sums over thousands of steps may drift by ~1e-4
+    assert np.all(np.abs(steps - k_exact) <= np.maximum(1, 5e-4 * k_exact))
+    assert np.all(ch == np.sign(vk))
+    # rt = t_particle + t_tmp, the clock being a float32 sum of dt
+    clock = np.zeros(N, F32)
+    for i in range(int(steps.max())):
+        clock = np.where(i < steps, (clock + F32(DT)).astype(F32), clock)
+    assert np.allclose(rts, clock.astype(np.float64) + ttmp, atol=1e-6)
+
+
+def test_ornstein_without_leak_is_the_ddm_at_z_half():
+    rng = np.random.default_rng(3)
+    K, N, L = 3, 200, 257
+    v = rng.uniform(-1, 1, K).astype(F32)
+    a = rng.uniform(0.8, 1.5, K).astype(F32)
+    z = np.full(K, 0.5, F32)  # start 0 exactly in both start formulas
+    t = rng.uniform(0.1, 0.5, K).astype(F32)
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    kw = dict(n_samples=N, delta_t=DT, normals=nz)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), **kw)
+    r1, c1, s1 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), g=np.zeros(K, F32), **kw)
+    assert np.array_equal(c0, c1) and np.array_equal(s0, s1) and np.array_equal(r0, r1)
+
+
+@pytest.mark.parametrize("g", [0.8, -0.5])
+def test_ornstein_noise_free_crossing_matches_the_linear_recurrence(g):
+    # s = 0: y_{n+1} = y_n + (v - g y_n) dt, so y_n = v/g + (y0 - v/g)(1 - g dt)^n; the walk stops
+    # at the first n with y_n > a. A wrong sign or order of the leak moves that step far.
+    v, a, z = 1.0, 0.6, 0.5
+    rts, ch, steps = E.ddm_flexbound([v], [a], [z], [0.0], [999.0], [0.0], n_samples=2, delta_t=DT,
+                                     g=[g], rng=np.random.default_rng(0))
+    n = np.arange(1, 20001)
+    yn = v / g + (0.0 - v / g) * (1 - g * DT) ** n
+    k = int(n[np.argmax(yn > a)])
+    assert np.all(np.abs(steps - k) <= 1) and np.all(ch == 1)
+    assert abs(k - 0.6 / DT) > 20  # the leak visibly changes the crossing step
