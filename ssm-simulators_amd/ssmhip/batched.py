@@ -31,6 +31,9 @@ MODELS = {
     "race_4_weibull": (L.MODEL_RACE, "weibull_cdf",
                        ["v0", "v1", "v2", "v3", "a", "z0", "z1", "z2", "z3", "t", "alpha", "beta"], 4),
     "lca_4": (L.MODEL_LCA, "constant", ["v0", "v1", "v2", "v3", "a", "z0", "z1", "z2", "z3", "g", "b", "t"], 4),
+    "full_ddm": (L.MODEL_FULL_DDM, "constant", ["v", "a", "z", "t", "sz", "sv", "st"], 1),
+    "ornstein": (L.MODEL_ORNSTEIN, "constant", ["v", "a", "z", "g", "t"], 1),
+    "ornstein_angle": (L.MODEL_ORNSTEIN, "angle", ["v", "a", "z", "g", "t", "theta"], 1),
 }
 
 
@@ -46,8 +49,10 @@ def records_for(model: str, theta: np.ndarray, delta_t: float, max_t: float, dea
     bnames = BOUNDARY_REGISTRY[bname][1]
     bparams = {k: col[k] for k in bnames}
     code = BOUNDARY_REGISTRY[bname][2]
-    if kind == L.MODEL_DDM:
-        rec = ddm_records(col["v"], col["a"], col["z"], col["t"], dl, np.full(n, s, F32), max_t, delta_t, code, bparams)
+    if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN):
+        extra = {k: col[k] for k in ("sz", "sv", "st", "g") if k in col}
+        rec = ddm_records(col["v"], col["a"], col["z"], col["t"], dl, np.full(n, s, F32), max_t, delta_t, code, bparams,
+                          **extra)
     else:
         v = np.stack([col[f"v{j}"] for j in range(P)], axis=1)
         z = np.stack([col[f"z{j}"] for j in range(P)], axis=1)

@@ -44,7 +44,7 @@ class StageTimes:
 
 def choice_values_for(model: str) -> list[int]:
     kind, _, _, P = MODELS[model]
-    return [-1, 1] if kind == L.MODEL_DDM else list(range(P))
+    return [-1, 1] if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN) else list(range(P))
 
 
 class LanKDELabeller:
