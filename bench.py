@@ -97,6 +97,10 @@ def time_graph(sim, steps: int, warmup: int):
     return s.elapsed_time(e) / steps
 
 
+def queue_order(args):
+    return None if args.queue_order == "index" else args.queue_order
+
+
 def accumulator_leg(args):
     """LCA-4 and race-4/Weibull, 1024 sets x 20,000 samples each, against the CPU restatement."""
     from oracle import cpu_baseline
@@ -106,7 +110,8 @@ def accumulator_leg(args):
     out = {}
     for model, kind in (("lca_4", 2), ("race_4_weibull", 1)):
         theta = sample_accumulator_sets(model)
-        sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto", tune_sample=None)
+        sim = batched.BatchedSimulation(model, theta, 20000, seed=3, launch="auto", tune_sample=None,
+                                        queue_order=queue_order(args))
         ms = time_graph(sim, max(2, args.steps), 1)
         steps = sim.steps()
         lp = sim.launch_params
@@ -331,6 +336,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--launch", type=str, default="auto", help="'auto' or block,blocks_per_cu,refill_min")
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU roof probe")
+    ap.add_argument("--queue-order", choices=("expected", "index"), default="expected",
+                    help="visiting order of the sets: longest expected walks first, or set index order")
     ap.add_argument("--headline-only", action="store_true", help="run the headline leg only")
     ap.add_argument("--legs", type=str, default="headline,accumulators,lan,surrogate",
                     help="comma list of legs to run (surrogate needs lan); without 'headline' the "
@@ -397,7 +404,7 @@ def headline_leg(args, torch, batched, roofline) -> dict:
     launch = "auto" if args.launch == "auto" else LaunchParams(*[int(x) for x in args.launch.split(",")])
     # the headline batch tunes its launch shape on itself (~3 s), not on a sample
     sim = batched.BatchedSimulation("angle", theta, ns, delta_t=0.001, max_t=20.0, seed=0, launch=launch,
-                                    tune_sample=None)
+                                    tune_sample=None, queue_order=queue_order(args))
     lp = sim.launch_params
     log(f"autotuned launch: block={lp.block_size} blocks/CU={lp.blocks_per_cu} refill_min={lp.refill_min}")
     sim.capture()
@@ -467,6 +474,7 @@ def headline_leg(args, torch, batched, roofline) -> dict:
             "em_steps_per_s": steps_per_launch / (ms / 1e3),
             "launch": {"block": lp.block_size, "blocks_per_cu": lp.blocks_per_cu, "refill_min": lp.refill_min,
                        "queue_block": lp.queue_block},
+            "queue_order": args.queue_order,
             "lane_utilisation": util,
             "host_handover": {"d2h_ms": d2h_ms, "bytes": int(d2h_bytes), "GB_per_s": d2h_bytes / d2h_ms / 1e6,
                               "trajectories_per_s_incl_d2h": traj / ((ms + d2h_ms) / 1e3)},

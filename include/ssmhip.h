@@ -43,6 +43,14 @@ extern "C" {
                                 (src/cssm/ddm_models.pyx:2177-2360)            */
 #define SSM_MODEL_ORNSTEIN 4 /* ornstein_uhlenbeck: DDM with leak, step
                                 ((v - g y) dt) + sqrt_st g (ornstein_models.pyx:47-200) */
+#define SSM_MODEL_LEVY 5     /* levy_flexbound: DDM walk with symmetric alpha-stable
+                                increments (v dt) + (s dt^(1/alpha)) x (levy_models.pyx:47-220) */
+#define SSM_MODEL_RDM 6      /* racing_diffusion_model: N unclamped accumulators from
+                                U(0, A) starts, first past b wins (race_models.pyx:925-1260) */
+#define SSM_MODEL_PAR2 7     /* ddm_flexbound_par2: high walk, then a low walk from t = 0;
+                                choice in 0..3, rt = max(t_h, t_l) (parallel_models.pyx:57-560) */
+#define SSM_MODEL_SEQ2 8     /* ddm_flexbound_seq2: the low walk continues where the high
+                                walk ended; choice in 0..3 (sequential_models.pyx:55-640) */
 
 #define SSM_BOUNDARY_CONSTANT 0 /* b(t) = a                                  */
 #define SSM_BOUNDARY_ANGLE 1    /* b(t) = a + t_s * slope, slope = -sin/cos  */
@@ -67,8 +75,13 @@ extern "C" {
 #define SSM_P_SZ 11          /* full DDM start variability sz                 */
 #define SSM_P_SV 12          /* full DDM drift variability sv                 */
 #define SSM_P_ST 13          /* full DDM non-decision-time variability st     */
+#define SSM_P_ALPHA 14       /* Levy stability alpha (0 < alpha <= 2); the scale
+                                s dt^(1/alpha) goes in SSM_P_SQRT_ST          */
+#define SSM_P_MAX_T 15       /* float32 max_t (two-stage models)              */
+/* Two-stage models: SSM_P_V = vh, SSM_P_Z = zh, SSM_P_VJ + 0/1 = vl1/vl2,
+ * SSM_P_ZJ + 0/1 = zl1/zl2.                                                  */
 #define SSM_P_VJ 16          /* race/LCA v_j, j < 16                          */
-#define SSM_P_ZJ 32          /* race/LCA z_j                                  */
+#define SSM_P_ZJ 32          /* race/LCA z_j; racing diffusion start range A  */
 #define SSM_P_SSTJ 48        /* race/LCA float32(sqrt(dt)) * s_j              */
 
 typedef struct ssm_em_problem {
@@ -93,6 +106,12 @@ typedef struct ssm_launch_config {
   int32_t refill_min;    /* idle lanes (1..64) before a wavefront refills     */
   int32_t queue_block;   /* largest index range a wavefront claims from the
                             global queue per atomic (0 = 64, at most 65536)  */
+  /* Optional visiting order of the sets (both NULL: index order). set_order
+   * int32[n_sets] is a permutation of 0..n_sets-1; queue_offsets int64[n_sets+1]
+   * the prefix sums of the set sizes taken in that order. Long-running sets
+   * first shortens the tail of a launch; outputs do not depend on it.        */
+  const int32_t* set_order;
+  const int64_t* queue_offsets;
 } ssm_launch_config;
 
 /* ---- library ---------------------------------------------------------------*/
@@ -103,18 +122,31 @@ int32_t ssm_device_cu_count(void);
 
 /* ---- Euler–Maruyama first-passage simulation ------------------------------
  * Replaces cssm.ddm_flexbound (src/cssm/ddm_models.pyx:589-878, sequential
- * loop :696-744), cssm.race_model (src/cssm/race_models.pyx:356-506) and
- * cssm.lca (src/cssm/race_models.pyx:757-920), batched over ragged sets.
+ * loop :696-744), cssm.race_model (src/cssm/race_models.pyx:356-506),
+ * cssm.lca (src/cssm/race_models.pyx:757-920), cssm.full_ddm,
+ * cssm.ornstein_uhlenbeck, cssm.levy_flexbound (src/cssm/levy_models.pyx:47-220),
+ * cssm.racing_diffusion_model (src/cssm/race_models.pyx:925-1260) and
+ * cssm.ddm_flexbound_par2 / ddm_flexbound_seq2 (src/cssm/parallel_models.pyx,
+ * src/cssm/sequential_models.pyx), batched over ragged sets.
  *   set_params_dev   float32[n_sets][SSM_SET_STRIDE]
  *   offsets_dev      int64[n_sets + 1], offsets[0] = 0, non-decreasing
  *   table_dev        float32[n_sets][n_steps] when boundary == TABLE, else NULL
  *   replay_normals_dev float32[n_total][replay_stride] (replay mode) or NULL;
  *                    draw d of trajectory i reads [i][d % replay_stride],
  *                    d = step * n_particles + particle (full DDM: [i][0] is
- *                    the drift draw and step d reads [i][(1 + d) % stride])
+ *                    the drift draw and step d reads [i][(1 + d) % stride];
+ *                    Levy: the alpha-stable variates themselves; racing
+ *                    diffusion: draws up to the winner's of its last step)
  *   replay_uniforms_dev float32[n_total] (replay + smooth_unif) or NULL;
- *                    full DDM replay: float32[n_total][3] = (u_t, u_z, u_smooth)
- *   rts_dev          float32[n_total]; choices_dev int8/int32[n_total]
+ *                    full DDM replay: float32[n_total][3] = (u_t, u_z, u_smooth);
+ *                    racing diffusion replay (always given):
+ *                    float32[n_total][n_particles + 1] = (start u_j..., u_smooth);
+ *                    two-stage replay (always given): float32[n_total][4] =
+ *                    (u_high, u_low_prior, u_smooth, u_low); their normals rows
+ *                    hold the high walk's draws, then the low walk's
+ *   rts_dev          float32[n_total]; two-stage par2: float32[3][n_total] =
+ *                    (rt, rt_high, rt_low) (the reference's rts_high / rts_low)
+ *   choices_dev      int8/int32[n_total]
  *   workspace_dev    uint64[SSM_EM_WORKSPACE_WORDS] scratch owned by the caller
  *                    ([0] work-queue head, [1] executed EM steps)
  * `launch` may be NULL (defaults). Two launches on `stream`: a workspace reset

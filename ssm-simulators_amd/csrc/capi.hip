@@ -103,7 +103,7 @@ using namespace ssmhip;
 
 extern "C" {
 
-int32_t ssm_abi_version(void) { return 1; }
+int32_t ssm_abi_version(void) { return 2; }
 
 const char* ssm_last_error(void) { return g_last_error.c_str(); }
 

@@ -10,6 +10,9 @@
 //   ddm_flexbound  src/cssm/ddm_models.pyx:696-744
 //   race_model     src/cssm/race_models.pyx:417-464
 //   lca            src/cssm/race_models.pyx:825-879
+//   full_ddm, ornstein_uhlenbeck, levy_flexbound (ddm_models.pyx:2284-2358,
+//   ornstein_models.pyx:156-198, levy_models.pyx:160-220) on the DDM walk, and
+//   racing_diffusion_model (race_models.pyx:1207-1248) on the accumulator walk
 // in float32 with the same operation order (this file is compiled with
 // -ffp-contract=off so no multiply-add is fused behind our back).
 //
@@ -25,6 +28,8 @@ namespace ssmhip {
 struct EmArgs {
   const float* set_params;
   const int64_t* offsets;
+  const int64_t* qoff;   // queue order: prefix of the set sizes in the order the queue visits sets
+  const int32_t* order;  // queue position -> set (NULL: sets in index order, qoff == offsets)
   const float* table;
   const float* replay_normals;
   const float* replay_uniforms;
@@ -82,6 +87,7 @@ __device__ __forceinline__ int find_set(const int64_t* __restrict__ off, int n_s
 // Position increment of one step of the single-particle diffusions, in the reference's order:
 //   DDM, full DDM   (v dt) + (sqrt_st g)             ddm_models.pyx:720, :2335
 //   Ornstein        ((v - g y) dt) + (sqrt_st g)     ornstein_models.pyx:173
+//   Levy            (v dt) + (dt_alpha x)            levy_models.pyx:192, x alpha-stable
 template <int MODEL>
 __device__ __forceinline__ float step_increment(float y, float vdt, float v, float leak, float dt, float sst,
                                                 float nz) {
@@ -91,7 +97,36 @@ __device__ __forceinline__ float step_increment(float y, float vdt, float v, flo
 
 // the single-particle diffusions that share the DDM walk
 template <int MODEL>
-constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM || MODEL == SSM_MODEL_ORNSTEIN;
+constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM || MODEL == SSM_MODEL_ORNSTEIN ||
+                            MODEL == SSM_MODEL_LEVY || MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
+// hierarchical two-choice DDMs: a high walk, then a low walk chosen by its outcome
+template <int MODEL>
+constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
+
+// Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
+// when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
+__device__ __forceinline__ bool two_stage_coin(float u, float y, float b) {
+  return b <= 0.f ? u <= 0.5f : u <= (y + b) / (2.0f * b);
+}
+
+// Symmetric alpha-stable variate by Chambers, Mallows & Stuck (1976), the
+// representation the reference samples with (draw_random_stable, _utils.pyx:369-470):
+//   X = sin(alpha U) / |cos U|^(1/alpha) * (|cos((1 - alpha) U)| / W)^((1 - alpha)/alpha)
+// with U ~ U(-pi/2 + 1e-7, pi/2 - 1e-7), W ~ Exp(1) floored at 1e-10, the result
+// clipped to +-1e10 and NaN mapped to 0. Angles are kept in revolutions for
+// v_sin_f32; cos y is taken as sin(1/4 - |y|) so that it keeps its relative
+// accuracy where it approaches 0 (|U| -> pi/2 is the heavy tail). alpha == 1
+// gives tan U through the same formula (the second power is x^0 = 1).
+__device__ __forceinline__ float stable_cms(uint32_t bu, uint32_t bw, float alpha, float alpha_inv, float scale) {
+  const float urev = (u32_to_unit(bu) - 0.5f) * 0.49999996817f;  // U / (2 pi), |U| <= pi/2 - 1e-7
+  const float w = fmaxf(-0.69314718055994531f * __builtin_amdgcn_logf(u32_to_open01(bw)), 1e-10f);
+  const float cu = fmaxf(__builtin_amdgcn_sinf(0.25f - fabsf(urev)), 1e-10f);                 // |cos U|
+  const float ca = fmaxf(__builtin_amdgcn_sinf(0.25f - fabsf((1.0f - alpha) * urev)), 1e-10f);  // |cos((1-alpha)U)|
+  const float lg = scale * (__builtin_amdgcn_logf(ca) - __builtin_amdgcn_logf(w)) - alpha_inv * __builtin_amdgcn_logf(cu);
+  float x = __builtin_amdgcn_sinf(alpha * urev) * __builtin_amdgcn_exp2f(lg);
+  x = fminf(fmaxf(x, -1e10f), 1e10f);
+  return x == x ? x : 0.0f;
+}
 
 template <int MODEL, int BOUND, bool REPLAY, int MAXP>
 __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
@@ -107,6 +142,15 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   // per-trajectory constants
   float vdt[MAXP], sst[MAXP], vraw[MAXP];
   float a = 0.f, b1 = 0.f, b2 = 0.f, t0 = 0.f, deadline = 0.f, dl_tmp = 0.f, leak = 0.f, inh = 0.f;
+  float alpha = 2.f, alpha_inv = 0.5f, alpha_scale = -0.5f;  // Levy stability and its two exponents
+  int win = -1;  // racing diffusion: first accumulator past the threshold in its last step
+  // two-stage models: stage (0 high, 1 low), choice so far, low choice already taken, the
+  // high walk's time, the boundary index the current stage started at, the high walk's draw
+  // count (the low walk's replay draws follow it), the four per-trajectory uniforms
+  int stage = 0, ch2 = 0, ix0 = 0, nh = 0;
+  bool decided = false;
+  float t_h = 0.f, vl1 = 0.f, vl2 = 0.f, zl1 = 0.f, zl2 = 0.f, max_t = 0.f;
+  float uu[4] = {0.f, 0.f, 0.f, 0.f};
   const float* row = nullptr;
   unsigned long long my_steps = 0;
   unsigned long long iters = 0;  // wave-uniform: step iterations issued
@@ -130,26 +174,47 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
       if (state == kDone) {
         // result of the finished walk (ddm_models.pyx:735-744, race_models.pyx:454-464)
         float su = 0.f;
+        const float t_end = MODEL == SSM_MODEL_PAR2 ? fmaxf(t_h, t) : t;  // par2: rt from max(t_h, t_l)
         if (A.smooth_unif) {
           float u;
-          if constexpr (REPLAY) {
-            // full DDM rows carry (u_t, u_z, u_smooth); the others one value per trajectory
-            u = MODEL == SSM_MODEL_FULL_DDM ? A.replay_uniforms[g * 3 + 2] : A.replay_uniforms[g];
+          if constexpr (kTwoStage<MODEL>) {
+            u = uu[2];
+          } else if constexpr (REPLAY) {
+            // full DDM rows carry (u_t, u_z, u_smooth), racing-diffusion rows (u_start..., u_smooth);
+            // the others one value per trajectory
+            if constexpr (MODEL == SSM_MODEL_FULL_DDM) u = A.replay_uniforms[g * 3 + 2];
+            else if constexpr (MODEL == SSM_MODEL_RDM) u = A.replay_uniforms[g * (P + 1) + P];
+            else u = A.replay_uniforms[g];
           } else {
             u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, 0u, 1u}, A.k0, A.k1);
             u = u32_to_unit(r.x);
           }
           // compute_smooth_unif, _utils.pyx:668-695
           // the 0.5 literals are C doubles: formed in double, rounded to float once
-          if (t == 0.f) su = (float)((double)u * 0.5 * (double)dt);
-          else if (t < dl_tmp) su = (float)((0.5 - (double)u) * (double)dt);
+          if (t_end == 0.f) su = (float)((double)u * 0.5 * (double)dt);
+          else if (t_end < dl_tmp) su = (float)((0.5 - (double)u) * (double)dt);
           else su = 0.f;
         }
-        float rt = t + t0 + su;
+        float rt;
+        if constexpr (MODEL == SSM_MODEL_PAR2)
+          rt = (float)(((double)t_end + (double)t0) + (double)su);  // parallel_models.pyx:538: fmax is a double
+        else
+          rt = t + t0 + su;
         if (rt >= deadline || deadline <= 0.f) rt = -999.f;  // enforce_deadline, _utils.pyx:697-710
         int choice;
-        if constexpr (kDiffusion<MODEL>) {
+        if constexpr (kTwoStage<MODEL>) {
+          // the low choice from the low walk's exit unless it was taken at the transition
+          if (!decided) {
+            const float bx = ix < A.n_steps ? boundary_at<BOUND>(ix, dt, a, b1, b2, row) : 0.f;
+            ch2 += two_stage_coin(uu[3], x[0], bx) ? 1 : 0;
+          }
+          choice = ch2;
+        } else if constexpr (kDiffusion<MODEL>) {
           choice = (x[0] > 0.f) - (x[0] < 0.f);  // sign(), _utils.pyx:474-484
+        } else if constexpr (MODEL == SSM_MODEL_RDM) {
+          // race_models.pyx:1240-1248: no winner or past the deadline -> (-999, -1)
+          choice = win;
+          if (rt == -999.f || win < 0) { rt = -999.f; choice = -1; }
         } else {
           choice = 0;  // np.argmax: first maximum
           float best = x[0];
@@ -160,10 +225,16 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         const int64_t o = A.out_layout == 0 ? g : (int64_t)n * A.n_sets + k;
         if (o < A.total) {  // layout 1 assumes uniform offsets; never write past the buffers
           A.rts[o] = rt;
+          if constexpr (MODEL == SSM_MODEL_PAR2) {  // rts_high, rts_low (parallel_models.pyx:539-540)
+            A.rts[A.total + o] = t_h + t0;
+            A.rts[2 * A.total + o] = t + t0;
+          }
           if (A.choice_bytes == 1) ((int8_t*)A.choices)[o] = (int8_t)choice;
           else ((int32_t*)A.choices)[o] = choice;
         }
-        my_steps += (unsigned long long)ix;
+        if constexpr (kTwoStage<MODEL>) my_steps += (unsigned long long)(nh + ix - ix0);
+        else my_steps += (unsigned long long)ix;
+        if constexpr (MODEL == SSM_MODEL_RDM) my_steps += win >= 0;  // the winning step leaves t, ix as they were
         state = kEmpty;
       }
       if (!exhausted) {
@@ -188,7 +259,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           nb_base = (int64_t)readlane_u64(b, leader);
           if (nb_base + blk >= A.total) queue_done = true;
           nb_end = nb_base + blk < A.total ? nb_base + blk : A.total;
-          if (nb_base < nb_end) nb_k = find_set(A.offsets, A.n_sets, nb_base);
+          if (nb_base < nb_end) nb_k = find_set(A.qoff, A.n_sets, nb_base);
         }
         const int64_t need_new = cnt - from_local;
         if (r >= cnt) {
@@ -201,7 +272,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           q_lo = q_hi;
         }
         if (q_lo < q_hi)
-          while (q_k < A.n_sets - 1 && A.offsets[q_k + 1] <= q_lo) ++q_k;  // keep q_k = set of q_lo
+          while (q_k < A.n_sets - 1 && A.qoff[q_k + 1] <= q_lo) ++q_k;  // keep q_k = queue position of q_lo
         if (q_lo >= q_hi && queue_done) exhausted = true;
         if (state == kEmpty) {
           const int j = lanes_below(want);
@@ -215,10 +286,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             kk = nb_k;
           }
           if (gi < A.total) {
-            g = gi;
-            k = kk;
-            while (k < A.n_sets - 1 && A.offsets[k + 1] <= gi) ++k;
-            n = (int)(gi - A.offsets[k]);
+            int jq = kk;  // queue position of gi
+            while (jq < A.n_sets - 1 && A.qoff[jq + 1] <= gi) ++jq;
+            n = (int)(gi - A.qoff[jq]);
+            k = A.order ? A.order[jq] : jq;
+            g = A.order ? A.offsets[k] + n : gi;  // the trajectory's index in the set-major outputs
             const float* sp = A.set_params + (int64_t)k * SSM_SET_STRIDE;
             t0 = sp[SSM_P_T0];
             deadline = sp[SSM_P_DEADLINE];
@@ -233,11 +305,31 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               const float z = sp[SSM_P_Z];
               vdt[0] = sp[SSM_P_V] * dt;
               sst[0] = sp[SSM_P_SQRT_ST];
-              if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
-                // ornstein_models.pyx:165: the literal 2 is an int, so this stays in float
+              if constexpr (MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_LEVY || kTwoStage<MODEL>) {
+                // ornstein_models.pyx:165, levy_models.pyx:183, parallel_models.pyx:452,
+                // sequential_models.pyx:460: the literal 2 is an int, so this stays in float
                 x[0] = (-b0) + ((z * 2.0f) * b0);
                 leak = sp[SSM_P_G];
                 vraw[0] = sp[SSM_P_V];
+                if constexpr (MODEL == SSM_MODEL_LEVY) {
+                  alpha = sp[SSM_P_ALPHA];
+                  alpha_inv = 1.0f / alpha;
+                  alpha_scale = (1.0f - alpha) / alpha;
+                }
+                if constexpr (kTwoStage<MODEL>) {
+                  stage = 0; ch2 = 0; ix0 = 0; nh = 0; decided = false; t_h = 0.f;
+                  vl1 = sp[SSM_P_VJ + 0]; vl2 = sp[SSM_P_VJ + 1];
+                  zl1 = sp[SSM_P_ZJ + 0]; zl2 = sp[SSM_P_ZJ + 1];
+                  max_t = sp[SSM_P_MAX_T];
+                  if constexpr (REPLAY) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) uu[i] = A.replay_uniforms[g * 4 + i];
+                  } else {
+                    const u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, 0u, 5u}, A.k0, A.k1);
+                    uu[0] = u32_to_unit(r.x); uu[1] = u32_to_unit(r.y);
+                    uu[2] = u32_to_unit(r.z); uu[3] = u32_to_unit(r.w);
+                  }
+                }
               } else {
                 // ddm_models.pyx:719 / :2291: the literal 2.0 is a C double, so the sum is
                 // formed in double and rounded to float once
@@ -278,6 +370,28 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   vraw[j] = 0.f; vdt[j] = 0.f; sst[j] = 0.f; x[j] = -1.f;
                 }
               }
+              if constexpr (MODEL == SSM_MODEL_RDM) {
+                // race_models.pyx:1207-1208: start u_j * A with u_j uniform on [0, 1]
+                win = -1;
+#pragma unroll
+                for (int q = 0; q < (MAXP + 3) / 4; ++q) {
+                  if (4 * q < P) {
+                    float u4[4];
+                    if constexpr (REPLAY) {
+#pragma unroll
+                      for (int i = 0; i < 4; ++i)
+                        u4[i] = 4 * q + i < P ? A.replay_uniforms[g * (P + 1) + 4 * q + i] : 0.f;
+                    } else {
+                      const u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)q, 3u}, A.k0, A.k1);
+                      u4[0] = u32_to_unit(r.x); u4[1] = u32_to_unit(r.y);
+                      u4[2] = u32_to_unit(r.z); u4[3] = u32_to_unit(r.w);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                      if (4 * q + i < P) x[4 * q + i] = u4[i] * sp[SSM_P_ZJ + 4 * q + i];
+                  }
+                }
+              }
             }
             t = 0.f;
             ix = 0;
@@ -298,11 +412,24 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         // four steps per Philox block; every walk starts block-aligned
         float nz[4];
         if constexpr (REPLAY) {
-          // a full-DDM row starts with the drift draw, the walk's draws follow it
+          // a full-DDM row starts with the drift draw, the walk's draws follow it; a two-stage
+          // row holds the high walk's draws, then the low walk's
           constexpr int off = MODEL == SSM_MODEL_FULL_DDM ? 1 : 0;
           const float* rb = A.replay_normals + g * A.replay_stride;
+          const int64_t d0 = kTwoStage<MODEL> ? (int64_t)nh + (ix - ix0) : (int64_t)(off + ix);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) nz[i] = rb[(int64_t)(off + ix + i) % A.replay_stride];
+          for (int i = 0; i < 4; ++i) nz[i] = rb[(d0 + i) % A.replay_stride];
+        } else if constexpr (MODEL == SSM_MODEL_LEVY) {
+          // two Philox blocks per four steps: (U, W) pairs for four stable variates
+          const u32x4 r0 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u}, A.k0, A.k1);
+          const u32x4 r1 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 2u}, A.k0, A.k1);
+          nz[0] = stable_cms(r0.x, r0.y, alpha, alpha_inv, alpha_scale);
+          nz[1] = stable_cms(r0.z, r0.w, alpha, alpha_inv, alpha_scale);
+          nz[2] = stable_cms(r1.x, r1.y, alpha, alpha_inv, alpha_scale);
+          nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
+        } else if constexpr (kTwoStage<MODEL>) {
+          // each stage's draws start block-aligned in a stream of their own (0 high, 4 low)
+          philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ix - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
         } else {
           philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
         }
@@ -343,10 +470,51 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             }
           }
         }
-        if (!alive) state = kDone;
+        if (!alive) {
+          if constexpr (kTwoStage<MODEL>) {
+            if (stage == 0) {
+              // the high walk ended: its choice by the coin rule at the exit bound, then the low
+              // walk of that choice (parallel_models.pyx:474-489, sequential_models.pyx:474-533)
+              const float bx = ix < A.n_steps ? boundary_at<BOUND>(ix, dt, a, b1, b2, row) : 0.f;
+              const bool hi = two_stage_coin(uu[0], x[0], bx);
+              ch2 = hi ? 2 : 0;
+              const float zl = hi ? zl2 : zl1;
+              vdt[0] = (hi ? vl2 : vl1) * dt;
+              t_h = t;
+              nh = ix;
+              stage = 1;
+              alive = true;
+              if constexpr (MODEL == SSM_MODEL_PAR2) {
+                // the low walk starts afresh at t = 0 from b(0)
+                const float b0 = boundary_at<BOUND>(0, dt, a, b1, b2, row);
+                x[0] = (-b0) + ((zl * 2.0f) * b0);
+                t = 0.f;
+                ix = 0;
+                ix0 = 0;
+              } else {
+                ix0 = ix;
+                if (t >= max_t) {
+                  // no time left: the low choice from its prior, no low walk (:474-499)
+                  ch2 += uu[1] <= zl ? 1 : 0;
+                  decided = true;
+                  alive = false;
+                } else {
+                  // the low walk continues from the high walk's end, from b(ix); a start
+                  // outside the bounds takes the prior choice (u < zl) and does not walk
+                  x[0] = (-bx) + ((zl * 2.0f) * bx);
+                  if ((x[0] >= bx) || (x[0] <= -bx)) {
+                    ch2 += uu[1] < zl ? 1 : 0;
+                    decided = true;
+                  }
+                }
+              }
+            }
+          }
+          if (!alive) state = kDone;
+        }
       } else {
         bool go = ix < A.n_steps && t <= dl_tmp;
-        if (go) {
+        if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
           const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
 #pragma unroll
           for (int j = 0; j < MAXP; ++j)
@@ -374,7 +542,8 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               }
             }
           }
-          if constexpr (MODEL == SSM_MODEL_RACE) {
+          if constexpr (MODEL == SSM_MODEL_RDM) {
+          } else if constexpr (MODEL == SSM_MODEL_RACE) {
 #pragma unroll
             for (int j = 0; j < MAXP; ++j) {
               if (j < P) {
@@ -401,8 +570,24 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
 #pragma unroll
             for (int j = 0; j < MAXP; ++j) x[j] = xnew[j];
           }
-          t += dt;
-          ++ix;
+          if constexpr (MODEL == SSM_MODEL_RDM) {
+            // race_models.pyx:1221-1237: accumulators move in order and the first to reach
+            // b ends the walk before the rest move and before t advances; the later ones'
+            // positions are never read, so all move here and the first crossing wins
+            const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+#pragma unroll
+            for (int j = MAXP - 1; j >= 0; --j) {
+              if (j < P) {
+                x[j] = x[j] + (vdt[j] + sst[j] * nz[j]);
+                if (x[j] >= b) win = j;
+              }
+            }
+            if (win >= 0) state = kDone;
+          }
+          if (MODEL != SSM_MODEL_RDM || win < 0) {
+            t += dt;
+            ++ix;
+          }
         }
       }
     }
@@ -413,7 +598,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if (lane_id() == 0) {
     if (s) atomicAdd(A.ws + 1, s);
-    atomicAdd(A.ws + 2, iters * (MODEL == SSM_MODEL_DDM ? 256ull : 64ull));
+    atomicAdd(A.ws + 2, iters * (kDiffusion<MODEL> ? 256ull : 64ull));  // diffusions: 4 steps per iteration
   }
 }
 
@@ -451,6 +636,14 @@ static KernelFn pick_kernel(int model, int bound, int p, bool replay) {
     return replay ? pick_bound<SSM_MODEL_FULL_DDM, true>(bound, p) : pick_bound<SSM_MODEL_FULL_DDM, false>(bound, p);
   if (model == SSM_MODEL_ORNSTEIN)
     return replay ? pick_bound<SSM_MODEL_ORNSTEIN, true>(bound, p) : pick_bound<SSM_MODEL_ORNSTEIN, false>(bound, p);
+  if (model == SSM_MODEL_LEVY)
+    return replay ? pick_bound<SSM_MODEL_LEVY, true>(bound, p) : pick_bound<SSM_MODEL_LEVY, false>(bound, p);
+  if (model == SSM_MODEL_PAR2)
+    return replay ? pick_bound<SSM_MODEL_PAR2, true>(bound, p) : pick_bound<SSM_MODEL_PAR2, false>(bound, p);
+  if (model == SSM_MODEL_SEQ2)
+    return replay ? pick_bound<SSM_MODEL_SEQ2, true>(bound, p) : pick_bound<SSM_MODEL_SEQ2, false>(bound, p);
+  if (model == SSM_MODEL_RDM)
+    return replay ? pick_bound<SSM_MODEL_RDM, true>(bound, p) : pick_bound<SSM_MODEL_RDM, false>(bound, p);
   if (model == SSM_MODEL_RACE) return replay ? pick_bound<SSM_MODEL_RACE, true>(bound, p) : pick_bound<SSM_MODEL_RACE, false>(bound, p);
   return replay ? pick_bound<SSM_MODEL_LCA, true>(bound, p) : pick_bound<SSM_MODEL_LCA, false>(bound, p);
 }
@@ -459,9 +652,11 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
                     const float* table, const float* rn, const float* ru, float* rts, void* choices,
                     uint64_t* ws) {
   if (!P) return fail(SSM_EINVAL, "problem descriptor is NULL");
-  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_ORNSTEIN) return fail(SSM_EINVAL, "unknown model");
+  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_SEQ2) return fail(SSM_EINVAL, "unknown model");
   if (P->boundary < SSM_BOUNDARY_CONSTANT || P->boundary > SSM_BOUNDARY_TABLE) return fail(SSM_EINVAL, "unknown boundary");
-  const bool diffusion = P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM || P->model == SSM_MODEL_ORNSTEIN;
+  const bool diffusion = P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM ||
+                         P->model == SSM_MODEL_ORNSTEIN || P->model == SSM_MODEL_LEVY ||
+                         P->model == SSM_MODEL_PAR2 || P->model == SSM_MODEL_SEQ2;
   if (diffusion && P->n_particles != 1) return fail(SSM_EINVAL, "single-particle models require n_particles == 1");
   if (P->n_particles < 1 || P->n_particles > SSM_MAX_PARTICLES)
     return fail(SSM_EINVAL, "n_particles must be in [1, 16]");
@@ -481,6 +676,10 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
     if (P->replay_stride > 0 && P->smooth_unif && !ru) return fail(SSM_EINVAL, "replay + smooth_unif needs uniforms");
     if (P->replay_stride > 0 && P->model == SSM_MODEL_FULL_DDM && !ru)
       return fail(SSM_EINVAL, "full-DDM replay needs uniforms (u_t, u_z, u_smooth per trajectory)");
+    if (P->replay_stride > 0 && (P->model == SSM_MODEL_PAR2 || P->model == SSM_MODEL_SEQ2) && !ru)
+      return fail(SSM_EINVAL, "two-stage replay needs uniforms (four per trajectory)");
+    if (P->replay_stride > 0 && P->model == SSM_MODEL_RDM && !ru)
+      return fail(SSM_EINVAL, "racing-diffusion replay needs uniforms (start draws and u_smooth per trajectory)");
   }
   return SSM_OK;
 }
@@ -499,6 +698,8 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   if (refill < 1 || refill > 64) return fail(SSM_EINVAL, "refill_min must be in [1, 64]");
   if (L && (L->queue_block < 0 || L->queue_block > 65536))
     return fail(SSM_EINVAL, "queue_block must be in [0, 65536]");
+  if (L && (L->set_order != nullptr) != (L->queue_offsets != nullptr))
+    return fail(SSM_EINVAL, "set_order and queue_offsets go together");
   hipLaunchKernelGGL(ws_reset_kernel, dim3(1), dim3(64), 0, stream, (unsigned long long*)ws);
   SSM_HIP(hipGetLastError());
   if (P->n_total == 0) return SSM_OK;
@@ -506,6 +707,8 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   EmArgs A;
   A.set_params = set_params;
   A.offsets = offsets;
+  A.order = (L && L->set_order) ? L->set_order : nullptr;
+  A.qoff = A.order ? L->queue_offsets : offsets;
   A.table = table;
   A.replay_normals = rn;
   A.replay_uniforms = ru;

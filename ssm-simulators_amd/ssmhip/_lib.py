@@ -21,11 +21,12 @@ SSM_EHIP = -2
 SSM_ENOMEM = -3
 SSM_EUNSUPPORTED = -4
 
-MODEL_DDM, MODEL_RACE, MODEL_LCA, MODEL_FULL_DDM, MODEL_ORNSTEIN = 0, 1, 2, 3, 4
+MODEL_DDM, MODEL_RACE, MODEL_LCA, MODEL_FULL_DDM, MODEL_ORNSTEIN, MODEL_LEVY, MODEL_RDM = 0, 1, 2, 3, 4, 5, 6
+MODEL_PAR2, MODEL_SEQ2 = 7, 8
 BOUNDARY_CONSTANT, BOUNDARY_ANGLE, BOUNDARY_WEIBULL, BOUNDARY_TABLE = 0, 1, 2, 3
 MAX_PARTICLES = 16
 SET_STRIDE = 64
-P_T0, P_DEADLINE, P_DEADLINE_TMP, P_A, P_B1, P_B2, P_Z, P_V, P_SQRT_ST, P_G, P_BINH, P_SZ, P_SV, P_ST = range(14)
+P_T0, P_DEADLINE, P_DEADLINE_TMP, P_A, P_B1, P_B2, P_Z, P_V, P_SQRT_ST, P_G, P_BINH, P_SZ, P_SV, P_ST, P_ALPHA, P_MAX_T = range(16)
 P_VJ, P_ZJ, P_SSTJ = 16, 32, 48
 EM_WORKSPACE_WORDS = 4
 KDE_CHUNK = 8192
@@ -54,6 +55,8 @@ class LaunchConfig(C.Structure):
         ("blocks_per_cu", C.c_int32),
         ("refill_min", C.c_int32),
         ("queue_block", C.c_int32),
+        ("set_order", C.c_void_p),
+        ("queue_offsets", C.c_void_p),
     ]
 
 

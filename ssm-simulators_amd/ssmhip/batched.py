@@ -18,7 +18,8 @@ from . import _device as D
 from . import _lib as L
 from . import autotune
 from .boundary_functions import BOUNDARY_REGISTRY
-from .em import LaunchParams, accumulator_records, boundary_table, ddm_records, make_job
+from .em import (LaunchParams, accumulator_records, boundary_table, ddm_records, expected_steps, make_job,
+                 two_stage_records)
 
 F32 = np.float32
 
@@ -34,6 +35,13 @@ MODELS = {
     "full_ddm": (L.MODEL_FULL_DDM, "constant", ["v", "a", "z", "t", "sz", "sv", "st"], 1),
     "ornstein": (L.MODEL_ORNSTEIN, "constant", ["v", "a", "z", "g", "t"], 1),
     "ornstein_angle": (L.MODEL_ORNSTEIN, "angle", ["v", "a", "z", "g", "t", "theta"], 1),
+    "levy": (L.MODEL_LEVY, "constant", ["v", "a", "z", "alpha", "t"], 1),
+    "levy_angle": (L.MODEL_LEVY, "angle", ["v", "a", "z", "alpha", "t", "theta"], 1),
+    "racing_diffusion_3": (L.MODEL_RDM, "constant", ["v0", "v1", "v2", "A", "b", "t"], 3),
+    "ddm_par2": (L.MODEL_PAR2, "constant", ["vh", "vl1", "vl2", "a", "zh", "zl1", "zl2", "t"], 1),
+    "ddm_seq2": (L.MODEL_SEQ2, "constant", ["vh", "vl1", "vl2", "a", "zh", "zl1", "zl2", "t"], 1),
+    "ddm_par2_angle_no_bias": (L.MODEL_PAR2, "angle", ["vh", "vl1", "vl2", "a", "t", "theta"], 1),
+    "ddm_seq2_angle_no_bias": (L.MODEL_SEQ2, "angle", ["vh", "vl1", "vl2", "a", "t", "theta"], 1),
 }
 
 
@@ -47,17 +55,27 @@ def records_for(model: str, theta: np.ndarray, delta_t: float, max_t: float, dea
     col = {nm: theta[:, i] for i, nm in enumerate(names)}
     dl = np.full(n, 999.0, F32) if deadline is None else np.asarray(deadline, F32).reshape(n)
     bnames = BOUNDARY_REGISTRY[bname][1]
+    if kind == L.MODEL_RDM:
+        col["a"] = col["b"]  # the racing diffusion's threshold b is a constant boundary
+        col.update({f"z{j}": col["A"] for j in range(P)})  # start range U(0, A)
     bparams = {k: col[k] for k in bnames}
     code = BOUNDARY_REGISTRY[bname][2]
-    if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN):
+    if kind in (L.MODEL_PAR2, L.MODEL_SEQ2):
+        half = np.full(n, 0.5, F32)  # the no-bias variants fix the starts at 1/2
+        zs = [col.get(k, half) for k in ("zh", "zl1", "zl2")]
+        rec = two_stage_records(col["vh"], col["vl1"], col["vl2"], *zs, col["t"], dl, np.full(n, s, F32), max_t,
+                                delta_t, code, bparams)
+    elif kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN, L.MODEL_LEVY):
         extra = {k: col[k] for k in ("sz", "sv", "st", "g") if k in col}
+        if kind == L.MODEL_LEVY:
+            extra["alpha"] = col["alpha"]
         rec = ddm_records(col["v"], col["a"], col["z"], col["t"], dl, np.full(n, s, F32), max_t, delta_t, code, bparams,
                           **extra)
     else:
         v = np.stack([col[f"v{j}"] for j in range(P)], axis=1)
         z = np.stack([col[f"z{j}"] for j in range(P)], axis=1)
         g = col.get("g")
-        b = col.get("b")
+        b = col.get("b") if kind == L.MODEL_LCA else None
         rec = accumulator_records(v, z, col["t"], np.full((n, 1), s, F32), dl, max_t, delta_t, code, bparams,
                                   g=g, b=b)
     return kind, code, P, rec, bname, bparams
@@ -69,7 +87,7 @@ class BatchedSimulation:
     def __init__(self, model: str, theta: np.ndarray, n_samples, *, delta_t: float = 0.001, max_t: float = 20.0,
                  seed: int = 0, smooth_unif: bool = False, choice_dtype: str = "int8", launch="auto",
                  deadline=None, replay_normals=None, replay_uniforms=None, use_table: bool = False,
-                 tune_sample: float | None = 8e6):
+                 tune_sample: float | None = 8e6, queue_order="expected"):
         kind, code, P, rec, bname, bparams = records_for(model, theta, delta_t, max_t, deadline)
         n_sets = rec.shape[0]
         ns = np.broadcast_to(np.asarray(n_samples, np.int64), (n_sets,))
@@ -77,7 +95,7 @@ class BatchedSimulation:
             raise ValueError("n_samples must be >= 0")
         self.offsets = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
         table = None
-        if use_table:
+        if use_table or code == L.BOUNDARY_TABLE:  # boundaries without an in-register form always use one
             table = boundary_table(BOUNDARY_REGISTRY[bname][0], bparams, n_sets, max_t, delta_t)
             code = L.BOUNDARY_TABLE
         self.model = model
@@ -88,6 +106,16 @@ class BatchedSimulation:
                             choice_bytes=1 if choice_dtype == "int8" else 4, table=table,
                             replay_normals=replay_normals, replay_uniforms=replay_uniforms)
         self._graph = None
+        # queue visiting order: "expected" = sets by a mean-walk-length proxy, longest first
+        # (shortens the launch tail; outputs are identical in any order), None = index order,
+        # or an explicit permutation of the sets
+        if isinstance(queue_order, str):
+            if queue_order != "expected":
+                raise ValueError(f"unknown queue_order {queue_order!r}")
+            order = np.argsort(-expected_steps(rec, kind, P, max_t, delta_t), kind="stable")
+        else:
+            order = queue_order
+        self.job.set_queue_order(order)
         if launch == "auto":
             if tune_sample is None:  # tune on the batch itself
                 self.launch_params = autotune.tune(self.job)
@@ -108,6 +136,10 @@ class BatchedSimulation:
         sub = make_job(kind, code, P, rec[idx], offs, delta_t=delta_t, max_t=max_t, seed=seed,
                        smooth_unif=smooth_unif, out_layout=0, choice_bytes=1,
                        table=None if table is None else table[idx])
+        if self.job.set_order is not None:  # the sample keeps the batch's relative queue order
+            rank = np.empty(self.n_sets, np.int64)
+            rank[self.job.set_order.cpu().numpy()] = np.arange(self.n_sets)
+            sub.set_queue_order(np.argsort(rank[idx], kind="stable"))
         return autotune.tune(sub)
 
     # ---- eager launch -------------------------------------------------------
@@ -119,7 +151,7 @@ class BatchedSimulation:
         """Capture reset + persistent kernel into a graph executable (pointers baked in)."""
         if self._graph is not None:
             return
-        lc = self.launch_params.as_struct()
+        lc = self.job.launch_config(self.launch_params)
         h = C.c_void_p()
         rc = L.load().ssm_em_graph_create(C.byref(self.job.prob), C.byref(lc), *self.job.pointers(), C.byref(h))
         L.check(rc, "ssm_em_graph_create")

@@ -1,6 +1,6 @@
 """Boundary functions b(t) with the reference's float32 NumPy semantics.
 
-Mirrors ssms/basic_simulators/boundary_functions.py:14-122. The simulators
+Mirrors ssms/basic_simulators/boundary_functions.py:14-162. The simulators
 evaluate `constant`, `angle` and `weibull_cdf` in registers on the GPU; any
 other callable with signature ``f(t, **params)`` is evaluated here on the host
 over ``t_s`` and handed to the kernel as a per-set table.
@@ -30,11 +30,39 @@ def weibull_cdf(t=1, a: float = 1.0, alpha: float = 1.0, beta: float = 1.0):
     return a * np.exp(-np.power(np.divide(t, beta), alpha))
 
 
-# name -> (function, parameter names, in-kernel boundary code)
+def addm_collapse(t=0, a: float = 1.0, b: float = 0.5):
+    """b(t) = a - b t, the attentional DDM's collapsing bound (boundary_functions.py:33-54)."""
+    return a - np.multiply(t, b)
+
+
+def generalized_logistic(t=1, a: float = 1.0, B: float = 2.0, M: float = 3.0, v: float = 0.5):  # noqa: N803
+    """b(t) = a + 1 - 1 / (1 + exp(-B (t - M)))^(1/v) (boundary_functions.py:76-100)."""
+    offset = 1 - (1 / np.power(1 + np.exp(-B * (t - M)), 1 / v))
+    return a + offset
+
+
+def conflict_gamma(t=None, a: float = 1.0, theta: float = 0.5, scale: float = 1.0, alphaGamma: float = 1.01,  # noqa: N803
+                   scaleGamma: float = 0.3):  # noqa: N803
+    """b(t) = a + scale * GammaPDF(t; alphaGamma, scaleGamma) + t * (-sin(theta)/cos(theta))
+    (boundary_functions.py:125-162); t defaults to arange(0, 20, 0.1) as there."""
+    from scipy.stats import gamma  # noqa: PLC0415
+
+    if t is None:
+        t = np.arange(0, 20, 0.1)
+    gamma_bump = scale * gamma.pdf(t, a=alphaGamma, loc=0, scale=scaleGamma)
+    angle_offset = np.multiply(t, (-np.sin(theta) / np.cos(theta)))
+    return a + gamma_bump + angle_offset
+
+
+# name -> (function, parameter names, in-kernel boundary code); the last three are evaluated
+# on the host into a per-set table
 BOUNDARY_REGISTRY = {
     "constant": (constant, ["a"], _lib.BOUNDARY_CONSTANT),
     "angle": (angle, ["a", "theta"], _lib.BOUNDARY_ANGLE),
     "weibull_cdf": (weibull_cdf, ["a", "alpha", "beta"], _lib.BOUNDARY_WEIBULL),
+    "addm_collapse": (addm_collapse, ["a", "b"], _lib.BOUNDARY_TABLE),
+    "generalized_logistic": (generalized_logistic, ["a", "B", "M", "v"], _lib.BOUNDARY_TABLE),
+    "conflict_gamma": (conflict_gamma, ["a", "theta", "scale", "alphaGamma", "scaleGamma"], _lib.BOUNDARY_TABLE),
 }
 
 _REFERENCE_MODULE = "ssms.basic_simulators.boundary_functions"

@@ -100,6 +100,44 @@ def _lba(n, bounds, defaults):
     }
 
 
+def _z_half():
+    def f(theta, cfg, n):
+        for p in ("zh", "zl1", "zl2"):
+            theta[p] = np.tile(np.array([0.5], dtype=np.float32), n)
+        return theta
+
+    return [f]
+
+
+def _two_stage(kind, variant):
+    """ddm_par2.py / ddm_seq2.py: the full model and the no-bias variants (z's fixed at 1/2)
+    with constant, conflict_gamma, angle and weibull_cdf bounds."""
+    v3 = ["vh", "vl1", "vl2"]
+    lo3, hi3 = [-4.0] * 3, [4.0] * 3
+    table = {
+        "": (v3 + ["a", "zh", "zl1", "zl2", "t"], [lo3 + [0.3, 0.2, 0.2, 0.2, 0.0], hi3 + [2.5, 0.8, 0.8, 0.8, 2.0]],
+             [0.0, 0.0, 0.0, 1.0, 0.5, 0.5, 0.5, 1.0], "constant"),
+        "_no_bias": (v3 + ["a", "t"], [lo3 + [0.3, 0.0], hi3 + [2.5, 2.0]], [0.0, 0.0, 0.0, 1.0, 1.0], "constant"),
+        "_conflict_gamma_no_bias": (
+            v3 + ["t", "a", "theta", "scale", "alphaGamma", "scaleGamma"],
+            [lo3 + [0.0, 0.3, 0.0, 0.0, 1.1, 0.5], hi3 + [2.0, 2.5, 0.5, 5.0, 5.0, 5.0]],
+            [0.0, 0.0, 0.0, 1.0, 1.0, 0.5, 1.0, 2, 2], "conflict_gamma"),
+        "_angle_no_bias": (v3 + ["a", "t", "theta"], [lo3 + [0.3, 0.0, -0.1], hi3 + [2.5, 2.0, 1.0]],
+                           [0.0, 0.0, 0.0, 1.0, 1.0, 0.0], "angle"),
+        "_weibull_no_bias": (v3 + ["a", "t", "alpha", "beta"], [lo3 + [0.3, 0.0, 0.31, 0.31], hi3 + [2.5, 2.0, 4.99, 6.99]],
+                             [0.0, 0.0, 0.0, 1.0, 1.0] + ([0.0, 3.0] if kind == "par2" else [2.5, 3.5]), "weibull_cdf"),
+    }
+    params, bounds, dflt, bname = table[variant]
+    fun, bnames, _ = bf.BOUNDARY_REGISTRY[bname]
+    return {
+        "name": f"ddm_{kind}{variant}", "params": params, "param_bounds": bounds, "boundary_name": bname,
+        "boundary": fun, "boundary_params": list(bnames), "n_params": len(params), "default_params": dflt,
+        "nchoices": 4, "choices": [0, 1, 2, 3], "n_particles": 1,
+        "simulator": cssm.ddm_flexbound_par2 if kind == "par2" else cssm.ddm_flexbound_seq2,
+        "simulation_transforms": [] if variant == "" else _z_half(),
+    }
+
+
 def _registry() -> dict:
     reg = {
         # ddm.py:7-26, angle.py:7-25, weibull.py:7-28
@@ -120,6 +158,21 @@ def _registry() -> dict:
         "ornstein_angle": _ddm_like("ornstein_angle", ["v", "a", "z", "g", "t", "theta"],
                                     [[-2.0, 0.3, 0.1, -1.0, 1e-3, -0.1], [2.0, 3.0, 0.9, 1.0, 2, 1.3]],
                                     [0.0, 1.0, 0.5, 0.0, 1e-3, 0.1], "angle", cssm.ornstein_uhlenbeck),
+        # levy.py:7-48
+        "levy": _ddm_like("levy", ["v", "a", "z", "alpha", "t"], [[-3.0, 0.3, 0.1, 1.0, 1e-3], [3.0, 3.0, 0.9, 2.0, 2]],
+                          [0.0, 1.0, 0.5, 1.5, 0.1], "constant", cssm.levy_flexbound),
+        "levy_angle": _ddm_like("levy_angle", ["v", "a", "z", "alpha", "t", "theta"],
+                                [[-3.0, 0.3, 0.1, 1.0, 1e-3, -0.1], [3.0, 3.0, 0.9, 2.0, 2, 1.3]],
+                                [0.0, 1.0, 0.5, 1.5, 0.1, 0.01], "angle", cssm.levy_flexbound),
+        # racing_diffusion.py:11-36
+        "racing_diffusion_3": {
+            "name": "racing_diffusion_3", "params": ["v0", "v1", "v2", "A", "b", "t"],
+            "param_bounds": [[0.0, 0.0, 0.0, 0.0, 0.5, 0.0], [2.5, 2.5, 2.5, 1.0, 3.0, 2.0]],
+            "boundary_name": "constant", "boundary": bf.constant, "boundary_params": [], "n_params": 6,
+            "default_params": [1.0, 1.0, 1.0, 0.5, 1.5, 1e-3], "nchoices": 3, "choices": [0, 1, 2],
+            "n_particles": 3, "simulator": cssm.racing_diffusion_model,
+            "simulation_transforms": [column_stack(["v0", "v1", "v2"], "v"), expand_dims(["t", "A", "b"])],
+        },
         # race.py:14-352, lca.py:15-112
         "race_2": _race(2), "race_3": _race(3), "race_4": _race(4),
         "lca_3": _race(3, lca=True), "lca_4": _race(4, lca=True),
@@ -136,6 +189,11 @@ def _registry() -> dict:
             "n_particles": 3, "simulator": cssm.lba_angle, "simulation_transforms": _lba_angle_3_transforms(),
         },
     }
+    # ddm_par2.py:10-185, ddm_seq2.py:11-190
+    for kind in ("par2", "seq2"):
+        for variant in ("", "_no_bias", "_conflict_gamma_no_bias", "_angle_no_bias", "_weibull_no_bias"):
+            cfg = _two_stage(kind, variant)
+            reg[cfg["name"]] = cfg
     for cfg in reg.values():
         if cfg.get("boundary_params") is None:
             cfg["boundary_params"] = list(bf.BOUNDARY_REGISTRY[cfg["boundary_name"]][1])

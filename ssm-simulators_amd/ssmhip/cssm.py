@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib as L
 from .boundary_functions import constant, kernel_boundary_code, time_grid
-from .em import accumulator_records, boundary_table, ddm_records, make_job, uniform_offsets
+from .em import accumulator_records, boundary_table, ddm_records, make_job, two_stage_records, uniform_offsets
 
 MAX_THREADS = 256  # _constants.pxi:10
 OMISSION_SENTINEL = -999.0
@@ -156,6 +156,126 @@ def ornstein_uhlenbeck(v, z, g, t, deadline, s, delta_t=0.001, max_t=20, n_sampl
                       n_trials, boundary_fun, boundary_params, random_state, return_option, smooth_unif,
                       n_threads, replay_normals, replay_uniforms, {"g": g},
                       {"v": v, "z": z, "g": g, "t": t, "deadline": deadline, "s": s})
+
+
+def levy_flexbound(v, z, alpha, t, deadline, s, delta_t=0.001, max_t=20, n_samples=20000, n_trials=1,
+                   boundary_fun=None, boundary_params={}, random_state=None, return_option="full",
+                   smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """Levy flight between ±b(t): step (v dt) + s dt^(1/alpha) X with X symmetric alpha-stable
+    (levy_models.pyx:47-348). The boundary level comes from boundary_params (e.g. "a").
+    Replay rows hold the alpha-stable variates themselves."""
+    out = _diffusion("levy_flexbound", L.MODEL_LEVY, v, None, z, t, deadline, s, max_t, delta_t, n_samples,
+                     n_trials, boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads,
+                     replay_normals, replay_uniforms, {"alpha": alpha},
+                     {"v": v, "z": z, "t": t, "alpha": alpha, "s": s})
+    return out
+
+
+def racing_diffusion_model(v, b, A, t, s, deadline, delta_t=0.001, max_t=20, n_samples=2000, n_trials=1,
+                           random_state=None, return_option="full", smooth_unif=False, n_threads=1,
+                           replay_normals=None, replay_uniforms=None, **kwargs):
+    """Racing diffusion (Tillman et al. 2020): N Wiener accumulators without a floor, started
+    at U(0, A), first to reach b wins (race_models.pyx:925-1300). rt = steps completed before
+    the winning step + t (+ smooth_u); no winner by the deadline -> (-999, -1). Replay:
+    replay_uniforms[i] = (start u_0 .. u_{N-1}, u_smooth)."""
+    _check_threads(n_threads)
+    if return_option not in ("full", "minimal"):
+        raise ValueError('return_option must be either "full" or "minimal"')
+    v = np.asarray(v, np.float32)
+    if v.ndim != 2:
+        raise ValueError("v must have shape (n_trials, n_particles)")
+    n_particles = v.shape[1]
+    if n_particles > L.MAX_PARTICLES:
+        raise ValueError(f"racing_diffusion_model parallel execution requires n_particles <= {L.MAX_PARTICLES}, "
+                         f"got {n_particles}. Use n_threads=1 for larger particle counts.")
+    n_trials = int(n_trials)
+    col = lambda x: np.asarray(x, np.float32).reshape(n_trials, -1)[:, :1]  # noqa: E731
+    max_t = float(np.float32(max_t))
+    delta_t = float(np.float32(delta_t))
+    rec = accumulator_records(v, col(A), col(t), col(s), deadline, max_t, delta_t, L.BOUNDARY_CONSTANT,
+                              {"a": col(b)})
+    rts, choices = _run(L.MODEL_RDM, L.BOUNDARY_CONSTANT, n_particles, rec, int(n_samples), n_trials, max_t,
+                        delta_t, _seed(random_state), smooth_unif, constant, {"a": col(b)}, replay_normals,
+                        replay_uniforms)
+    minimal = _minimal_meta("rdm_simulator", list(np.arange(0, n_particles, 1)), int(n_samples), n_trials)
+    if return_option == "minimal":
+        return {"rts": rts, "choices": choices, "metadata": minimal}
+    traj = np.full((time_grid(max_t, delta_t).shape[0], n_particles), -999.0, dtype=np.float32)
+    full = _full_meta(minimal, {"v": v, "b": b, "A": A, "t": t, "deadline": deadline, "s": s},
+                      {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads}, traj=traj,
+                      extra={f"v{i}": v[:, i] for i in range(n_particles)})
+    return {"rts": rts, "choices": choices, "metadata": full}
+
+
+def _two_stage(name, model, vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t, max_t, n_samples, n_trials,
+               boundary_fun, boundary_params, random_state, return_option, smooth_unif, n_threads, replay_normals,
+               replay_uniforms):
+    _check_threads(n_threads)
+    if return_option not in ("full", "minimal"):
+        raise ValueError('return_option must be either "full" or "minimal"')
+    boundary_fun = boundary_fun or constant
+    boundary_params = dict(boundary_params or {})
+    if "a" not in boundary_params:
+        raise ValueError("boundary_params must provide 'a'")
+    code = kernel_boundary_code(boundary_fun)
+    max_t = float(np.float32(max_t))
+    delta_t = float(np.float32(delta_t))
+    n_samples, n_trials = int(n_samples), int(n_trials)
+    rec = two_stage_records(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, max_t, delta_t, code, boundary_params)
+    table = boundary_table(boundary_fun, boundary_params, n_trials, max_t, delta_t) if code == L.BOUNDARY_TABLE else None
+    job = make_job(model, code, 1, rec, uniform_offsets(n_trials, n_samples), delta_t=delta_t, max_t=max_t,
+                   seed=_seed(random_state), smooth_unif=smooth_unif, out_layout=1, choice_bytes=4, table=table,
+                   replay_normals=replay_normals, replay_uniforms=replay_uniforms)
+    job.launch()
+    total = n_samples * n_trials
+    shape = (n_samples, n_trials, 1)
+    planes = job.rts.cpu().numpy()
+    rts = planes[:total].reshape(shape)
+    choices = job.choices[:total].cpu().numpy().astype(np.intc).reshape(shape)
+    extra = {}
+    if model == L.MODEL_PAR2:  # parallel_models.pyx:570: rts_low / rts_high beside rts
+        cap = planes.size // 3
+        extra = {"rts_low": planes[2 * cap:2 * cap + total].reshape(shape),
+                 "rts_high": planes[cap:cap + total].reshape(shape)}
+    minimal = _minimal_meta(name, [0, 1, 2, 3], n_samples, n_trials, boundary_fun.__name__)
+    if return_option == "minimal":
+        return {"rts": rts, "choices": choices, "metadata": minimal, **extra}
+    traj = np.full((time_grid(max_t, delta_t).shape[0], 3), -999.0, dtype=np.float32)
+    full = _full_meta(
+        minimal,
+        {"vh": vh, "vl1": vl1, "vl2": vl2, "zh": zh, "zl1": zl1, "zl2": zl2, "t": t, "deadline": deadline, "s": s},
+        {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads}, boundary_fun=boundary_fun,
+        boundary=_boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) if n_trials > 0 else None,
+        traj=traj, boundary_params=boundary_params,
+    )
+    return {"rts": rts, "choices": choices, "metadata": full, **extra}
+
+
+def ddm_flexbound_par2(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t=0.001, max_t=20, n_samples=20000,
+                       n_trials=1, print_info=True, boundary_fun=None, boundary_params={}, random_state=None,
+                       return_option="full", smooth_unif=False, n_threads=1, replay_normals=None,
+                       replay_uniforms=None, **kwargs):
+    """Hierarchical two-choice DDM, parallel form (parallel_models.pyx:57-603): a high walk
+    (vh, zh) and, from t = 0, the low walk its outcome selects (vl1, zl1 or vl2, zl2);
+    choices 0..3, rt = max(t_h, t_l) + t (+ smooth_u); returns rts_high / rts_low too.
+    Replay: rows of normals hold the high walk's draws then the low walk's; replay_uniforms
+    is [n_total][4] = (u_high, unused, u_smooth, u_low)."""
+    return _two_stage("ddm_flexbound_par2", L.MODEL_PAR2, vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t, max_t,
+                      n_samples, n_trials, boundary_fun, boundary_params, random_state, return_option, smooth_unif,
+                      n_threads, replay_normals, replay_uniforms)
+
+
+def ddm_flexbound_seq2(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t=0.001, max_t=20, n_samples=20000,
+                       n_trials=1, print_info=True, boundary_fun=None, boundary_params={}, random_state=None,
+                       return_option="full", smooth_unif=False, n_threads=1, replay_normals=None,
+                       replay_uniforms=None, **kwargs):
+    """Hierarchical two-choice DDM, sequential form (sequential_models.pyx:55-660): the low
+    walk continues from where and when the high walk ended; past max_t, or with a low start
+    outside the bounds, the low choice comes from its prior zl. Replay: as par2, with
+    replay_uniforms[i][1] the prior-choice draw."""
+    return _two_stage("ddm_flexbound_seq2", L.MODEL_SEQ2, vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t, max_t,
+                      n_samples, n_trials, boundary_fun, boundary_params, random_state, return_option, smooth_unif,
+                      n_threads, replay_normals, replay_uniforms)
 
 
 def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n_trials, boundary_fun,

@@ -67,10 +67,20 @@ def fill_boundary(rec: np.ndarray, code: int, params: dict) -> None:
         rec[:, L.P_B2] = col("beta")
 
 
+def levy_scale32(s, alpha, delta_t) -> np.ndarray:
+    """s * pow(delta_t, 1.0 / alpha) (levy_models.pyx:166): float s times a double pow,
+    rounded to float once; the scale of the alpha-stable increment over one step."""
+    s64 = np.asarray(s, F32).astype(np.float64)
+    a64 = np.asarray(alpha, F32).astype(np.float64)
+    return (s64 * np.power(np.float64(F32(delta_t)), 1.0 / a64)).astype(F32)
+
+
 def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None, sv=None, st=None,
-                g=None) -> np.ndarray:
+                g=None, alpha=None) -> np.ndarray:
     """Records of the single-particle diffusions: DDM; with sz/sv/st the full DDM
-    (ddm_models.pyx:2177-2360); with g the Ornstein–Uhlenbeck leak (ornstein_models.pyx:47-200)."""
+    (ddm_models.pyx:2177-2360); with g the Ornstein–Uhlenbeck leak (ornstein_models.pyx:47-200);
+    with alpha the Levy flight (levy_models.pyx:47-220), whose noise scale s dt^(1/alpha)
+    replaces sqrt(dt) s."""
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     n = v.shape[0]
     rec = new_records(n)
@@ -83,9 +93,30 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     rec[:, L.P_DEADLINE] = deadline
     rec[:, L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
     rec[:, L.P_SQRT_ST] = (sqrt_dt32(delta_t) * s).astype(F32)
+    if alpha is not None:
+        al = np.asarray(alpha, F32).reshape(n, -1)[:, 0]
+        if np.any(~(al > 0)) or np.any(al > 2):
+            raise ValueError("alpha must be in (0, 2]")
+        rec[:, L.P_ALPHA] = al
+        rec[:, L.P_SQRT_ST] = levy_scale32(s, al, delta_t)
     a_b = np.asarray(bparams.get("a", a), F32).reshape(n, -1)[:, 0]
     rec[:, L.P_A] = a_b
     fill_boundary(rec, code, bparams)
+    return rec
+
+
+def two_stage_records(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, max_t, delta_t, code, bparams) -> np.ndarray:
+    """Records of the hierarchical two-choice DDMs (ddm_flexbound_par2 / _seq2): the high walk
+    in the DDM slots (v = vh, z = zh), the two low walks' drifts and starts in the first two
+    accumulator slots, and max_t (seq2 stops at the high walk when t_h >= max_t)."""
+    vh = np.asarray(vh, F32).reshape(-1)
+    n = vh.shape[0]
+    rec = ddm_records(vh, bparams.get("a", np.ones(n, F32)), zh, t, deadline, s, max_t, delta_t, code, bparams)
+    rec[:, L.P_VJ + 0] = np.asarray(vl1, F32).reshape(-1)
+    rec[:, L.P_VJ + 1] = np.asarray(vl2, F32).reshape(-1)
+    rec[:, L.P_ZJ + 0] = np.asarray(zl1, F32).reshape(-1)
+    rec[:, L.P_ZJ + 1] = np.asarray(zl2, F32).reshape(-1)
+    rec[:, L.P_MAX_T] = F32(max_t)
     return rec
 
 
@@ -139,6 +170,29 @@ class EmJob:
     rts: object
     choices: object
     workspace: object  # torch int64 [4]
+    set_order: object | None = None  # torch int32 [n_sets]: queue visiting order, or None
+    queue_offsets: object | None = None  # torch int64 [n_sets + 1]: sizes' prefix in that order
+
+    def launch_config(self, launch: LaunchParams | None = None) -> L.LaunchConfig:
+        lc = (launch or LaunchParams()).as_struct()
+        lc.set_order = D.ptr(self.set_order)
+        lc.queue_offsets = D.ptr(self.queue_offsets)
+        return lc
+
+    def set_queue_order(self, order) -> None:
+        """Visit the sets in `order` (a permutation of the set indices; None: index order).
+        Results do not depend on it; putting long-running sets first shortens the tail."""
+        if order is None:
+            self.set_order = self.queue_offsets = None
+            return
+        order = np.asarray(order, np.int64)
+        off = self.offsets.cpu().numpy()
+        n = off.shape[0] - 1
+        if order.shape != (n,) or not np.array_equal(np.sort(order), np.arange(n)):
+            raise ValueError("order must be a permutation of the set indices")
+        sizes = (off[1:] - off[:-1])[order]
+        self.set_order = D.to_dev(order.astype(np.int32))
+        self.queue_offsets = D.to_dev(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64))
 
     def pointers(self):
         return (
@@ -153,7 +207,7 @@ class EmJob:
         )
 
     def launch(self, launch: LaunchParams | None = None, stream=None) -> None:
-        lc = (launch or LaunchParams()).as_struct()
+        lc = self.launch_config(launch)
         rc = L.load().ssm_em_simulate(C.byref(self.prob), C.byref(lc), *self.pointers(), D.stream_ptr(stream))
         L.check(rc, "ssm_em_simulate")
 
@@ -165,6 +219,29 @@ class EmJob:
         """Lane-step slots issued in the last launch (64 lanes x the steps of each wave
         iteration); steps() / lane_slots() is the fraction of lanes doing useful steps."""
         return int(self.workspace[2].item())
+
+
+def expected_steps(records: np.ndarray, kind: int, n_particles: int, max_t: float, delta_t: float) -> np.ndarray:
+    """A cheap per-set proxy of the mean walk length (in steps), used only to order the queue:
+    the mean exit time of a Wiener process with drift v and unit noise from (-a, a) started at
+    -a + 2 z a (constant bound) for the diffusions; a / max_j v_j for the accumulators."""
+    rec = np.asarray(records, np.float64)
+    a = np.maximum(rec[:, L.P_A], 1e-6)
+    if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN, L.MODEL_LEVY, L.MODEL_PAR2, L.MODEL_SEQ2):
+        v = rec[:, L.P_V]  # two-stage models: the high walk's
+        s2 = np.maximum((rec[:, L.P_SQRT_ST] ** 2) / delta_t, 1e-12)  # noise variance s^2
+        A_, x0 = 2 * a, 2 * a * np.clip(rec[:, L.P_Z], 0.0, 1.0)
+        mu = v / s2
+        with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+            e1 = np.expm1(np.clip(-2 * mu * x0, -700, 700))
+            e2 = np.expm1(np.clip(-2 * mu * A_, -700, 700))
+            t = (A_ * e1 / e2 - x0) / v
+        t = np.where(np.abs(v) < 1e-6 * np.sqrt(s2) / A_, x0 * (A_ - x0) / s2, t)
+    else:
+        vmax = rec[:, L.P_VJ:L.P_VJ + n_particles].max(axis=1)
+        t = np.where(vmax > 1e-6, a / np.maximum(vmax, 1e-6), max_t)
+    t = np.nan_to_num(t, nan=max_t, posinf=max_t)
+    return np.clip(t, 0.0, max_t) / delta_t
 
 
 def make_job(
@@ -203,6 +280,16 @@ def make_job(
         stride = int(replay_normals.shape[1])
         if stride < 1:
             raise ValueError("replay stride must be >= 1")
+    if replay_uniforms is not None:
+        # values per trajectory: full DDM (u_t, u_z, u_smooth), racing diffusion (u_start x P, u_smooth)
+        per = {L.MODEL_FULL_DDM: 3, L.MODEL_RDM: n_particles + 1, L.MODEL_PAR2: 4, L.MODEL_SEQ2: 4}.get(model, 1)
+        replay_uniforms = np.ascontiguousarray(replay_uniforms, dtype=F32)
+        if replay_uniforms.size != n_total * per:
+            raise ValueError(f"replay_uniforms must hold {per} value(s) per trajectory ({n_total * per})")
+    if stride and model == L.MODEL_RDM and replay_uniforms is None:
+        raise ValueError("racing-diffusion replay needs replay_uniforms (start draws and u_smooth)")
+    if stride and model in (L.MODEL_PAR2, L.MODEL_SEQ2) and replay_uniforms is None:
+        raise ValueError("two-stage replay needs replay_uniforms (four per trajectory)")
     if table is not None:
         table = np.ascontiguousarray(table, dtype=F32)
         if table.shape != (n_sets, nst):
@@ -230,7 +317,8 @@ def make_job(
         table=D.to_dev(table) if table is not None else None,
         replay_normals=D.to_dev(replay_normals) if replay_normals is not None else None,
         replay_uniforms=D.to_dev(np.ascontiguousarray(replay_uniforms, F32)) if replay_uniforms is not None else None,
-        rts=D.empty((max(n_total, 1),), t.float32),
+        # par2 also returns the high and the low walk's rts: (rt, rt_high, rt_low) planes
+        rts=D.empty(((3 if model == L.MODEL_PAR2 else 1) * max(n_total, 1),), t.float32),
         choices=D.empty((max(n_total, 1),), cdt),
         workspace=D.zeros((L.EM_WORKSPACE_WORDS,), t.int64),
     )

@@ -44,7 +44,9 @@ class StageTimes:
 
 def choice_values_for(model: str) -> list[int]:
     kind, _, _, P = MODELS[model]
-    return [-1, 1] if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN) else list(range(P))
+    if kind in (L.MODEL_PAR2, L.MODEL_SEQ2):
+        return [0, 1, 2, 3]
+    return [-1, 1] if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN, L.MODEL_LEVY) else list(range(P))
 
 
 class LanKDELabeller:

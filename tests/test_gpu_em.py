@@ -208,9 +208,16 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
                                      g=rng.uniform(0, 0.3, K).astype(F32), b=rng.uniform(0, 0.3, K).astype(F32))
         kind, bound = L.MODEL_LCA, L.BOUNDARY_CONSTANT
     outs = []
-    for lp in [em.LaunchParams(64, 1, 1, 1), em.LaunchParams(64, 1, 1, 64), em.LaunchParams(256, 8, 8, 64),
-               em.LaunchParams(512, 4, 3, 7), em.LaunchParams(1024, 2, 64, 4096)]:
+    expected = np.argsort(-em.expected_steps(rec, kind, P, 20.0, DT), kind="stable")
+    cases = [(em.LaunchParams(64, 1, 1, 1), None), (em.LaunchParams(64, 1, 1, 64), None),
+             (em.LaunchParams(256, 8, 8, 64), None), (em.LaunchParams(512, 4, 3, 7), None),
+             (em.LaunchParams(1024, 2, 64, 4096), None),
+             # the queue's visiting order of the sets changes nothing either
+             (em.LaunchParams(256, 8, 8, 64), expected), (em.LaunchParams(64, 1, 1, 1), rng.permutation(K)),
+             (em.LaunchParams(512, 4, 3, 7), np.arange(K)[::-1])]
+    for lp, order in cases:
         job = em.make_job(kind, bound, P, rec, offs, delta_t=DT, max_t=20.0, seed=5, smooth_unif=True)
+        job.set_queue_order(order)
         job.launch(lp)
         torch.cuda.synchronize()
         n = int(offs[-1])
@@ -218,6 +225,45 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
     for r, c, st in outs[1:]:
         assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1]) and st == outs[0][2]
     assert np.all(outs[0][0] > 0)
+
+
+def test_replay_is_independent_of_queue_order_on_ragged_sets(torch_gpu):
+    """Replay rows are addressed by the trajectory's set-major index, whatever queue position
+    it is drawn from: the oracle's trajectories come back in any visiting order of the sets."""
+    import torch
+
+    L, em = _ssm()
+    sizes = np.array([5, 0, 300, 1, 77, 0, 129], np.int64)
+    K, L_ = sizes.size, 301
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    print(f"[case] replay queue order, sizes {sizes.tolist()}", flush=True)
+    v, a, z, t, bp = _ddm_case("angle", K, 21)
+    rng = np.random.default_rng(22)
+    normals = rng.standard_normal((n, L_)).astype(F32)
+    uniforms = rng.uniform(size=n).astype(F32)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, L.BOUNDARY_ANGLE, bp)
+    outs = []
+    for order in (None, rng.permutation(K), np.arange(K)[::-1]):
+        job = em.make_job(L.MODEL_DDM, L.BOUNDARY_ANGLE, 1, rec, offs, delta_t=DT, max_t=20.0, seed=1,
+                          smooth_unif=True, replay_normals=normals, replay_uniforms=uniforms)
+        job.set_queue_order(order)
+        job.launch(em.LaunchParams(64, 1, 1, 3))
+        torch.cuda.synchronize()
+        outs.append((job.rts[:n].cpu().numpy(), job.choices[:n].cpu().numpy()))
+    for r, c in outs[1:]:
+        assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1])
+    for k in range(K):  # each set against the oracle on its own rows
+        lo, hi = offs[k], offs[k + 1]
+        if hi == lo:
+            continue
+        one = lambda x: x[k:k + 1]
+        r0, c0, _ = E.ddm_flexbound(one(v), one(a), one(z), one(t), np.full(1, 999, F32), np.ones(1, F32),
+                                    n_samples=int(hi - lo), delta_t=DT, boundary="angle",
+                                    bparams={q: one(w) for q, w in bp.items()}, smooth_unif=True,
+                                    normals=normals[lo:hi], uniforms=uniforms[lo:hi])
+        assert np.array_equal(c0, outs[1][1][lo:hi])
+        assert np.all(np.abs(r0 - outs[1][0][lo:hi]) <= DT)
 
 
 # ----------------------------------------------------------------------------- native RNG statistics
@@ -421,3 +467,206 @@ def test_variants_native_match_numpy_oracle_statistically(torch_gpu, model):
         x, y = r0[c0 == ch], r1[c1 == ch]
         assert stats.ks_2samp(x, y).pvalue > 0.01
         _quantile_check(x, y)
+
+
+# ----------------------------------------------------------------------------- Levy flight
+@pytest.mark.parametrize("bname", ["constant", "angle"])
+def test_levy_replay_matches_oracle(torch_gpu, bname):
+    """Replay rows hold alpha-stable variates (drawn per set's alpha by the oracle's sampler)."""
+    L, _ = _ssm()
+    K, N, L_ = 12, 400, 1031
+    print(f"[case] levy replay {bname}", flush=True)
+    v, a, z, t, bp = _ddm_case(bname, K, 61)
+    rng = np.random.default_rng(62)
+    alpha = rng.uniform(1.0, 2.0, K).astype(F32)
+    alpha[0] = 1.0
+    alpha[1] = 2.0
+    noise = E.draw_stable(rng, K * N * L_, np.repeat(alpha, N * L_)).reshape(K * N, L_)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 boundary=bname, bparams=bp, smooth_unif=True, normals=noise, uniforms=unif,
+                                 alpha=alpha)
+    r1, c1, steps = _run_gpu_diffusion(L.MODEL_LEVY, v, a, z, t, bp, bname, N, noise, unif, True, alpha=alpha)
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - r1) <= DT)
+    assert np.mean(r0 == r1) > 0.999
+    assert steps == int(s0.sum())
+
+
+@pytest.mark.parametrize("alpha", [1.2, 1.7])
+def test_levy_native_matches_numpy_oracle_statistically(torch_gpu, alpha):
+    L, _ = _ssm()
+    N = 100_000
+    print(f"[case] levy native alpha={alpha}", flush=True)
+    v, a, z, t = (np.array([x], F32) for x in (0.7, 1.3, 0.45, 0.25))
+    al = np.array([alpha], F32)
+    r1, c1, _ = _run_gpu_diffusion(L.MODEL_LEVY, v, a, z, t, {"a": a}, "constant", N, smooth=True, seed=13,
+                                   alpha=al)
+    r0, c0, _ = E.ddm_flexbound(v, a, z, t, [999.0], [1.0], n_samples=N, delta_t=DT, smooth_unif=True,
+                                rng=np.random.default_rng(14), alpha=al)
+    p0, p1 = np.mean(c0 == 1), np.mean(c1 == 1)
+    assert abs(p0 - p1) <= 3 * math.sqrt(2 * p0 * (1 - p0) / N)
+    for ch in (-1, 1):
+        x, y = r0[c0 == ch], r1[c1 == ch]
+        assert stats.ks_2samp(x, y).pvalue > 0.01
+        _quantile_check(x, y)
+
+
+def test_levy_native_mean_exit_time_matches_getoor(torch_gpu):
+    """E[T] = (1 - x^2)^(alpha/2) / Gamma(1 + alpha) on (-1, 1) for the symmetric stable process
+    (Getoor 1961): pins the in-kernel sampler's law and step scale without the oracle."""
+    L, _ = _ssm()
+    N = 100_000
+    alphas = np.array([1.0, 1.4, 1.8], F32)
+    K = alphas.size
+    ones = np.ones(K, F32)
+    r, _, _ = _run_gpu_diffusion(L.MODEL_LEVY, np.zeros(K, F32), ones, np.full(K, 0.5, F32), np.zeros(K, F32),
+                                 {"a": ones}, "constant", N, seed=21, alpha=alphas)
+    for k, al in enumerate(alphas):
+        rk = r[k * N:(k + 1) * N]
+        exact = 1.0 / math.gamma(1 + float(al))
+        assert abs(rk.mean() - exact) < 0.07 * exact + 4 * rk.std() / math.sqrt(N), (al, rk.mean(), exact)
+
+
+# ----------------------------------------------------------------------------- racing diffusion
+def _rdm_case(K, P, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(0.0, 2.5, (K, P)).astype(F32)
+    A = rng.uniform(0.0, 1.0, K).astype(F32)
+    b = rng.uniform(0.5, 3.0, K).astype(F32)
+    t = rng.uniform(0.0, 2.0, K).astype(F32)
+    return v, A, b, t
+
+
+def _run_gpu_rdm(v, A, b, t, dl, N, normals=None, unif=None, smooth=False, seed=1):
+    import torch
+
+    L, em = _ssm()
+    K, P = v.shape
+    rec = em.accumulator_records(v, A.reshape(K, 1), t, np.ones((K, 1), F32), dl, 20.0, DT, L.BOUNDARY_CONSTANT,
+                                 {"a": b})
+    job = em.make_job(L.MODEL_RDM, L.BOUNDARY_CONSTANT, P, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0,
+                      seed=seed, smooth_unif=smooth, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    return job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy(), job.steps()
+
+
+@pytest.mark.parametrize("P", [3, 6])
+def test_racing_diffusion_replay_matches_oracle(torch_gpu, P):
+    K, N, L_ = 10, 300, 3001
+    print(f"[case] racing diffusion replay P={P}", flush=True)
+    v, A, b, t = _rdm_case(K, P, 70 + P)
+    dl = np.full(K, 999.0, F32)
+    dl[1] = F32(t[1] + 0.3)  # a deadline that cuts walks off
+    dl[2] = F32(0.0)         # deadline <= 0: every trajectory omitted
+    rng = np.random.default_rng(72)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=(K * N, P + 1)).astype(F32)
+    r0, c0, s0 = E.racing_diffusion(v, A, b, t, np.ones(K, F32), dl, n_samples=N, delta_t=DT, smooth_unif=True,
+                                    normals=normals, uniforms=unif)
+    r1, c1, steps = _run_gpu_rdm(v, A, b, t, dl, N, normals, unif, True)
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(r0, r1)
+    assert steps == int(s0.sum())
+    assert np.all(c1[2 * N:3 * N] == -1) and np.any(c1[N:2 * N] == -1)
+
+
+def test_racing_diffusion_native_matches_numpy_oracle_statistically(torch_gpu):
+    N = 100_000
+    v = np.array([[1.2, 0.9, 0.6]], F32)
+    A, b, t = np.array([0.5], F32), np.array([1.4], F32), np.array([0.2], F32)
+    r1, c1, _ = _run_gpu_rdm(v, A, b, t, np.array([999.0], F32), N, smooth=True, seed=17)
+    r0, c0, _ = E.racing_diffusion(v, A, b, t, [1.0], [999.0], n_samples=N, delta_t=DT, smooth_unif=True,
+                                   rng=np.random.default_rng(18))
+    _compare_samples(r1, c1, r0, c0, (0, 1, 2))
+
+
+# ----------------------------------------------------------------------------- two-stage DDMs
+def _two_stage_case(K, seed):
+    rng = np.random.default_rng(seed)
+    v = [rng.uniform(-3, 3, K).astype(F32) for _ in range(3)]
+    z = [rng.uniform(0.2, 0.8, K).astype(F32) for _ in range(3)]
+    t = rng.uniform(0.0, 1.0, K).astype(F32)
+    return v, z, t
+
+
+def _run_gpu_two_stage(model, v, z, t, dl, bp, bname, N, normals=None, unif=None, smooth=False, seed=1,
+                       max_t=20.0):
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    K = t.shape[0]
+    fun = {"constant": BF.constant, "angle": BF.angle}[bname]
+    code = BF.kernel_boundary_code(fun)
+    rec = em.two_stage_records(*v, *z, t, dl, np.ones(K, F32), max_t, DT, code, bp)
+    job = em.make_job(model, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=max_t, seed=seed,
+                      smooth_unif=smooth, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    planes = job.rts.cpu().numpy()
+    return planes, job.choices[: K * N].cpu().numpy(), job.steps()
+
+
+@pytest.mark.parametrize("kind,bname", [("par2", "constant"), ("par2", "angle"), ("seq2", "constant"),
+                                        ("seq2", "angle")])
+def test_two_stage_replay_matches_oracle(torch_gpu, kind, bname):
+    L, _ = _ssm()
+    K, N, L_ = 12, 300, 4099
+    print(f"[case] two-stage replay {kind} {bname}", flush=True)
+    v, z, t = _two_stage_case(K, 81)
+    _, a, _, _, bp = _ddm_case(bname, K, 82)
+    z[1][0], z[2][0] = F32(1.2), F32(-0.2)  # seq2: low starts outside the bounds, prior choice without a walk
+    dl = np.full(K, 999.0, F32)
+    dl[1] = F32(t[1] + 0.4)
+    rng = np.random.default_rng(83)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=(K * N, 4)).astype(F32)
+    r0, c0, s0, rh, rl = E.two_stage(kind, *v, *z, t, dl, np.ones(K, F32), n_samples=N, delta_t=DT, boundary=bname,
+                                     bparams=bp, smooth_unif=True, normals=normals, uniforms=unif)
+    model = L.MODEL_PAR2 if kind == "par2" else L.MODEL_SEQ2
+    planes, c1, steps = _run_gpu_two_stage(model, v, z, t, dl, bp, bname, N, normals, unif, True)
+    n = K * N
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(r0 - planes[:n]) <= DT) and np.mean(r0 == planes[:n]) > 0.999
+    assert steps == int(s0.sum())
+    if kind == "par2":
+        assert np.array_equal(rh, planes[n:2 * n]) and np.array_equal(rl, planes[2 * n:3 * n])
+    assert set(np.unique(c1)) == {0, 1, 2, 3}
+
+
+def test_seq2_replay_at_max_t_matches_oracle(torch_gpu):
+    """Walks that never leave wide bounds reach max_t: the low choice from its prior."""
+    L, _ = _ssm()
+    K, N, L_ = 2, 500, 2003
+    v = [np.zeros(K, F32)] * 3
+    z = [np.full(K, 0.5, F32), np.array([0.25, 0.7], F32), np.array([0.8, 0.3], F32)]
+    t = np.array([0.1, 0.2], F32)
+    bp = {"a": np.full(K, 50.0, F32)}
+    rng = np.random.default_rng(84)
+    normals = (0.05 * rng.standard_normal((K * N, L_))).astype(F32)
+    unif = rng.uniform(size=(K * N, 4)).astype(F32)
+    dl = np.full(K, 999.0, F32)
+    r0, c0, s0, _, _ = E.two_stage("seq2", *v, *z, t, dl, np.ones(K, F32), n_samples=N, max_t=2.0, delta_t=DT,
+                                   bparams=bp, normals=normals, uniforms=unif)
+    planes, c1, steps = _run_gpu_two_stage(L.MODEL_SEQ2, v, z, t, dl, bp, "constant", N, normals, unif, max_t=2.0)
+    assert np.array_equal(c0, c1) and np.array_equal(r0, planes[:K * N]) and steps == int(s0.sum())
+    assert np.all(planes[:K * N] > 2.0)
+
+
+@pytest.mark.parametrize("kind", ["par2", "seq2"])
+def test_two_stage_native_matches_numpy_oracle_statistically(torch_gpu, kind):
+    L, _ = _ssm()
+    N = 100_000
+    v = [np.array([x], F32) for x in (0.9, 0.6, -0.8)]
+    z = [np.array([x], F32) for x in (0.55, 0.4, 0.6)]
+    t = np.array([0.3], F32)
+    bp = {"a": np.array([1.1], F32)}
+    model = L.MODEL_PAR2 if kind == "par2" else L.MODEL_SEQ2
+    planes, c1, _ = _run_gpu_two_stage(model, v, z, t, np.array([999.0], F32), bp, "constant", N, smooth=True,
+                                       seed=19)
+    r0, c0, _, _, _ = E.two_stage(kind, *v, *z, t, [999.0], [1.0], n_samples=N, delta_t=DT, bparams=bp,
+                                  smooth_unif=True, rng=np.random.default_rng(20))
+    _compare_samples(planes[:N], c1, r0, c0, (0, 1, 2, 3))

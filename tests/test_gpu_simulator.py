@@ -142,3 +142,42 @@ def test_full_ddm_and_ornstein_through_the_simulator(torch_gpu):
     oa = Simulator("ornstein_angle").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "g": 0.5, "t": 0.3, "theta": 0.3},
                                               n_samples=2000, random_state=0)
     assert np.mean(oa["rts"]) < np.mean(o["rts"])  # collapsing bounds shorten decisions
+
+
+def test_levy_and_racing_diffusion_through_the_simulator(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    lv = Simulator("levy").simulate({"v": 0.5, "a": 1.2, "z": 0.5, "alpha": 1.5, "t": 0.3}, n_samples=4000,
+                                    random_state=0)
+    assert lv["rts"].shape == (4000, 1) and lv["metadata"]["simulator"] == "levy_flexbound"
+    assert lv["metadata"]["model"] == "levy" and set(np.unique(lv["choices"])) <= {-1, 1}
+    la = Simulator("levy_angle").simulate({"v": 0.5, "a": 1.2, "z": 0.5, "alpha": 1.5, "t": 0.3, "theta": 0.5},
+                                          n_samples=2000, random_state=0)
+    assert np.mean(la["rts"]) < np.mean(lv["rts"])  # collapsing bounds shorten decisions
+    rd = Simulator("racing_diffusion_3").simulate({"v0": 2.0, "v1": 1.0, "v2": 0.5, "A": 0.5, "b": 1.5, "t": 0.2},
+                                                  n_samples=4000, random_state=0)
+    assert rd["metadata"]["simulator"] == "rdm_simulator" and rd["metadata"]["possible_choices"] == [0, 1, 2]
+    ch = rd["choices"].reshape(-1)
+    assert np.mean(ch == 0) > np.mean(ch == 1) > np.mean(ch == 2)  # faster accumulators win more often
+    assert np.all(rd["rts"][ch >= 0] >= 0.2)
+
+
+def test_two_stage_models_through_the_simulator(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    theta = {"vh": 1.0, "vl1": 0.5, "vl2": -0.5, "a": 1.2, "zh": 0.5, "zl1": 0.5, "zl2": 0.5, "t": 0.2}
+    p = Simulator("ddm_par2").simulate(theta, n_samples=3000, random_state=0)
+    q = Simulator("ddm_seq2").simulate(theta, n_samples=3000, random_state=0)
+    assert p["metadata"]["simulator"] == "ddm_flexbound_par2" and q["metadata"]["simulator"] == "ddm_flexbound_seq2"
+    assert p["metadata"]["possible_choices"] == [0, 1, 2, 3]
+    # the extra arrays keep the simulator's [n_samples, n_trials, 1] shape (only rts and choices are squeezed)
+    assert p["rts"].shape == (3000, 1) and p["rts_high"].shape == (3000, 1, 1) and "rts_high" not in q
+    assert np.mean(q["rts"]) > np.mean(p["rts"])  # stages in sequence take longer than in parallel
+    hi, lo = p["rts_high"][:, 0], p["rts_low"][:, 0]
+    assert np.all(p["rts"] >= hi - 1e-3) and np.all(p["rts"] >= lo - 1e-3)
+    for name in ("ddm_par2_no_bias", "ddm_seq2_angle_no_bias", "ddm_par2_weibull_no_bias",
+                 "ddm_seq2_conflict_gamma_no_bias"):
+        cfg = Simulator(name).config
+        th = dict(zip(cfg["params"], cfg["default_params"]))
+        out = Simulator(name).simulate(th, n_samples=500, random_state=1)
+        assert set(np.unique(out["choices"])) <= {0, 1, 2, 3}, name

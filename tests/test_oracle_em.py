@@ -260,3 +260,241 @@ def test_ornstein_noise_free_crossing_matches_the_linear_recurrence(g):
     k = int(n[np.argmax(yn > a)])
     assert np.all(np.abs(steps - k) <= 1) and np.all(ch == 1)
     assert abs(k - 0.6 / DT) > 20  # the leak visibly changes the crossing step
+
+
+# ----------------------------------------------------------------------------- Levy flight
+@pytest.mark.parametrize("alpha", [1.0, 1.25, 1.6, 2.0])
+def test_stable_sampler_matches_the_stable_law(alpha):
+    """CMS draws against SciPy's symmetric stable law (Cauchy at 1, N(0, 2) at 2); a wrong
+    exponent or a swapped cosine moves the law far beyond the KS bound."""
+    x = E.draw_stable(np.random.default_rng(int(alpha * 100)), 3000, F32(alpha)).astype(np.float64)
+    if alpha == 2.0:
+        p = stats.kstest(x / math.sqrt(2.0), "norm").pvalue
+    elif alpha == 1.0:
+        p = stats.kstest(x, "cauchy").pvalue
+    else:
+        p = stats.kstest(x, stats.levy_stable(alpha, 0.0).cdf).pvalue
+    assert p > 1e-3, p
+
+
+def test_stable_sampler_special_values():
+    u = np.array([0.0, 0.3, -1.2], F32)
+    w = np.array([1.0, 0.5, 2.0], F32)
+    # alpha = 1: tan(u) whatever w; alpha = 2: 2 sin(u) sqrt(w) (sin 2u / sqrt(cos u) * sqrt(w / cos u))
+    assert np.allclose(E.stable_cms(u, w, F32(1.0)), np.tan(u), rtol=1e-6)
+    assert np.allclose(E.stable_cms(u, w, F32(2.0)), 2 * np.sin(u) * np.sqrt(w), rtol=1e-5, atol=1e-7)
+
+
+def test_levy_scale_is_the_self_similar_step():
+    # dt^(1/alpha) s: the square-root scale at alpha = 2, dt itself at alpha = 1
+    assert E.levy_scale32([1.0], [2.0], DT)[0] == E.sqrt_dt32(DT)
+    assert E.levy_scale32([2.0], [1.0], DT)[0] == F32(2 * np.float64(F32(DT)))
+
+
+def test_levy_at_alpha_two_walks_as_the_ddm_on_the_same_draws():
+    """At alpha = 2, s = 1 the Levy step scale is float32(sqrt(dt)), and at z = 1/2 both start
+    formulas give 0: fed the same increments, the two walks agree exactly."""
+    rng = np.random.default_rng(4)
+    K, N, L = 3, 200, 257
+    v = rng.uniform(-1, 1, K).astype(F32)
+    a = rng.uniform(0.8, 1.5, K).astype(F32)
+    z = np.full(K, 0.5, F32)
+    t = rng.uniform(0.1, 0.5, K).astype(F32)
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    kw = dict(n_samples=N, delta_t=DT, normals=nz)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), **kw)
+    r1, c1, s1 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), alpha=np.full(K, 2.0, F32), **kw)
+    assert np.array_equal(c0, c1) and np.array_equal(s0, s1) and np.array_equal(r0, r1)
+
+
+def test_levy_native_at_alpha_two_is_the_ddm_with_root_two_noise():
+    # S_2 = N(0, 2): in law the alpha = 2 flight is the DDM with s = sqrt(2)
+    v, a, z, n = 0.5, 1.2, 0.5, 6000
+    r0, c0, _ = E.ddm_flexbound([v], [a], [z], [0.0], [999.0], [1.0], n_samples=n, delta_t=DT, alpha=[2.0],
+                                rng=np.random.default_rng(7))
+    r1, c1, _ = E.ddm_flexbound([v], [a], [z], [0.0], [999.0], [math.sqrt(2.0)], n_samples=n, delta_t=DT,
+                                rng=np.random.default_rng(8))
+    p0, p1 = np.mean(c0 == 1), np.mean(c1 == 1)
+    assert abs(p0 - p1) < 4 * math.sqrt(p0 * (1 - p0) * 2 / n)
+    assert stats.ks_2samp(r0, r1).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("alpha,x0", [(1.0, 0.0), (1.3, 0.0), (1.7, 0.4), (2.0, 0.0)])
+def test_levy_mean_exit_time_matches_getoor(alpha, x0):
+    """Getoor (1961): the symmetric alpha-stable process with exponent |theta|^alpha leaves
+    (-1, 1) from x after E[T] = (1 - x^2)^(alpha/2) / Gamma(1 + alpha) on average. This pins the
+    sampler's normalisation and the dt^(1/alpha) step scale together (a wrong exponent in
+    either shifts E[T] by far more than the bound); the walk is monitored at the steps only,
+    which lengthens it by a few per cent."""
+    n = 4000
+    z = (x0 + 1.0) / 2.0
+    r, _, _ = E.ddm_flexbound([0.0], [1.0], [z], [0.0], [999.0], [1.0], n_samples=n, delta_t=DT, alpha=[alpha],
+                              rng=np.random.default_rng(int(alpha * 10)))
+    exact = (1 - x0 * x0) ** (alpha / 2) / math.gamma(1 + alpha)
+    se = r.std() / math.sqrt(n)
+    assert abs(r.mean() - exact) < 0.07 * exact + 4 * se, (r.mean(), exact)
+
+
+# ----------------------------------------------------------------------------- racing diffusion
+def _rdm_loop(v, A, b, t, s, dl, u, normals, smooth):
+    """One trajectory, scalar float32, in the control flow of race_models.pyx:1200-1255."""
+    P = len(v)
+    dt = F32(DT)
+    sst = F32(E.sqrt_dt32(DT) * F32(s))
+    x = [F32(F32(u[j]) * F32(A)) for j in range(P)]
+    tp, m, winner, found = F32(0.0), 0, -1, False
+    dl_tmp = min(F32(20.0), F32(F32(dl) - F32(t)))
+    while not found and tp <= dl_tmp:
+        for j in range(P):
+            x[j] = F32(x[j] + F32(F32(F32(v[j]) * dt) + F32(sst * normals[m % normals.size])))
+            m += 1
+            if x[j] >= F32(b):
+                found, winner = True, j
+                break
+        if found:
+            break
+        tp = F32(tp + dt)
+    su = E.smooth_unif32(np.array([tp]), np.array([dl_tmp]), dt, np.array([u[P]]))[0] if smooth else F32(0)
+    rt = F32(F32(tp + F32(t)) + su)
+    if rt >= F32(dl) or F32(dl) <= 0:
+        rt = F32(-999.0)
+    if rt == F32(-999.0) or not found:
+        return F32(-999.0), -1
+    return rt, winner
+
+
+def test_racing_diffusion_matches_the_scalar_loop():
+    rng = np.random.default_rng(12)
+    K, N, P, L = 3, 40, 3, 3001
+    v = rng.uniform(0.2, 2.5, (K, P)).astype(F32)
+    A = rng.uniform(0.0, 1.0, K).astype(F32)
+    b = rng.uniform(0.5, 1.5, K).astype(F32)
+    t = rng.uniform(0.0, 0.5, K).astype(F32)
+    t[2] = F32(0.1)
+    dl = np.array([999.0, 999.0, 0.4], F32)  # one set with a deadline that cuts walks off
+    normals = rng.standard_normal((K * N, L)).astype(F32)
+    unif = rng.uniform(size=(K * N, P + 1)).astype(F32)
+    r, c, _ = E.racing_diffusion(v, A, b, t, np.ones(K, F32), dl, n_samples=N, delta_t=DT, smooth_unif=True,
+                                 normals=normals, uniforms=unif)
+    for i in range(K * N):
+        k = i // N
+        r1, c1 = _rdm_loop(v[k], A[k], b[k], t[k], 1.0, dl[k], unif[i], normals[i], True)
+        assert r[i] == r1 and c[i] == c1, i
+    assert np.any(c == -1) and np.any(c >= 0)
+
+
+def test_racing_diffusion_noise_free_crossing_step():
+    # s = 0: accumulator j reaches b after ceil((b - u_j A) / (v_j dt)) steps; the earliest wins and
+    # rt counts the steps completed before the winning one
+    v = np.array([[1.0, 2.0, 1.5]], F32)
+    u = np.array([[0.5, 0.1, 0.9, 0.3]], F32)
+    r, c, st = E.racing_diffusion(v, [0.4], [1.0], [0.2], [0.0], [999.0], n_samples=1,
+                                  normals=np.zeros((1, 8), F32), uniforms=u)
+    need = np.ceil((1.0 - u[0, :3] * 0.4) / (v[0] * DT))
+    j = int(np.argmin(need))
+    assert c[0] == j == 2 and abs(st[0] - need[j]) <= 1
+    assert abs(r[0] - ((st[0] - 1) * DT + 0.2)) < 2e-5
+
+
+def test_racing_diffusion_one_accumulator_is_inverse_gaussian():
+    # A = 0, one accumulator: first passage of a drifted Wiener process through b, minus the one
+    # step that is not counted; mean (b + beta sqrt(dt)) / v - dt (Siegmund), variance b / v^3
+    v, b, n = 1.5, 1.2, 20000
+    r, c, _ = E.racing_diffusion(np.full((1, 1), v, F32), [0.0], [b], [0.0], [1.0], [999.0], n_samples=n,
+                                 delta_t=DT, rng=np.random.default_rng(13))
+    assert np.all(c == 0)
+    mean = (b + ddm_analytic.SIEGMUND_BETA * SQDT) / v - DT
+    assert abs(r.mean() - mean) < 4 * math.sqrt(b / v**3 / n)
+    assert abs(r.var() - b / v**3) < 0.1 * b / v**3
+
+
+# ----------------------------------------------------------------------------- two-stage DDMs
+def _siegmund(v, a, z):
+    a_eff = a + ddm_analytic.SIEGMUND_BETA * SQDT
+    z_eff = ((2 * z - 1) * a + a_eff) / (2 * a_eff)
+    return ddm_analytic.p_upper(v, a_eff, z_eff), ddm_analytic.mean_decision_time(v, a_eff, z_eff)
+
+
+@pytest.mark.parametrize("kind", ["par2", "seq2"])
+def test_two_stage_choices_and_times_match_the_wiener_closed_forms(kind):
+    """With a constant bound both walks are independent Wiener exits: the four choice
+    probabilities are the products of the absorption probabilities (high walk, then the low
+    walk picked by it), and for seq2 the decision time is T_h + T_l, so its mean is the sum of
+    the mean exit times; for par2 it is max(T_h, T_l), at least either mean. A swapped low
+    walker, a wrong coin or a missing stage moves these beyond the bounds."""
+    vh, vl1, vl2, a, zh, zl1, zl2 = 0.9, 0.6, -0.8, 1.0, 0.55, 0.4, 0.6
+    n = 40000
+    r, c, _, rh, rl = E.two_stage(kind, [vh], [vl1], [vl2], [zh], [zl1], [zl2], [0.0], [999.0], [1.0], n_samples=n,
+                                  bparams={"a": [a]}, rng=np.random.default_rng(21 if kind == "par2" else 22))
+    ph, th = _siegmund(vh, a, zh)
+    p1, t1 = _siegmund(vl1, a, zl1)
+    p2, t2 = _siegmund(vl2, a, zl2)
+    expect = [(1 - ph) * (1 - p1), (1 - ph) * p1, ph * (1 - p2), ph * p2]
+    for j in range(4):
+        pj = np.mean(c == j)
+        assert abs(pj - expect[j]) < 4 * math.sqrt(expect[j] * (1 - expect[j]) / n) + 1e-3, (j, pj, expect[j])
+    tl = (1 - ph) * t1 + ph * t2
+    if kind == "seq2":
+        assert abs(r.mean() - (th + tl)) < 4 * r.std() / math.sqrt(n) + 0.004
+        assert rh is None and rl is None
+    else:
+        assert r.mean() > max(th, tl) and r.mean() < th + tl
+        assert abs(rh.mean() - th) < 4 * rh.std() / math.sqrt(n) + 0.003
+        assert np.array_equal(r, np.maximum(rh, rl))
+
+
+def test_two_stage_high_walk_is_the_ddm_walk_on_the_same_draws():
+    """Replay: the high-level walk consumes the first draws of each row exactly as the
+    single-stage walk with the same float start (the Ornstein form at zero leak) does."""
+    rng = np.random.default_rng(23)
+    K, N, L = 3, 200, 4001
+    vh = rng.uniform(-1, 1, K).astype(F32)
+    zh = rng.uniform(0.3, 0.7, K).astype(F32)
+    a = rng.uniform(0.8, 1.5, K).astype(F32)
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    uu = rng.uniform(size=(K * N, 4)).astype(F32)
+    ones, zeros = np.ones(K, F32), np.zeros(K, F32)
+    r0, _, s0 = E.ddm_flexbound(vh, a, zh, zeros, np.full(K, 999, F32), ones, n_samples=N, delta_t=DT, g=zeros,
+                                normals=nz)
+    _, _, _, rh, _ = E.two_stage("par2", vh, vh, vh, zh, zh, zh, zeros, np.full(K, 999, F32), ones, n_samples=N,
+                                 bparams={"a": a}, normals=nz, uniforms=uu)
+    assert np.array_equal(rh, r0)
+
+
+def test_seq2_stops_at_max_t_with_the_prior_low_choice():
+    """A walk that never leaves wide bounds reaches max_t: seq2 then takes the high choice by
+    the coin rule at the walk's position and the low choice with probability zl, and reports
+    t_particle (+ t0) as rt."""
+    n = 4000
+    r, c, _, _, _ = E.two_stage("seq2", [0.0], [0.0], [0.0], [0.5], [0.25], [0.8], [0.1], [999.0], [0.05],
+                                n_samples=n, max_t=2.0, bparams={"a": [50.0]}, rng=np.random.default_rng(24))
+    assert np.all(r > 2.0)
+    hi = c >= 2
+    assert abs(np.mean(hi) - 0.5) < 4 * math.sqrt(0.25 / n)  # y stays near 0: ratio ~ 1/2
+    assert abs(np.mean(c[~hi] == 1) - 0.25) < 4 * math.sqrt(0.25 * 0.75 / max(1, (~hi).sum()))
+    assert abs(np.mean(c[hi] == 3) - 0.8) < 4 * math.sqrt(0.8 * 0.2 / max(1, hi.sum()))
+
+
+def test_seq2_low_start_outside_the_bound_takes_the_prior_choice_without_walking():
+    """Low start -b + 2 zl b outside [-b, b] (zl = 1.2 above, -0.2 below): seq2 takes the low
+    choice by u < zl (always +1, never +1) and does not walk, so rt and the step count are the
+    high walk's, which the single-stage walk on the same draws gives."""
+    rng = np.random.default_rng(25)
+    K, N, L = 2, 300, 4001
+    vh = np.array([0.5, -0.4], F32)
+    zh = np.array([0.5, 0.6], F32)
+    a = np.array([1.0, 0.8], F32)
+    zl1 = np.array([1.2, -0.2], F32)
+    zl2 = np.array([-0.2, 1.2], F32)
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    uu = rng.uniform(size=(K * N, 4)).astype(F32)
+    t0, zeros, ones = np.array([0.2, 0.3], F32), np.zeros(K, F32), np.ones(K, F32)
+    r, c, steps, _, _ = E.two_stage("seq2", vh, vh, vh, zh, zl1, zl2, t0, np.full(K, 999, F32), ones, n_samples=N,
+                                    bparams={"a": a}, normals=nz, uniforms=uu)
+    r0, c0, s0 = E.ddm_flexbound(vh, a, zh, t0, np.full(K, 999, F32), ones, n_samples=N, delta_t=DT, g=zeros,
+                                 normals=nz)
+    assert np.array_equal(steps, s0) and np.array_equal(r, r0)
+    hi = np.repeat(np.arange(K), N)
+    zl_used = np.where(c >= 2, zl2[hi], zl1[hi])
+    assert np.array_equal(c % 2 == 1, zl_used > 1)
+    assert np.array_equal(c >= 2, c0 == 1)  # the high coin is decided by the exit side
