@@ -150,6 +150,65 @@ def accumulator_leg(args):
     return out
 
 
+def hssm_leg(args):
+    """HSSM posterior-predictive draws: 1000 posterior draws x 1000 data trials of the angle model
+    (per-trial drift), one sample each, through hssm_support.rng_fn; the device-only rate of the
+    same 1e6 one-sample sets from a replayed graph; the CPU restatement of the reference's
+    parallel path (per-trial boundary precompute + walk) on a sample of the rows."""
+    import torch
+
+    from oracle import cpu_baseline
+    from oracle import em_numpy as E
+    from ssmhip import batched
+    from ssmhip import hssm_support as H
+
+    draws, trials = 1000, 1000
+    rng = np.random.default_rng(17)
+    lo = np.array([-3.0, 0.3, 0.1, 1e-3, -0.1])
+    hi = np.array([3.0, 3.0, 0.9, 2.0, 1.3])
+    post = rng.uniform(lo, hi, size=(draws, 5)).astype(np.float32)
+    x = rng.uniform(-1, 1, trials).astype(np.float32)  # a per-trial covariate of the drift
+    v = np.clip(post[:, :1] + 0.5 * x[None, :], -3, 3).astype(np.float32)
+    args_ = [v] + [post[:, j:j + 1] for j in range(1, 5)]
+    fn = H.get_simulator_fun_internal("angle")
+    H.rng_fn([a[:10] for a in args_], None, np.random.default_rng(0), fn, 2)  # warm-up (module load)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = H.rng_fn(args_, None, np.random.default_rng(1), fn, 2)
+    wall = time.perf_counter() - t0
+    rows = draws * trials
+    assert out.shape == (draws, trials, 2)
+    theta = np.column_stack([np.broadcast_to(a, (draws, trials)).reshape(-1) for a in args_]).astype(np.float32)
+    sim = batched.BatchedSimulation("angle", theta, 1, seed=2, launch="auto", tune_sample=None,
+                                    queue_order=queue_order(args))
+    ms = time_graph(sim, max(2, args.steps), 1)
+    rec = {"workload": "angle, 1000 posterior draws x 1000 trials, 1 sample each", "rows": rows,
+           "rows_per_s_end_to_end": rows / wall, "end_to_end_s": wall,
+           "device_rows_per_s": rows / (ms / 1e3), "device_ms": ms}
+    if not args.no_cpu:
+        # every 5th row, in chunks of 5000 (the per-trial boundary tables of a chunk take 400 MB)
+        idx = np.arange(0, rows, 5)
+        sec = 0.0
+        for c0 in range(0, idx.size, 5000):
+            th = theta[idx[c0:c0 + 5000]]
+            K = th.shape[0]
+            t1 = time.perf_counter()
+            table = E.boundary_table("angle", {"a": th[:, 1], "theta": th[:, 4]}, K, 20.0, 0.001)
+            dl = np.minimum(np.float32(20.0), (np.float32(999.0) - th[:, 3]).astype(np.float32))
+            cpu_baseline.em_simulate(0, th[:, 0], th[:, 2], np.full(K, E.sqrt_dt32(0.001), np.float32), th[:, 3],
+                                     np.full(K, 999.0, np.float32), dl, table, np.arange(K + 1, dtype=np.int64),
+                                     dt=0.001, seed=1, n_threads=cpu_baseline.n_cores())
+            sec += time.perf_counter() - t1
+        rec["cpu_baseline"] = {"value": idx.size / sec, "unit": "rows/s", "cores": cpu_baseline.n_cores(),
+                               "kind": "port",
+                               "sample": f"{idx.size} of the 1e6 rows (every 5th): NumPy boundary per trial + "
+                                         f"C++/OpenMP walk, {sec:.2f} s"}
+        rec["gpu_over_cpu_end_to_end"] = rec["rows_per_s_end_to_end"] / rec["cpu_baseline"]["value"]
+    log(f"hssm rng_fn: {rows / wall:.3e} rows/s end to end ({wall:.2f} s), device {rows / (ms / 1e3):.3e} rows/s"
+        + (f", {rec['gpu_over_cpu_end_to_end']:.0f}x CPU" if "gpu_over_cpu_end_to_end" in rec else ""))
+    return rec
+
+
 def lan_leg(args):
     """KDE-labelled LAN rows: 1024 angle sets x 1e5 simulations -> 800 KDE + 200 uniform rows per set."""
     import torch
@@ -339,7 +398,7 @@ def main() -> int:
     ap.add_argument("--queue-order", choices=("expected", "index"), default="expected",
                     help="visiting order of the sets: longest expected walks first, or set index order")
     ap.add_argument("--headline-only", action="store_true", help="run the headline leg only")
-    ap.add_argument("--legs", type=str, default="headline,accumulators,lan,surrogate",
+    ap.add_argument("--legs", type=str, default="headline,accumulators,hssm,lan,surrogate",
                     help="comma list of legs to run (surrogate needs lan); without 'headline' the "
                          "JSON line carries no headline value (for profiling one leg)")
     ap.add_argument("--surrogate-epochs", type=int, default=150)
@@ -361,10 +420,10 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     legs = {"headline"} if args.headline_only else set(args.legs.split(","))
-    unknown = legs - {"headline", "accumulators", "lan", "surrogate"}
+    unknown = legs - {"headline", "accumulators", "hssm", "lan", "surrogate"}
     if unknown or ("surrogate" in legs and "lan" not in legs):
-        print(f"bad --legs {args.legs!r}: legs are headline, accumulators, lan, surrogate (surrogate needs lan)",
-              file=sys.stderr)
+        print(f"bad --legs {args.legs!r}: legs are headline, accumulators, hssm, lan, surrogate (surrogate needs "
+              f"lan)", file=sys.stderr)
         return 2
     out = {"metric": "angle-DDM trajectories/s", "value": None, "unit": "trajectories/s", "n_gpus": 1,
            "legs": sorted(legs)}
@@ -372,6 +431,8 @@ def main() -> int:
         out = headline_leg(args, torch, batched, roofline)
     if "accumulators" in legs:
         out["accumulators"] = accumulator_leg(args)
+    if "hssm" in legs:
+        out["hssm"] = hssm_leg(args)
     if "lan" in legs:
         c4, lab, rows = lan_leg(args)
         out["lan_rows"] = c4

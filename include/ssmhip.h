@@ -255,6 +255,20 @@ int32_t ssm_aux_labels(int32_t n_sets, int32_t n_choices, const int32_t* choice_
                        const int64_t* offsets_dev, double max_t, int32_t* counts_dev, int32_t* nogo_dev,
                        int32_t* hist128_dev, int32_t* hist256_dev, void* stream);
 
+/* ssm_filter_stats replaces the statistics of SimulationPipeline._is_valid_simulation
+ * (ssms/dataset_generators/pipelines/simulation_pipeline.py:199-256): for each set k
+ * and choice c, over the draws of [offsets[k], offsets[k+1]) with choice
+ * choice_values[c] and rt != -999,
+ *   stats_dev[k][c][0..4] = (mode, mean, std, mode count / n (0 when n < 5), n)
+ * with the mode the smallest of the most frequent values (scipy.stats.mode) and
+ * (-1, -1, 1, 0, 0) for a choice without draws. Mean and std are formed in double.
+ * workspace_dev: at least ssm_filter_workspace_bytes(n_sets, n_total) bytes
+ * (keys and rocPRIM segmented-sort storage); n_total < 2^32.                */
+int32_t ssm_filter_workspace_bytes(int32_t n_sets, int64_t n_total, size_t* bytes);
+int32_t ssm_filter_stats(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, const float* rts_dev,
+                         const void* choices_dev, int32_t choice_bytes, const int64_t* offsets_dev, int64_t n_total,
+                         void* workspace_dev, size_t workspace_bytes, double* stats_dev, void* stream);
+
 /* ssm_theta_uniform replaces UniformParameterSampler._sample_parameter
  * (ssms/dataset_generators/parameter_samplers/uniform_sampler.py:24-51) for
  * independent bounds: theta_dev[k][j] = float32(lo[j] + (hi[j] - lo[j]) u) in

@@ -6,9 +6,12 @@ parameter set, and the float32 concatenation of TrainingDataGenerator
 (ssms/dataset_generators/lan_mlp.py:533-541) over sets.
 The histograms use np.histogram with explicit np.linspace edges, the same NumPy
 calls the reference makes.
+Also the simulation filter's per-choice statistics and keep rule
+(simulation_pipeline.py:199-256).
 Pinned by: a hand-built case whose counts are written out by hand in
 tests/test_oracle_aux.py (rts exactly on bin edges, at max_t, above max_t,
-omitted, choices outside possible_choices).
+omitted, choices outside possible_choices); the filter statistics by a
+hand-written case (ties for the mode, an empty choice, fewer than 5 values).
 """
 
 from __future__ import annotations
@@ -72,3 +75,35 @@ def labels_all_sets(rts, choices, offsets, possible_choices, max_t=20.0) -> dict
         "binned_128": cat("binned_128"),
         "binned_256": cat("binned_256"),
     }
+
+
+def filter_stats_one_set(rts, choices, possible_choices) -> np.ndarray:
+    """Per-choice (mode, mean, std, mode_cnt_rel, count) of SimulationPipeline._is_valid_simulation
+    (simulation_pipeline.py:218-238): over the rts of each choice without omissions,
+    scipy.stats.mode (smallest of the most frequent values), np.mean / np.std of the float32
+    values, mode count / n (0 below 5 values); (-1, -1, 1, 0, 0) for a choice without draws."""
+    from scipy.stats import mode
+
+    rts = np.asarray(rts, np.float32).reshape(-1)
+    choices = np.asarray(choices).reshape(-1)
+    out = np.zeros((len(possible_choices), 5))
+    for i, c in enumerate(possible_choices):
+        x = rts[(choices == c) & (rts != np.float32(OMISSION))]
+        n = len(x)
+        if n == 0:
+            out[i] = (-1, -1, 1, 0, 0)
+            continue
+        m, cnt = mode(x, keepdims=False)
+        out[i] = (m, np.mean(x), np.std(x), 0 if n < 5 else cnt / n, n)
+    return out
+
+
+def filter_keep(stats: np.ndarray, filters: dict) -> bool:
+    """The keep rule over the choices' statistics (simulation_pipeline.py:239-246); thresholds
+    default to keep-everything when a filter key is absent."""
+    keep = True
+    for mode_, mean_, std_, rel_, n_ in stats:
+        keep = keep & (mode_ <= filters.get("mode", float("inf"))) & (mean_ <= filters.get("mean_rt", float("inf"))) \
+            & (std_ >= filters.get("std", 0.0)) & (rel_ <= filters.get("mode_cnt_rel", 1.0)) \
+            & (n_ >= filters.get("choice_cnt", 0))
+    return bool(keep)

@@ -29,6 +29,7 @@ SOURCES = {
     "lba_kernel.hip": ["-ffp-contract=off"],
     "kde_kernel.hip": [],
     "aux_kernel.hip": [],
+    "filter_kernel.hip": [],
     # MFMA results in VGPRs (no accumulator-register copies before tanh)
     "mlp_kernel.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "probe_kernel.hip": [],

@@ -71,6 +71,10 @@ int32_t aux_labels_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* cho
                            int32_t* hist256, hipStream_t s);
 int32_t theta_uniform_enqueue(int32_t n_sets, int32_t n_params, const double* lo, const double* hi, uint64_t seed,
                               float* theta, hipStream_t s);
+int32_t filter_workspace_bytes(int32_t n_sets, int64_t n_total, size_t* bytes);
+int32_t filter_stats_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice_values, const float* rts,
+                             const void* choices, int32_t choice_bytes, const int64_t* offsets, int64_t n_total,
+                             void* ws, size_t ws_bytes, double* stats, hipStream_t stream);
 int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
                              const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
                              const int32_t* choice_values, float* rows, hipStream_t stream);
@@ -239,6 +243,17 @@ int32_t ssm_aux_labels(int32_t n_sets, int32_t n_choices, const int32_t* choice_
 int32_t ssm_theta_uniform(int32_t n_sets, int32_t n_params, const double* lo_dev, const double* hi_dev,
                           uint64_t seed, float* theta_dev, void* stream) {
   return theta_uniform_enqueue(n_sets, n_params, lo_dev, hi_dev, seed, theta_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_filter_workspace_bytes(int32_t n_sets, int64_t n_total, size_t* bytes) {
+  return filter_workspace_bytes(n_sets, n_total, bytes);
+}
+
+int32_t ssm_filter_stats(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, const float* rts_dev,
+                         const void* choices_dev, int32_t choice_bytes, const int64_t* offsets_dev, int64_t n_total,
+                         void* workspace_dev, size_t workspace_bytes, double* stats_dev, void* stream) {
+  return filter_stats_enqueue(n_sets, n_choices, choice_values_dev, rts_dev, choices_dev, choice_bytes, offsets_dev,
+                              n_total, workspace_dev, workspace_bytes, stats_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows,

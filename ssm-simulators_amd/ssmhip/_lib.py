@@ -123,6 +123,10 @@ _SIGS = {
         [C.c_int32, C.c_int32, _VP, C.c_int32, _VP, _VP, C.c_int32, _VP, C.c_double, _VP, _VP, _VP, _VP, _VP],
     ),
     "ssm_theta_uniform": (C.c_int32, [C.c_int32, C.c_int32, _VP, _VP, C.c_uint64, _VP, _VP]),
+    "ssm_filter_workspace_bytes": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(C.c_size_t)]),
+    "ssm_filter_stats": (
+        C.c_int32, [C.c_int32, C.c_int32, _VP, _VP, _VP, C.c_int32, _VP, C.c_int64, _VP, C.c_size_t, _VP, _VP]
+    ),
     "ssm_mfma_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (

@@ -34,9 +34,11 @@ def deadline_tmp32(max_t: float, deadline: np.ndarray, t: np.ndarray) -> np.ndar
 
 
 def angle_slope32(theta: np.ndarray) -> np.ndarray:
-    """-sin(theta)/cos(theta) evaluated per scalar in float32, as boundary_functions.angle does."""
+    """-sin(theta)/cos(theta) in float32, the slope boundary_functions.angle forms from each
+    trial's scalar theta. NumPy's float32 sin/cos run one SIMD loop with masked tails for
+    any length, so the array call gives the per-scalar values (tests/test_host.py checks it)."""
     th = np.asarray(theta, F32).reshape(-1)
-    return np.array([(-np.sin(x) / np.cos(x)) for x in th], dtype=F32)
+    return (-np.sin(th) / np.cos(th)).astype(F32)
 
 
 def boundary_table(fun, params: dict, n_sets: int, max_t: float, delta_t: float) -> np.ndarray:
@@ -83,7 +85,9 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     replaces sqrt(dt) s."""
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     n = v.shape[0]
-    rec = new_records(n)
+    # the single-particle slots all lie in the first 16 floats: fill them contiguously, then
+    # place them in the 64-float records with one strided copy
+    rec = np.zeros((n, 16), F32)
     for slot, val in ((L.P_SZ, sz), (L.P_SV, sv), (L.P_ST, st), (L.P_G, g)):
         if val is not None:
             rec[:, slot] = np.asarray(val, F32).reshape(n, -1)[:, 0]
@@ -102,7 +106,9 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     a_b = np.asarray(bparams.get("a", a), F32).reshape(n, -1)[:, 0]
     rec[:, L.P_A] = a_b
     fill_boundary(rec, code, bparams)
-    return rec
+    full = new_records(n)
+    full[:, :16] = rec
+    return full
 
 
 def two_stage_records(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, max_t, delta_t, code, bparams) -> np.ndarray:

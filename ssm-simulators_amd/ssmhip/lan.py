@@ -16,6 +16,7 @@ choice) + n_neg negative-RT rows labelled -66.77497, 800 / 100 / 100 by default
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -55,7 +56,14 @@ class LanKDELabeller:
     def __init__(self, model: str, theta: np.ndarray, *, n_samples: int = 100_000, n_kde: int = 800,
                  n_unif: int = 100, n_neg: int = 100, delta_t: float = 0.001, max_t: float = 20.0, seed: int = 0,
                  smooth_unif: bool = True, shuffle: bool = True, launch="auto", lb: float = -66.774,
-                 tune_sample: float | None = 8e6):
+                 tune_sample: float | None = 8e6, filters: dict | None = None, theta_sampler=None,
+                 max_attempts: int = 1000):
+        """`filters`: the simulation filter thresholds (generator_config["simulator"]["filters"]);
+        a set that fails them is re-drawn with `theta_sampler(n, seed)` and re-simulated, up
+        to `max_attempts` times (SimulationPipeline.generate_for_parameter_set,
+        simulation_pipeline.py:103-115); sets that never pass are dropped from the output."""
+        if filters and theta_sampler is None:
+            raise ValueError("a simulation filter needs a theta_sampler to re-draw rejected sets")
         self.model = model
         self.theta = np.ascontiguousarray(theta, np.float32)
         self.n_sets = self.theta.shape[0]
@@ -68,6 +76,13 @@ class LanKDELabeller:
         self.d_theta = D.to_dev(self.theta)
         self.d_choice_values = D.to_dev(np.asarray(self.choice_values, np.int32))
         self.times = StageTimes()
+        self.filters = dict(filters) if filters else None
+        self.theta_sampler = theta_sampler
+        self.max_attempts = int(max_attempts)
+        self.n_samples = n_samples
+        self.sim_kwargs = dict(delta_t=delta_t, max_t=max_t, smooth_unif=smooth_unif, choice_dtype="int8")
+        self.n_resampled = 0
+        self.kept = np.ones(self.n_sets, bool)
 
     @property
     def rows_per_set(self) -> int:
@@ -85,6 +100,8 @@ class LanKDELabeller:
         mark(0)
         if simulate:
             self.sim.run()
+            if self.filters:
+                self._apply_filter()
         mark(1)
         bank = KDEBank(self.sim.rts, self.sim.choices, self.sim.job.offsets, self.choice_values)
         mark(2)
@@ -104,11 +121,71 @@ class LanKDELabeller:
         self.bank, self.points, self.ll = bank, pts, ll
         if self.n_neg:
             rows = self._append_negative(rows)
+        if not self.kept.all():  # sets the filter rejected on every attempt carry no rows
+            keep = D.to_dev(np.flatnonzero(self.kept).astype(np.int64))
+            rows = rows.view(self.n_sets, -1, rows.shape[1])[keep].reshape(-1, rows.shape[1]).contiguous()
         if ev is not None:
             ev[5].synchronize()
             self.times = StageTimes(ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]),
                                     ev[3].elapsed_time(ev[4]), ev[4].elapsed_time(ev[5]))
         return rows
+
+    def filter_stats(self, sim=None) -> np.ndarray:
+        """Per (set, choice) (mode, mean, std, mode share, count) of a simulation's rts
+        without omissions (ssm_filter_stats), float64 [n_sets, n_choices, 5] on the host."""
+        t = D.require_gpu()
+        sim = sim or self.sim
+        S, C = sim.n_sets, len(self.choice_values)
+        nb = ctypes.c_size_t(0)
+        L.check(L.load().ssm_filter_workspace_bytes(S, sim.n_total, ctypes.byref(nb)), "ssm_filter_workspace_bytes")
+        ws = D.empty((max(1, nb.value),), t.uint8)
+        stats = D.empty((S, C, 5), t.float64)
+        rc = L.load().ssm_filter_stats(S, C, D.ptr(self.d_choice_values), D.ptr(sim.rts), D.ptr(sim.choices),
+                                       sim.job.prob.choice_bytes, D.ptr(sim.job.offsets), sim.n_total, D.ptr(ws),
+                                       nb.value, D.ptr(stats), D.stream_ptr())
+        L.check(rc, "ssm_filter_stats")
+        return stats.cpu().numpy()
+
+    def _keep(self, stats: np.ndarray) -> np.ndarray:
+        """The keep rule of simulation_pipeline.py:239-246 for every set at once."""
+        f = self.filters
+        mode_, mean_, std_, rel_, n_ = (stats[:, :, i] for i in range(5))
+        ok = ((mode_ <= f.get("mode", np.inf)) & (mean_ <= f.get("mean_rt", np.inf)) & (std_ >= f.get("std", 0.0))
+              & (rel_ <= f.get("mode_cnt_rel", 1.0)) & (n_ >= f.get("choice_cnt", 0)))
+        return ok.all(axis=1)
+
+    def _apply_filter(self) -> None:
+        """Re-draw and re-simulate the rejected sets until they pass or the attempts run out;
+        accepted replacements are copied into the batch's rts / choices and theta."""
+        t = D.torch()
+        n = self.sim.offsets[1:] - self.sim.offsets[:-1]
+        if not np.all(n == n[0]):
+            raise ValueError("the simulation filter needs the same n_samples for every set")
+        n0 = int(n[0])
+        rejected = np.flatnonzero(~self._keep(self.filter_stats()))
+        rts = self.sim.rts.view(self.n_sets, n0)
+        chs = self.sim.choices.view(self.n_sets, n0)
+        attempt = 0
+        while rejected.size and attempt < self.max_attempts:
+            attempt += 1
+            th = np.ascontiguousarray(self.theta_sampler(rejected.size, self.seed + 104_729 * attempt), np.float32)
+            sub = BatchedSimulation(self.model, th, n0, seed=self.seed + 7 * attempt, launch=self.sim.launch_params,
+                                    **self.sim_kwargs)
+            sub.run()
+            ok = self._keep(self.filter_stats(sub))
+            if ok.any():
+                src = D.to_dev(np.flatnonzero(ok).astype(np.int64))
+                dst = D.to_dev(rejected[ok].astype(np.int64))
+                rts[dst] = sub.rts.view(-1, n0)[src]
+                chs[dst] = sub.choices.view(-1, n0)[src]
+                self.theta[rejected[ok]] = th[ok]
+                self.n_resampled += int(ok.sum())
+            sub.close()
+            rejected = rejected[~ok]
+        self.kept[:] = True
+        self.kept[rejected] = False
+        self.d_theta = D.to_dev(self.theta)
+        t.cuda.current_stream().synchronize()
 
     def _append_negative(self, rows):
         """Add n_neg negative-RT rows per set: rt ~ U(-1, 1e-4), random choice, label -66.77497
@@ -178,9 +255,9 @@ class LanKDELabeller:
         lan_data / lan_labels from `rows`, and with `aux` the auxiliary labels of the same
         simulations (call before the next simulate)."""
         r = rows.cpu().numpy()
-        out = {"lan_data": r[:, :-1], "lan_labels": r[:, -1], "theta": self.theta.astype(np.float32)}
+        out = {"lan_data": r[:, :-1], "lan_labels": r[:, -1], "theta": self.theta[self.kept].astype(np.float32)}
         if aux:
-            out.update(self.aux_labels())
+            out.update({k: v[self.kept] for k, v in self.aux_labels().items()})
         return out
 
 

@@ -84,3 +84,68 @@ def test_theta_sampler_bounds_determinism_and_uniformity(torch_gpu):
     assert abs(np.corrcoef(a[:, 0], a[:, 1])[0, 1]) < 0.05
     with pytest.raises(ValueError):
         sample_theta(10, [1.0], [0.0])
+
+
+@pytest.mark.parametrize("choice_bytes", [1, 4])
+def test_filter_stats_match_oracle(torch_gpu, choice_bytes):
+    """Per-set, per-choice (mode, mean, std, mode share, count) of ragged sets against the
+    scipy/NumPy restatement: counts, modes and shares exact (ties: the smaller value), mean and
+    std within float32 rounding."""
+    import ctypes
+
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+
+    torch = torch_gpu
+    rng = np.random.default_rng(5)
+    sizes = np.array([0, 1, 4, 5, 3000, 777, 0, 20000])
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    # a coarse grid gives ties (several values sharing the top count), plus continuous values
+    rts = np.where(rng.random(n) < 0.5, rng.integers(0, 40, n) * np.float32(0.025),
+                   rng.uniform(0.0, 3.0, n)).astype(np.float32)
+    rts[rng.random(n) < 0.03] = -999.0
+    vals = [-1, 1, 7]  # 7 never occurs: an empty choice in every set
+    ch = rng.choice([-1, 1, 3], n, p=[0.45, 0.45, 0.1]).astype(np.int8 if choice_bytes == 1 else np.int32)
+    S, C = sizes.size, len(vals)
+    d_vals, d_rts, d_ch, d_offs = (D.to_dev(x) for x in (np.array(vals, np.int32), rts, ch, offs))
+    nb = ctypes.c_size_t(0)
+    L.check(L.load().ssm_filter_workspace_bytes(S, n, ctypes.byref(nb)))
+    ws = D.empty((nb.value,), torch.uint8)
+    stats = D.empty((S, C, 5), torch.float64)
+    rc = L.load().ssm_filter_stats(S, C, D.ptr(d_vals), D.ptr(d_rts), D.ptr(d_ch), choice_bytes, D.ptr(d_offs), n,
+                                   D.ptr(ws), nb.value, D.ptr(stats), D.stream_ptr())
+    L.check(rc)
+    got = stats.cpu().numpy()
+    for k in range(S):
+        ref = aux_ref.filter_stats_one_set(rts[offs[k]:offs[k + 1]], ch[offs[k]:offs[k + 1]], vals)
+        assert np.array_equal(got[k][:, [0, 3, 4]], ref[:, [0, 3, 4]]), k
+        assert np.allclose(got[k][:, 1:3], ref[:, 1:3], rtol=2e-6, atol=1e-6), k
+    assert np.all(got[:, 2, :] == [-1, -1, 1, 0, 0])
+
+
+def test_filter_resamples_rejected_sets_until_they_pass(torch_gpu):
+    """A filter on the mean rt: rejected parameter sets are re-drawn and re-simulated; every
+    returned set passes, and an impossible filter drops all sets."""
+    from ssmhip.lan import LanKDELabeller, sample_theta
+
+    lo, hi = [-3.0, 0.3, 0.1, 1e-3, -0.1], [3.0, 3.0, 0.9, 2.0, 1.3]
+
+    def sampler(n, seed):
+        return sample_theta(n, lo, hi, seed=seed)
+
+    theta = sampler(64, 1)
+    filt = {"mean_rt": 1.5, "choice_cnt": 10}
+    lab = LanKDELabeller("angle", theta, n_samples=2000, n_kde=80, n_unif=10, n_neg=10, seed=3, launch=None,
+                         filters=filt, theta_sampler=sampler)
+    rows = lab.run()
+    st = lab.filter_stats()
+    assert lab.n_resampled > 0 and lab.kept.all()
+    assert np.all(st[:, :, 1] <= 1.5) and np.all(st[:, :, 4] >= 10)
+    out = lab.output_dict(rows)
+    assert out["lan_data"].shape == (64 * 100, 7) and out["theta"].shape == (64, 5)
+    assert np.array_equal(out["theta"], lab.theta)
+    never = LanKDELabeller("angle", theta[:4], n_samples=500, n_kde=8, n_unif=1, n_neg=1, seed=4, launch=None,
+                           filters={"mean_rt": -1.0}, theta_sampler=sampler, max_attempts=3)
+    rows = never.run()
+    assert not never.kept.any() and rows.shape[0] == 0

@@ -116,3 +116,14 @@ def test_training_data_file_format(tmp_path):
         back = pickle.load(f)  # a file this test wrote
     assert set(back) == {"lan_data", "lan_labels", "binned_128", "generator_config", "model_config"}
     assert np.array_equal(back["lan_data"], data["lan_data"]) and back["model_config"]["name"] == "angle"
+
+
+def test_angle_slope_array_call_equals_the_per_trial_scalar_calls():
+    """The reference evaluates the angle boundary with one scalar theta per trial; the records
+    take the slope from one array call, which must give the same float32 values."""
+    from ssmhip import em
+
+    th = np.random.default_rng(0).uniform(-0.1, 1.3, 50_000).astype(np.float32)
+    th[:3] = np.float32([0.0, 1.3, -0.1])
+    scalar = np.array([(-np.sin(x) / np.cos(x)) for x in th], dtype=np.float32)
+    assert np.array_equal(em.angle_slope32(th), scalar)

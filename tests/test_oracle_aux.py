@@ -36,3 +36,22 @@ def test_concatenation_shapes_and_float32():
     two = aux_ref.labels_all_sets(rts, np.where(ch > 0, 1, -1), offs, [-1, 1])
     assert two["cpn_labels"].shape == (3, 1)
     assert np.isclose(two["cpn_labels"][0, 0], np.mean(np.where(ch[:100] > 0, 1, -1) == 1))
+
+
+def test_filter_stats_hand_case():
+    """Mode ties resolve to the smaller value, omissions and foreign choices are excluded, a
+    choice with under 5 values has mode share 0, an empty choice gets (-1, -1, 1, 0, 0)."""
+    rts = np.array([0.5, 0.7, 0.7, 0.5, 0.9, -999.0, 0.3, 0.3, 0.3, 0.8, 0.8, 1.0], np.float32)
+    ch = np.array([1, 1, 1, 1, 1, 1, -1, -1, 5, 1, 1, 1])
+    st = aux_ref.filter_stats_one_set(rts, ch, [-1, 1, 2])
+    # choice -1: [0.3, 0.3] -> mode 0.3 (count 2), n = 2 < 5 -> share 0
+    assert st[0, 0] == np.float32(0.3) and st[0, 3] == 0 and st[0, 4] == 2
+    # choice 1: [0.5, 0.7, 0.7, 0.5, 0.9, 0.8, 0.8, 1.0]: 0.5, 0.7, 0.8 twice each -> 0.5, share 2/8
+    x = np.array([0.5, 0.7, 0.7, 0.5, 0.9, 0.8, 0.8, 1.0], np.float32)
+    assert st[1, 0] == np.float32(0.5) and st[1, 3] == 0.25 and st[1, 4] == 8
+    assert abs(st[1, 1] - x.mean()) < 1e-7 and abs(st[1, 2] - x.std()) < 1e-7
+    assert tuple(st[2]) == (-1, -1, 1, 0, 0)
+    assert aux_ref.filter_keep(st, {})
+    assert not aux_ref.filter_keep(st, {"choice_cnt": 1})  # choice 2 has no draws
+    assert not aux_ref.filter_keep(st, {"mean_rt": 0.5}) and aux_ref.filter_keep(st, {"mean_rt": 0.8})
+    assert not aux_ref.filter_keep(st, {"mode_cnt_rel": 0.2}) and aux_ref.filter_keep(st, {"mode_cnt_rel": 0.25})
