@@ -51,6 +51,7 @@ extern "C" {
                                 choice in 0..3, rt = max(t_h, t_l) (parallel_models.pyx:57-560) */
 #define SSM_MODEL_SEQ2 8     /* ddm_flexbound_seq2: the low walk continues where the high
                                 walk ended; choice in 0..3 (sequential_models.pyx:55-640) */
+#define SSM_MODEL_DDM_LEGACY 9 /* ddm: bounds 0 and a, start z a (ddm_models.pyx:349-582) */
 
 #define SSM_BOUNDARY_CONSTANT 0 /* b(t) = a                                  */
 #define SSM_BOUNDARY_ANGLE 1    /* b(t) = a + t_s * slope, slope = -sin/cos  */

@@ -3,6 +3,7 @@
 Follows, operation by operation in float32:
   ddm_flexbound  src/cssm/ddm_models.pyx:696-744  (start :719, walk :721-733,
                  smooth_unif :735-739, rt :741, sign :742, deadline :744)
+  ddm            src/cssm/ddm_models.pyx:453-482 (legacy bounds 0 and a)
   full_ddm       src/cssm/ddm_models.pyx:2284-2358 (variability of v, z, t)
   ornstein_uhlenbeck  src/cssm/ornstein_models.pyx:156-198 (leaky drift)
   levy_flexbound src/cssm/levy_models.pyx:160-220 (alpha-stable increments, CMS
@@ -153,7 +154,7 @@ def _finish(rts, t_el, t0, dl_tmp, deadline, dt, smooth, uniforms):
 
 def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
                   bparams=None, smooth_unif=False, normals=None, uniforms=None, rng=None, table=None,
-                  sz=None, sv=None, st=None, g=None, alpha=None):
+                  sz=None, sv=None, st=None, g=None, alpha=None, legacy=False):
     """Returns (rts float32[n_total], choices int32[n_total], steps int64[n_total]), set-major.
 
     With sz/sv/st: full_ddm (src/cssm/ddm_models.pyx:2284-2358): per trajectory the drift
@@ -165,7 +166,9 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     -b0 + (2z) b0 (the literal 2 is an int) and step ((v - g y) dt) + sqrt_st gm.
     With alpha: levy_flexbound (src/cssm/levy_models.pyx:160-220): the same float start, step
     (v dt) + (delta_t_alpha x) with x symmetric alpha-stable (stable_cms) and
-    delta_t_alpha = levy_scale32(s, alpha, dt); replay `normals` then hold the stable variates."""
+    delta_t_alpha = levy_scale32(s, alpha, dt); replay `normals` then hold the stable variates.
+    With legacy=True: ddm (src/cssm/ddm_models.pyx:453-482): bounds 0 and a (inclusive), float
+    start z a, the same step; `a` is the constant bound."""
     full = sz is not None or sv is not None or st is not None
     ou = g is not None
     levy = alpha is not None
@@ -187,7 +190,9 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     if levy:
         alpha_k = np.asarray(alpha, F32).reshape(-1)
         sqrt_st_k = levy_scale32(s, alpha_k, delta_t)
-    if ou or levy:
+    if legacy:
+        y = (z[lane_set] * b0).astype(F32)
+    elif ou or levy:
         y = ((-b0) + ((z[lane_set] * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
     else:
         y = ((-b0).astype(np.float64) + (z[lane_set].astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
@@ -221,7 +226,8 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
         inb = ixa < nst
         b = B[ks, np.minimum(ixa, nst - 1)]
         ya = y[active]
-        go = inb & (ya >= -b) & (ya <= b) & (tt[active] <= dl_tmp_k[ks])
+        lo = F32(0.0) if legacy else -b
+        go = inb & (ya >= lo) & (ya <= b) & (tt[active] <= dl_tmp_k[ks])
         active = active[go]
         if not active.size:
             break

@@ -98,7 +98,16 @@ __device__ __forceinline__ float step_increment(float y, float vdt, float v, flo
 // the single-particle diffusions that share the DDM walk
 template <int MODEL>
 constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM || MODEL == SSM_MODEL_ORNSTEIN ||
-                            MODEL == SSM_MODEL_LEVY || MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
+                            MODEL == SSM_MODEL_LEVY || MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2 ||
+                            MODEL == SSM_MODEL_DDM_LEGACY;
+
+// The walk continues while y lies inside the bounds: [-b, b], or [0, b] for the legacy DDM
+// (ddm_models.pyx:463: y <= a and y >= 0), both inclusive.
+template <int MODEL>
+__device__ __forceinline__ bool inside(float y, float b) {
+  if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) return (y <= b) && (y >= 0.f);
+  else return (y >= -b) && (y <= b);
+}
 // hierarchical two-choice DDMs: a high walk, then a low walk chosen by its outcome
 template <int MODEL>
 constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
@@ -305,7 +314,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               const float z = sp[SSM_P_Z];
               vdt[0] = sp[SSM_P_V] * dt;
               sst[0] = sp[SSM_P_SQRT_ST];
-              if constexpr (MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_LEVY || kTwoStage<MODEL>) {
+              if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) {
+                x[0] = z * b0;  // ddm_models.pyx:457: start z a, in float
+              } else if constexpr (MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_LEVY || kTwoStage<MODEL>) {
                 // ornstein_models.pyx:165, levy_models.pyx:183, parallel_models.pyx:452,
                 // sequential_models.pyx:460: the literal 2 is an int, so this stays in float
                 x[0] = (-b0) + ((z * 2.0f) * b0);
@@ -442,7 +453,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           for (int i = 0; i < 4; ++i) {
             if (alive) {
               const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-              if ((x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp)) {
+              if (inside<MODEL>(x[0], b) && (t <= dl_tmp)) {
                 x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                 t += dt;
                 ++ix;
@@ -458,7 +469,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               bool go = ix < A.n_steps;  // never read past a boundary table
               if (go) {
                 const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-                go = (x[0] >= -b) && (x[0] <= b) && (t <= dl_tmp);
+                go = inside<MODEL>(x[0], b) && (t <= dl_tmp);
               }
               if (go) {
                 x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
@@ -638,6 +649,8 @@ static KernelFn pick_kernel(int model, int bound, int p, bool replay) {
     return replay ? pick_bound<SSM_MODEL_ORNSTEIN, true>(bound, p) : pick_bound<SSM_MODEL_ORNSTEIN, false>(bound, p);
   if (model == SSM_MODEL_LEVY)
     return replay ? pick_bound<SSM_MODEL_LEVY, true>(bound, p) : pick_bound<SSM_MODEL_LEVY, false>(bound, p);
+  if (model == SSM_MODEL_DDM_LEGACY)
+    return replay ? pick_bound<SSM_MODEL_DDM_LEGACY, true>(bound, p) : pick_bound<SSM_MODEL_DDM_LEGACY, false>(bound, p);
   if (model == SSM_MODEL_PAR2)
     return replay ? pick_bound<SSM_MODEL_PAR2, true>(bound, p) : pick_bound<SSM_MODEL_PAR2, false>(bound, p);
   if (model == SSM_MODEL_SEQ2)
@@ -652,11 +665,12 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
                     const float* table, const float* rn, const float* ru, float* rts, void* choices,
                     uint64_t* ws) {
   if (!P) return fail(SSM_EINVAL, "problem descriptor is NULL");
-  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_SEQ2) return fail(SSM_EINVAL, "unknown model");
+  if (P->model < SSM_MODEL_DDM || P->model > SSM_MODEL_DDM_LEGACY) return fail(SSM_EINVAL, "unknown model");
   if (P->boundary < SSM_BOUNDARY_CONSTANT || P->boundary > SSM_BOUNDARY_TABLE) return fail(SSM_EINVAL, "unknown boundary");
   const bool diffusion = P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM ||
                          P->model == SSM_MODEL_ORNSTEIN || P->model == SSM_MODEL_LEVY ||
-                         P->model == SSM_MODEL_PAR2 || P->model == SSM_MODEL_SEQ2;
+                         P->model == SSM_MODEL_PAR2 || P->model == SSM_MODEL_SEQ2 ||
+                         P->model == SSM_MODEL_DDM_LEGACY;
   if (diffusion && P->n_particles != 1) return fail(SSM_EINVAL, "single-particle models require n_particles == 1");
   if (P->n_particles < 1 || P->n_particles > SSM_MAX_PARTICLES)
     return fail(SSM_EINVAL, "n_particles must be in [1, 16]");

@@ -22,7 +22,7 @@ SSM_ENOMEM = -3
 SSM_EUNSUPPORTED = -4
 
 MODEL_DDM, MODEL_RACE, MODEL_LCA, MODEL_FULL_DDM, MODEL_ORNSTEIN, MODEL_LEVY, MODEL_RDM = 0, 1, 2, 3, 4, 5, 6
-MODEL_PAR2, MODEL_SEQ2 = 7, 8
+MODEL_PAR2, MODEL_SEQ2, MODEL_DDM_LEGACY = 7, 8, 9
 BOUNDARY_CONSTANT, BOUNDARY_ANGLE, BOUNDARY_WEIBULL, BOUNDARY_TABLE = 0, 1, 2, 3
 MAX_PARTICLES = 16
 SET_STRIDE = 64

@@ -149,6 +149,9 @@ def _registry() -> dict:
         "weibull": _ddm_like("weibull", ["v", "a", "z", "t", "alpha", "beta"],
                              [[-2.5, 0.3, 0.2, 1e-3, 0.31, 0.31], [2.5, 2.5, 0.8, 2.0, 4.99, 6.99]],
                              [0.0, 1.0, 0.5, 1e-3, 3.0, 3.0], "weibull_cdf"),
+        # ddm.py:30-48 (the [0, a]-bound legacy simulator)
+        "ddm_legacy": _ddm_like("ddm_legacy", ["v", "a", "z", "t"], [[-3.0, 0.3, 0.1, 0.0], [3.0, 2.5, 0.9, 2.0]],
+                                [0.0, 1.0, 0.5, 1e-3], "constant", cssm.ddm),
         # full_ddm.py:11-33, ornstein.py:7-48
         "full_ddm": _ddm_like("full_ddm", ["v", "a", "z", "t", "sz", "sv", "st"],
                               [[-3.0, 0.3, 0.3, 0.25, 1e-3, 1e-3, 1e-3], [3.0, 2.5, 0.7, 2.25, 0.2, 2.0, 0.25]],

@@ -134,6 +134,20 @@ def ddm_flexbound(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20
                       {"v": v, "a": a, "z": z, "t": t, "s": s, "deadline": deadline})
 
 
+def ddm(v, a, z, t, deadline, s, max_t=20, delta_t=0.001, n_samples=20000, n_trials=10, random_state=None,
+        return_option="full", smooth_unif=False, n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):
+    """DDM between the bounds 0 and a, started at z a (ddm_models.pyx:349-582, sequential loop
+    :453-482): y += (v dt) + (sqrt(dt) s g) while 0 <= y <= a; choice = sign(y)."""
+    n_trials = int(n_trials)
+    a32 = np.asarray(a, np.float32).reshape(-1)
+    out = _diffusion("ddm", L.MODEL_DDM_LEGACY, v, a32, z, t, deadline, s, max_t, delta_t, n_samples, n_trials,
+                     constant, {"a": a32}, random_state, return_option, smooth_unif, n_threads, replay_normals,
+                     replay_uniforms, {}, {"v": v, "a": a, "z": z, "t": t, "deadline": deadline, "s": s})
+    if return_option == "full":  # the legacy simulator reports no boundary array (ddm_models.pyx:565-573)
+        out["metadata"].pop("boundary", None)
+    return out
+
+
 def full_ddm(v, a, z, t, sz, sv, st, deadline, s, delta_t=0.001, max_t=20, n_samples=20000, n_trials=1,
              boundary_fun=None, boundary_params={}, random_state=None, return_option="full", smooth_unif=False,
              n_threads=1, replay_normals=None, replay_uniforms=None, **kwargs):

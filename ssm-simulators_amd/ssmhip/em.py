@@ -233,7 +233,8 @@ def expected_steps(records: np.ndarray, kind: int, n_particles: int, max_t: floa
     -a + 2 z a (constant bound) for the diffusions; a / max_j v_j for the accumulators."""
     rec = np.asarray(records, np.float64)
     a = np.maximum(rec[:, L.P_A], 1e-6)
-    if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN, L.MODEL_LEVY, L.MODEL_PAR2, L.MODEL_SEQ2):
+    if kind in (L.MODEL_DDM, L.MODEL_FULL_DDM, L.MODEL_ORNSTEIN, L.MODEL_LEVY, L.MODEL_PAR2, L.MODEL_SEQ2,
+                L.MODEL_DDM_LEGACY):
         v = rec[:, L.P_V]  # two-stage models: the high walk's
         s2 = np.maximum((rec[:, L.P_SQRT_ST] ** 2) / delta_t, 1e-12)  # noise variance s^2
         A_, x0 = 2 * a, 2 * a * np.clip(rec[:, L.P_Z], 0.0, 1.0)

@@ -44,7 +44,7 @@ def test_em_rejects_bad_descriptor_before_touching_the_gpu():
     from ssmhip import _lib
 
     lib = _lib.load()
-    prob = _lib.EmProblem(model=9, boundary=0, n_particles=1, n_sets=1, n_total=1, n_steps=10, dt=0.001)
+    prob = _lib.EmProblem(model=99, boundary=0, n_particles=1, n_sets=1, n_total=1, n_steps=10, dt=0.001)
     rc = lib.ssm_em_simulate(C.byref(prob), None, *([None] * 8), None)
     assert rc == _lib.SSM_EINVAL
     assert "unknown model" in _lib.last_error()

@@ -670,3 +670,17 @@ def test_two_stage_native_matches_numpy_oracle_statistically(torch_gpu, kind):
     r0, c0, _, _, _ = E.two_stage(kind, *v, *z, t, [999.0], [1.0], n_samples=N, delta_t=DT, bparams=bp,
                                   smooth_unif=True, rng=np.random.default_rng(20))
     _compare_samples(planes[:N], c1, r0, c0, (0, 1, 2, 3))
+
+
+def test_legacy_ddm_replay_matches_oracle(torch_gpu):
+    L, _ = _ssm()
+    K, N, L_ = 12, 400, 1031
+    print("[case] legacy ddm replay", flush=True)
+    v, a, z, t, _ = _ddm_case("constant", K, 91)
+    rng = np.random.default_rng(92)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 smooth_unif=True, normals=normals, uniforms=unif, legacy=True)
+    r1, c1, steps = _run_gpu_diffusion(L.MODEL_DDM_LEGACY, v, a, z, t, {"a": a}, "constant", N, normals, unif, True)
+    assert np.array_equal(c0, c1) and np.array_equal(r0, r1) and steps == int(s0.sum())

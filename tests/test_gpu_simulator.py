@@ -181,3 +181,13 @@ def test_two_stage_models_through_the_simulator(torch_gpu):
         th = dict(zip(cfg["params"], cfg["default_params"]))
         out = Simulator(name).simulate(th, n_samples=500, random_state=1)
         assert set(np.unique(out["choices"])) <= {0, 1, 2, 3}, name
+
+
+def test_legacy_ddm_through_the_simulator(torch_gpu):
+    from ssmhip.simulator import Simulator
+
+    out = Simulator("ddm_legacy").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3}, n_samples=4000, random_state=0)
+    assert out["metadata"]["simulator"] == "ddm" and "boundary" not in out["metadata"]
+    assert set(np.unique(out["choices"])) <= {-1, 1} and np.all(out["rts"] > 0.3)
+    base = Simulator("ddm").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3}, n_samples=4000, random_state=0)
+    assert np.mean(out["rts"]) < np.mean(base["rts"])  # bounds a apart instead of 2a

@@ -498,3 +498,39 @@ def test_seq2_low_start_outside_the_bound_takes_the_prior_choice_without_walking
     zl_used = np.where(c >= 2, zl2[hi], zl1[hi])
     assert np.array_equal(c % 2 == 1, zl_used > 1)
     assert np.array_equal(c >= 2, c0 == 1)  # the high coin is decided by the exit side
+
+
+# ----------------------------------------------------------------------------- legacy DDM
+@pytest.mark.parametrize("v,a,z", [(1.0, 1.5, 0.5), (-0.7, 1.0, 0.35)])
+def test_legacy_ddm_matches_wiener_closed_form(v, a, z):
+    """Bounds 0 and a from z a: the Wiener absorption probability and mean exit time of the
+    interval (0, a) (both bounds moved out by beta sqrt(dt))."""
+    n = 20000
+    rts, ch, _ = E.ddm_flexbound([v], [a], [z], [0.0], [999.0], [1.0], n_samples=n, delta_t=DT, legacy=True,
+                                 rng=np.random.default_rng(31))
+    sh = ddm_analytic.SIEGMUND_BETA * SQDT
+    # ddm_analytic works on (-A, A) from (2z' - 1) A: width a + 2 sh, start z a + sh from the lower bound
+    A = (a + 2 * sh) / 2
+    z_eff = (z * a + sh) / (2 * A)
+    p = ddm_analytic.p_upper(v, A, z_eff)
+    assert abs(np.mean(ch == 1) - p) < 4 * math.sqrt(p * (1 - p) / n)
+    mt = ddm_analytic.mean_decision_time(v, A, z_eff)
+    assert abs(rts.mean() - mt) < 4 * rts.std() / math.sqrt(n) + 0.003
+
+
+def test_legacy_ddm_is_the_symmetric_walk_shifted_by_half_the_bound():
+    """With the same draws, the [0, a] walk from z a and the [-a/2, a/2] walk from (z - 1/2) a
+    differ by the constant a/2 only; with a and z exact in binary and a short walk, float
+    rounding leaves the two crossing steps and choices equal."""
+    rng = np.random.default_rng(32)
+    K, N, L = 2, 300, 2001
+    nz = rng.standard_normal((K * N, L)).astype(F32)
+    v = np.array([0.5, -1.0], F32)
+    z = np.array([0.5, 0.25], F32)
+    a = np.array([2.0, 1.0], F32)
+    t = np.zeros(K, F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 legacy=True, normals=nz)
+    r1, c1, s1 = E.ddm_flexbound(v, a / 2, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
+                                 normals=nz)
+    assert np.mean(s0 == s1) > 0.99 and np.mean(c0 == c1) > 0.99
