@@ -102,11 +102,13 @@ constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DD
                             MODEL == SSM_MODEL_DDM_LEGACY;
 
 // The walk continues while y lies inside the bounds: [-b, b], or [0, b] for the legacy DDM
-// (ddm_models.pyx:463: y <= a and y >= 0), both inclusive.
+// (ddm_models.pyx:463: y <= a and y >= 0), both inclusive. -b <= y <= b is tested as
+// |y| <= b, one compare with an abs source modifier; the two agree for every float y and b,
+// signed zeros, infinities and NaN (false either way) included.
 template <int MODEL>
 __device__ __forceinline__ bool inside(float y, float b) {
   if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) return (y <= b) && (y >= 0.f);
-  else return (y >= -b) && (y <= b);
+  else return fabsf(y) <= b;
 }
 // hierarchical two-choice DDMs: a high walk, then a low walk chosen by its outcome
 template <int MODEL>

@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const float b = a + ((float)ix * dt) * slope;
-      const bool go = (y >= -b) && (y <= b) && (t <= 20.f);
+      const bool go = (fabsf(y) <= b) && (t <= 20.f);  // the simulator's bound test
       const float yn = y + (vdt + sst * nz[i]);
       y = go ? yn : -y;
       t = go ? t + dt : 0.f;
