@@ -114,6 +114,10 @@ __device__ __forceinline__ bool inside(float y, float b) {
 template <int MODEL>
 constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
 
+// blocks of four diffusion steps per iteration of the walk loop (the loop's own work, the
+// state ballots and the refill test, is paid once per iteration)
+constexpr int kSubBlocks = 2;
+
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
 __device__ __forceinline__ bool two_stage_coin(float u, float y, float b) {
@@ -420,69 +424,79 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
 
     // ---------------- advance running lanes ----------------
     ++iters;
-    if (state == kRunning) {
-      if constexpr (kDiffusion<MODEL>) {
-        // four steps per Philox block; every walk starts block-aligned
-        float nz[4];
-        if constexpr (REPLAY) {
-          // a full-DDM row starts with the drift draw, the walk's draws follow it; a two-stage
-          // row holds the high walk's draws, then the low walk's
-          constexpr int off = MODEL == SSM_MODEL_FULL_DDM ? 1 : 0;
-          const float* rb = A.replay_normals + g * A.replay_stride;
-          const int64_t d0 = kTwoStage<MODEL> ? (int64_t)nh + (ix - ix0) : (int64_t)(off + ix);
+    if constexpr (kDiffusion<MODEL>) {
+      // up to kSubBlocks blocks of four steps per loop iteration, one Philox block each;
+      // every walk starts block-aligned, so the draws do not depend on the grouping
+      bool alive = state == kRunning;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) nz[i] = rb[(d0 + i) % A.replay_stride];
-        } else if constexpr (MODEL == SSM_MODEL_LEVY) {
-          // two Philox blocks per four steps: (U, W) pairs for four stable variates
-          const u32x4 r0 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u}, A.k0, A.k1);
-          const u32x4 r1 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 2u}, A.k0, A.k1);
-          nz[0] = stable_cms(r0.x, r0.y, alpha, alpha_inv, alpha_scale);
-          nz[1] = stable_cms(r0.z, r0.w, alpha, alpha_inv, alpha_scale);
-          nz[2] = stable_cms(r1.x, r1.y, alpha, alpha_inv, alpha_scale);
-          nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
-        } else if constexpr (kTwoStage<MODEL>) {
-          // each stage's draws start block-aligned in a stream of their own (0 high, 4 low)
-          philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ix - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
-        } else {
-          philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
+      for (int sb = 0; sb < kSubBlocks; ++sb) {
+        if (sb > 0) {
+          if (__ballot(alive) == 0) break;  // wave-uniform: no lane left to step
+          ++iters;
         }
-        bool alive = true;
-        // Steps use exec-mask branches rather than selects: with every lane usually on the
-        // same side, a branch costs scalar instructions while a select costs VALU issue slots.
-        if (__ballot(ix + 4 > A.n_steps) == 0) {
-          // common case: no lane can reach the end of t_s in this block, no index guard needed
+        if (alive) {
+          float nz[4];
+          if constexpr (REPLAY) {
+            // a full-DDM row starts with the drift draw, the walk's draws follow it; a two-stage
+            // row holds the high walk's draws, then the low walk's
+            constexpr int off = MODEL == SSM_MODEL_FULL_DDM ? 1 : 0;
+            const float* rb = A.replay_normals + g * A.replay_stride;
+            const int64_t d0 = kTwoStage<MODEL> ? (int64_t)nh + (ix - ix0) : (int64_t)(off + ix);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            if (alive) {
-              const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-              if (inside<MODEL>(x[0], b) && (t <= dl_tmp)) {
-                x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
-                t += dt;
-                ++ix;
-              } else {
-                alive = false;
-              }
-            }
+            for (int i = 0; i < 4; ++i) nz[i] = rb[(d0 + i) % A.replay_stride];
+          } else if constexpr (MODEL == SSM_MODEL_LEVY) {
+            // two Philox blocks per four steps: (U, W) pairs for four stable variates
+            const u32x4 r0 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u}, A.k0, A.k1);
+            const u32x4 r1 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 2u}, A.k0, A.k1);
+            nz[0] = stable_cms(r0.x, r0.y, alpha, alpha_inv, alpha_scale);
+            nz[1] = stable_cms(r0.z, r0.w, alpha, alpha_inv, alpha_scale);
+            nz[2] = stable_cms(r1.x, r1.y, alpha, alpha_inv, alpha_scale);
+            nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
+          } else if constexpr (kTwoStage<MODEL>) {
+            // each stage's draws start block-aligned in a stream of their own (0 high, 4 low)
+            philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ix - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
+          } else {
+            philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
           }
-        } else {
+          // Steps use exec-mask branches rather than selects: with every lane usually on the
+          // same side, a branch costs scalar instructions while a select costs VALU issue slots.
+          if (__ballot(ix + 4 > A.n_steps) == 0) {
+            // common case: no lane can reach the end of t_s in this block, no index guard needed
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            if (alive) {
-              bool go = ix < A.n_steps;  // never read past a boundary table
-              if (go) {
+            for (int i = 0; i < 4; ++i) {
+              if (alive) {
                 const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-                go = inside<MODEL>(x[0], b) && (t <= dl_tmp);
+                if (inside<MODEL>(x[0], b) && (t <= dl_tmp)) {
+                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  t += dt;
+                  ++ix;
+                } else {
+                  alive = false;
+                }
               }
-              if (go) {
-                x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
-                t += dt;
-                ++ix;
-              } else {
-                alive = false;
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (alive) {
+                bool go = ix < A.n_steps;  // never read past a boundary table
+                if (go) {
+                  const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+                  go = inside<MODEL>(x[0], b) && (t <= dl_tmp);
+                }
+                if (go) {
+                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  t += dt;
+                  ++ix;
+                } else {
+                  alive = false;
+                }
               }
             }
           }
         }
+      }
+      if (state == kRunning) {
         if (!alive) {
           if constexpr (kTwoStage<MODEL>) {
             if (stage == 0) {
@@ -525,82 +539,82 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           }
           if (!alive) state = kDone;
         }
+      }
+    } else if (state == kRunning) {
+      bool go = ix < A.n_steps && t <= dl_tmp;
+      if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
+        const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+#pragma unroll
+        for (int j = 0; j < MAXP; ++j)
+          if (j < P && x[j] > b) go = false;  // check_finished, race_models.pyx:62-77
+      }
+      if (!go) {
+        state = kDone;
       } else {
-        bool go = ix < A.n_steps && t <= dl_tmp;
-        if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
-          const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+        float nz[MAXP];
+        if constexpr (REPLAY) {
+          const float* rb = A.replay_normals + g * A.replay_stride;
+          const int64_t d0 = (int64_t)ix * P;
 #pragma unroll
           for (int j = 0; j < MAXP; ++j)
-            if (j < P && x[j] > b) go = false;  // check_finished, race_models.pyx:62-77
-        }
-        if (!go) {
-          state = kDone;
+            nz[j] = (j < P) ? rb[(d0 + j) % A.replay_stride] : 0.f;
         } else {
-          float nz[MAXP];
-          if constexpr (REPLAY) {
-            const float* rb = A.replay_normals + g * A.replay_stride;
-            const int64_t d0 = (int64_t)ix * P;
 #pragma unroll
-            for (int j = 0; j < MAXP; ++j)
-              nz[j] = (j < P) ? rb[(d0 + j) % A.replay_stride] : 0.f;
-          } else {
+          for (int q = 0; q < (MAXP + 3) / 4; ++q) {
+            if (4 * q < P) {
+              float tmp[4];
+              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)ix, (uint32_t)(2 * q), A.k0, A.k1, tmp);
 #pragma unroll
-            for (int q = 0; q < (MAXP + 3) / 4; ++q) {
-              if (4 * q < P) {
-                float tmp[4];
-                philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)ix, (uint32_t)(2 * q), A.k0, A.k1, tmp);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                  if (4 * q + i < MAXP) nz[4 * q + i] = tmp[i];
-              }
+              for (int i = 0; i < 4; ++i)
+                if (4 * q + i < MAXP) nz[4 * q + i] = tmp[i];
             }
           }
-          if constexpr (MODEL == SSM_MODEL_RDM) {
-          } else if constexpr (MODEL == SSM_MODEL_RACE) {
+        }
+        if constexpr (MODEL == SSM_MODEL_RDM) {
+        } else if constexpr (MODEL == SSM_MODEL_RACE) {
 #pragma unroll
-            for (int j = 0; j < MAXP; ++j) {
-              if (j < P) {
-                const float xn = x[j] + (vdt[j] + sst[j] * nz[j]);  // race_models.pyx:441
-                x[j] = fmaxf(0.0f, xn);
-              }
+          for (int j = 0; j < MAXP; ++j) {
+            if (j < P) {
+              const float xn = x[j] + (vdt[j] + sst[j] * nz[j]);  // race_models.pyx:441
+              x[j] = fmaxf(0.0f, xn);
             }
-          } else {
-            float sum = 0.f;  // csum, _utils.pyx:486-502 (sequential float32 sum)
+          }
+        } else {
+          float sum = 0.f;  // csum, _utils.pyx:486-502 (sequential float32 sum)
 #pragma unroll
-            for (int j = 0; j < MAXP; ++j)
-              if (j < P) sum += x[j];
-            float xnew[MAXP];
+          for (int j = 0; j < MAXP; ++j)
+            if (j < P) sum += x[j];
+          float xnew[MAXP];
 #pragma unroll
-            for (int j = 0; j < MAXP; ++j) {
-              if (j < P) {
-                const float red = (-x[j]) + sum;  // race_models.pyx:851
-                const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
-                xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
-              } else {
-                xnew[j] = x[j];
-              }
+          for (int j = 0; j < MAXP; ++j) {
+            if (j < P) {
+              const float red = (-x[j]) + sum;  // race_models.pyx:851
+              const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
+              xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
+            } else {
+              xnew[j] = x[j];
             }
-#pragma unroll
-            for (int j = 0; j < MAXP; ++j) x[j] = xnew[j];
           }
-          if constexpr (MODEL == SSM_MODEL_RDM) {
-            // race_models.pyx:1221-1237: accumulators move in order and the first to reach
-            // b ends the walk before the rest move and before t advances; the later ones'
-            // positions are never read, so all move here and the first crossing wins
-            const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
 #pragma unroll
-            for (int j = MAXP - 1; j >= 0; --j) {
-              if (j < P) {
-                x[j] = x[j] + (vdt[j] + sst[j] * nz[j]);
-                if (x[j] >= b) win = j;
-              }
+          for (int j = 0; j < MAXP; ++j) x[j] = xnew[j];
+        }
+        if constexpr (MODEL == SSM_MODEL_RDM) {
+          // race_models.pyx:1221-1237: accumulators move in order and the first to reach
+          // b ends the walk before the rest move and before t advances; the later ones'
+          // positions are never read, so all move here and the first crossing wins
+          const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+#pragma unroll
+          for (int j = MAXP - 1; j >= 0; --j) {
+            if (j < P) {
+              x[j] = x[j] + (vdt[j] + sst[j] * nz[j]);
+              if (x[j] >= b) win = j;
             }
-            if (win >= 0) state = kDone;
           }
-          if (MODEL != SSM_MODEL_RDM || win < 0) {
-            t += dt;
-            ++ix;
-          }
+          if (win >= 0) state = kDone;
+        }
+        if (MODEL != SSM_MODEL_RDM || win < 0) {
+          t += dt;
+          ++ix;
         }
       }
     }
