@@ -116,7 +116,9 @@ constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
 
 // blocks of four diffusion steps per iteration of the walk loop (the loop's own work, the
 // state ballots and the refill test, is paid once per iteration)
-constexpr int kSubBlocks = 2;
+constexpr int kSubBlocks = 8;
+// steps per walk-loop iteration of the accumulator models (one Philox block per step)
+constexpr int kAccSteps = 4;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -540,81 +542,91 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           if (!alive) state = kDone;
         }
       }
-    } else if (state == kRunning) {
-      bool go = ix < A.n_steps && t <= dl_tmp;
-      if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
-        const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+    } else {
+      // up to kAccSteps steps per loop iteration (the loop's own work is paid once for them)
 #pragma unroll
-        for (int j = 0; j < MAXP; ++j)
-          if (j < P && x[j] > b) go = false;  // check_finished, race_models.pyx:62-77
-      }
-      if (!go) {
-        state = kDone;
-      } else {
-        float nz[MAXP];
-        if constexpr (REPLAY) {
-          const float* rb = A.replay_normals + g * A.replay_stride;
-          const int64_t d0 = (int64_t)ix * P;
-#pragma unroll
-          for (int j = 0; j < MAXP; ++j)
-            nz[j] = (j < P) ? rb[(d0 + j) % A.replay_stride] : 0.f;
-        } else {
-#pragma unroll
-          for (int q = 0; q < (MAXP + 3) / 4; ++q) {
-            if (4 * q < P) {
-              float tmp[4];
-              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)ix, (uint32_t)(2 * q), A.k0, A.k1, tmp);
-#pragma unroll
-              for (int i = 0; i < 4; ++i)
-                if (4 * q + i < MAXP) nz[4 * q + i] = tmp[i];
-            }
-          }
+      for (int sb = 0; sb < kAccSteps; ++sb) {
+        if (sb > 0) {
+          if (__ballot(state == kRunning) == 0) break;  // wave-uniform: no lane left to step
+          ++iters;
         }
-        if constexpr (MODEL == SSM_MODEL_RDM) {
-        } else if constexpr (MODEL == SSM_MODEL_RACE) {
+        if (state == kRunning) {
+          bool go = ix < A.n_steps && t <= dl_tmp;
+          if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
+            const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
 #pragma unroll
-          for (int j = 0; j < MAXP; ++j) {
-            if (j < P) {
-              const float xn = x[j] + (vdt[j] + sst[j] * nz[j]);  // race_models.pyx:441
-              x[j] = fmaxf(0.0f, xn);
-            }
+            for (int j = 0; j < MAXP; ++j)
+              if (j < P && x[j] > b) go = false;  // check_finished, race_models.pyx:62-77
           }
-        } else {
-          float sum = 0.f;  // csum, _utils.pyx:486-502 (sequential float32 sum)
+          if (!go) {
+            state = kDone;
+          } else {
+            float nz[MAXP];
+            if constexpr (REPLAY) {
+              const float* rb = A.replay_normals + g * A.replay_stride;
+              const int64_t d0 = (int64_t)ix * P;
 #pragma unroll
-          for (int j = 0; j < MAXP; ++j)
-            if (j < P) sum += x[j];
-          float xnew[MAXP];
-#pragma unroll
-          for (int j = 0; j < MAXP; ++j) {
-            if (j < P) {
-              const float red = (-x[j]) + sum;  // race_models.pyx:851
-              const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
-              xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
+              for (int j = 0; j < MAXP; ++j)
+                nz[j] = (j < P) ? rb[(d0 + j) % A.replay_stride] : 0.f;
             } else {
-              xnew[j] = x[j];
+#pragma unroll
+              for (int q = 0; q < (MAXP + 3) / 4; ++q) {
+                if (4 * q < P) {
+                  float tmp[4];
+                  philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)ix, (uint32_t)(2 * q), A.k0, A.k1, tmp);
+#pragma unroll
+                  for (int i = 0; i < 4; ++i)
+                    if (4 * q + i < MAXP) nz[4 * q + i] = tmp[i];
+                }
+              }
+            }
+            if constexpr (MODEL == SSM_MODEL_RDM) {
+            } else if constexpr (MODEL == SSM_MODEL_RACE) {
+#pragma unroll
+              for (int j = 0; j < MAXP; ++j) {
+                if (j < P) {
+                  const float xn = x[j] + (vdt[j] + sst[j] * nz[j]);  // race_models.pyx:441
+                  x[j] = fmaxf(0.0f, xn);
+                }
+              }
+            } else {
+              float sum = 0.f;  // csum, _utils.pyx:486-502 (sequential float32 sum)
+#pragma unroll
+              for (int j = 0; j < MAXP; ++j)
+                if (j < P) sum += x[j];
+              float xnew[MAXP];
+#pragma unroll
+              for (int j = 0; j < MAXP; ++j) {
+                if (j < P) {
+                  const float red = (-x[j]) + sum;  // race_models.pyx:851
+                  const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
+                  xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
+                } else {
+                  xnew[j] = x[j];
+                }
+              }
+#pragma unroll
+              for (int j = 0; j < MAXP; ++j) x[j] = xnew[j];
+            }
+            if constexpr (MODEL == SSM_MODEL_RDM) {
+              // race_models.pyx:1221-1237: accumulators move in order and the first to reach
+              // b ends the walk before the rest move and before t advances; the later ones'
+              // positions are never read, so all move here and the first crossing wins
+              const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
+#pragma unroll
+              for (int j = MAXP - 1; j >= 0; --j) {
+                if (j < P) {
+                  x[j] = x[j] + (vdt[j] + sst[j] * nz[j]);
+                  if (x[j] >= b) win = j;
+                }
+              }
+              if (win >= 0) state = kDone;
+            }
+            if (MODEL != SSM_MODEL_RDM || win < 0) {
+              t += dt;
+              ++ix;
             }
           }
-#pragma unroll
-          for (int j = 0; j < MAXP; ++j) x[j] = xnew[j];
-        }
-        if constexpr (MODEL == SSM_MODEL_RDM) {
-          // race_models.pyx:1221-1237: accumulators move in order and the first to reach
-          // b ends the walk before the rest move and before t advances; the later ones'
-          // positions are never read, so all move here and the first crossing wins
-          const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
-#pragma unroll
-          for (int j = MAXP - 1; j >= 0; --j) {
-            if (j < P) {
-              x[j] = x[j] + (vdt[j] + sst[j] * nz[j]);
-              if (x[j] >= b) win = j;
-            }
-          }
-          if (win >= 0) state = kDone;
-        }
-        if (MODEL != SSM_MODEL_RDM || win < 0) {
-          t += dt;
-          ++ix;
         }
       }
     }
