@@ -114,11 +114,15 @@ __device__ __forceinline__ bool inside(float y, float b) {
 template <int MODEL>
 constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
 
-// blocks of four diffusion steps per iteration of the walk loop (the loop's own work, the
-// state ballots and the refill test, is paid once per iteration)
-constexpr int kSubBlocks = 8;
-// steps per walk-loop iteration of the accumulator models (one Philox block per step)
-constexpr int kAccSteps = 4;
+// Work per iteration of the walk loop, whose own work (the state ballots and the refill
+// test) is then paid once for it: blocks of four steps for the diffusions, single steps
+// (one Philox block each) for up to 4 accumulators. More accumulators keep one step per
+// iteration (two would cost them a wave per SIMD in registers), and replay mode keeps
+// few, since its gathers would otherwise multiply the code size.
+template <bool REPLAY>
+constexpr int kSubBlocks = REPLAY ? 2 : 8;
+template <bool REPLAY, int MAXP>
+constexpr int kAccSteps = (REPLAY || MAXP > 4) ? 1 : 8;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
       // every walk starts block-aligned, so the draws do not depend on the grouping
       bool alive = state == kRunning;
 #pragma unroll
-      for (int sb = 0; sb < kSubBlocks; ++sb) {
+      for (int sb = 0; sb < kSubBlocks<REPLAY>; ++sb) {
         if (sb > 0) {
           if (__ballot(alive) == 0) break;  // wave-uniform: no lane left to step
           ++iters;
@@ -545,7 +549,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     } else {
       // up to kAccSteps steps per loop iteration (the loop's own work is paid once for them)
 #pragma unroll
-      for (int sb = 0; sb < kAccSteps; ++sb) {
+      for (int sb = 0; sb < kAccSteps<REPLAY, MAXP>; ++sb) {
         if (sb > 0) {
           if (__ballot(state == kRunning) == 0) break;  // wave-uniform: no lane left to step
           ++iters;
