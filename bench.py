@@ -242,8 +242,15 @@ def lan_leg(args):
            "kde_pairs_per_s": pairs / (st.eval_ms / 1e3)}
     if not args.no_probe:
         roof = roofline.probe_rate("kde_pair", iters=2048)
+        exp_rate = roofline.probe_rate("exp_f32", iters=2048)
         rec["kde_eval_roofline"] = {"bound": "valu (exp)", "achieved": rec["kde_pairs_per_s"] / 1e12,
-                                    "peak": roof / 1e12, "unit": "T pairs/s", "frac": rec["kde_pairs_per_s"] / roof}
+                                    "peak": roof / 1e12, "unit": "T pairs/s", "frac": rec["kde_pairs_per_s"] / roof,
+                                    # one v_exp_f32 per pair: the transcendental rate against the
+                                    # probe-measured single-instruction v_exp_f32 rate and the chip's
+                                    # published FP32 FMA rate (157.3 TFLOP/s / 2)
+                                    "exp_per_s": rec["kde_pairs_per_s"], "exp_probe_per_s": exp_rate,
+                                    "exp_frac_of_probe": rec["kde_pairs_per_s"] / exp_rate,
+                                    "exp_frac_of_fp32_fma_peak": rec["kde_pairs_per_s"] / roofline.PEAK_VALU_LANE_OPS}
     if not args.no_cpu:
         # CPU: per set, simulate 1e5 (MT19937 + Ziggurat, all cores) + exact float64 KDE at the same 1000 points
         pts_rt = lab.points.rt.cpu().numpy().reshape(1024, -1)

@@ -346,6 +346,24 @@ constexpr int kTile = 2048;
 // (<= 512, <= 256) splits the block's threads into 2 or 4 teams that cover the same
 // points against different quarters/halves of each sample tile, so every lane holds
 // live points; the teams' sums are added in LDS in a fixed order.
+// NQ of a thread's kPtsPerThread points against the samples [lo, hi) of the LDS tile, four
+// samples per step in tile order (the per-point summation order does not depend on NQ)
+template <int NQ>
+__device__ __forceinline__ void kde_pairs(const float* tile, int lo, int hi, const float (&x)[kPtsPerThread],
+                                          float (&acc)[kPtsPerThread]) {
+  for (int i = lo; i < hi; i += 4) {
+    const float4 s4 = *reinterpret_cast<const float4*>(&tile[i]);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      float d;
+      d = x[q] - s4.x; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+      d = x[q] - s4.y; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+      d = x[q] - s4.z; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+      d = x[q] - s4.w; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs A) {
   __shared__ __attribute__((aligned(16))) float tile[kTile];
   __shared__ float red[3 * kEvalThreads];
@@ -385,16 +403,19 @@ __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs
       const int tn4 = (tn + 3) & ~3;
       const int lo = team * seg;
       const int hi = lo + seg < tn4 ? lo + seg : tn4;
-      for (int i = lo; i < hi; i += 4) {
-        const float4 s4 = *reinterpret_cast<const float4*>(&tile[i]);
-#pragma unroll
-        for (int q = 0; q < kPtsPerThread; ++q) {
-          float d;
-          d = x[q] - s4.x; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
-          d = x[q] - s4.y; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
-          d = x[q] - s4.z; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
-          d = x[q] - s4.w; acc[q] += __builtin_amdgcn_exp2f(__builtin_fmaf(-d, d, kScaleLog2));
-        }
+      // point row q of this wave holds points pbase + q tpt + (wave's lanes): rows lying wholly
+      // past the group's last point are skipped (wave-uniform), so a pass that is only partly
+      // filled costs at most 63 idle lanes per team instead of up to 3/4 of its point slots
+      const int first = pbase + ((lt >> 6) << 6);  // this wave's lane-0 point of row 0
+      int nq = first < cnt ? (cnt - first + tpt - 1) / tpt : 0;
+      nq = nq < kPtsPerThread ? nq : kPtsPerThread;
+      static_assert(kPtsPerThread == 4, "kde_pairs dispatch below covers 1..4 rows");
+      switch (nq) {
+        case 4: kde_pairs<4>(tile, lo, hi, x, acc); break;
+        case 3: kde_pairs<3>(tile, lo, hi, x, acc); break;
+        case 2: kde_pairs<2>(tile, lo, hi, x, acc); break;
+        case 1: kde_pairs<1>(tile, lo, hi, x, acc); break;
+        default: break;
       }
     }
     if (teams > 1) {

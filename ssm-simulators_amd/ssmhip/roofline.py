@@ -16,8 +16,10 @@ KINDS = {"em_block": 0, "kde_pair": 1, "mad_u64_u32": 2, "xor_b32": 3, "fma_f32"
 # work units per lane-iteration: EM steps, KDE pairs, or instructions
 UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8, 9: 8}
 
-# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz: full-rate 32-bit VALU lane-ops/s
-PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# MI355X's published FP32 vector peak, 157.3 TFLOP/s, as FMA lane-operations/s (an FMA is
+# 2 FLOP); the EM kernel was measured issuing 48e12 lane-level VALU instructions/s, so a
+# 16-lane-per-SIMD reading of the chip (39.3e12/s) would understate it
+PEAK_VALU_LANE_OPS = 157.3e12 / 2
 
 
 def probe_rate(kind: str | int, iters: int = 4096, waves_per_simd: int = 8, reps: int = 3) -> float:
@@ -42,7 +44,8 @@ def probe_rate(kind: str | int, iters: int = 4096, waves_per_simd: int = 8, reps
 
 
 def instruction_table(iters: int = 2048) -> dict:
-    """Measured lane-instructions/s per probed instruction and its cost in full-rate slots."""
+    """Measured lane-instructions/s per probed instruction and its cost in slots of the chip's
+    published FP32 FMA rate."""
     out = {}
     for name in ("mad_u64_u32", "xor_b32", "fma_f32", "log_f32", "exp_f32", "sin_f32", "sqrt_f32"):
         r = probe_rate(name, iters)
