@@ -72,20 +72,42 @@ def _ddm_like(name, params, bounds, defaults, bname, simulator=None):
     }
 
 
-def _race(n, lca=False):
+def _race_z(n, zs, variant):
+    """z handling of the race/LCA variants: one z per accumulator (ColumnStackParameters),
+    one shared z duplicated n times (no_bias), or zeros (no_z; SetZeroArray)."""
+    def f(theta, cfg, n_trials):
+        if variant == "":
+            theta["z"] = np.column_stack([theta[s] for s in zs]).astype(np.float32)
+        elif variant == "no_bias":
+            theta["z"] = np.column_stack([theta["z"]] * n).astype(np.float32)
+        else:
+            theta["z"] = np.zeros((n_trials, n), dtype=np.float32)
+        return theta
+
+    return f
+
+
+def _race(n, lca=False, variant="", angle=False):
+    """race_{n} / lca_{n} and their no_bias / no_z / *_angle variants (race.py:14-480,
+    lca.py:15-330): same simulator, z from one column per accumulator, one shared
+    column or zeros, and the angle boundary (theta in [-0.1, 1.45])."""
     vs = [f"v{i}" for i in range(n)]
-    zs = [f"z{i}" for i in range(n)]
+    zs = {"": [f"z{i}" for i in range(n)], "no_bias": ["z"], "no_z": []}[variant]
     extra = ["g", "b"] if lca else []
-    params = vs + ["a"] + zs + extra + ["t"]
-    lo = [0.0] * n + [1.0] + [0.0] * n + ([-1.0, -1.0] if lca else []) + [0.0]
-    hi = [2.5] * n + [3.0] + [0.9] * n + ([1.0, 1.0] if lca else []) + [2.0]
-    dflt = [0.0] * n + [2.0] + [0.5] * n + ([0.0, 0.0] if lca else []) + [1e-3]
+    params = vs + ["a"] + zs + extra + ["t"] + (["theta"] if angle else [])
+    # lca.py gives theta a lower bound of -1.0 in its three-accumulator angle variants
+    theta_lo = -1.0 if (lca and angle and n == 3) else -0.1
+    lo = [0.0] * n + [1.0] + [0.0] * len(zs) + ([-1.0, -1.0] if lca else []) + [0.0] + ([theta_lo] if angle else [])
+    hi = [2.5] * n + [3.0] + [0.9] * len(zs) + ([1.0, 1.0] if lca else []) + [2.0] + ([1.45] if angle else [])
+    dflt = [0.0] * n + [2.0] + [0.5] * len(zs) + ([0.0, 0.0] if lca else []) + [1e-3] + ([0.0] if angle else [])
+    tag = (variant + "_" if variant else "") + ("angle_" if angle else "")
+    bname = "angle" if angle else "constant"
     return {
-        "name": f"{'lca' if lca else 'race'}_{n}", "params": params, "param_bounds": [lo, hi],
-        "boundary_name": "constant", "boundary": bf.constant, "n_params": len(params), "default_params": dflt,
-        "nchoices": n, "choices": list(range(n)), "n_particles": n,
+        "name": f"{'lca' if lca else 'race'}_{tag}{n}", "params": params, "param_bounds": [lo, hi],
+        "boundary_name": bname, "boundary": bf.angle if angle else bf.constant, "n_params": len(params),
+        "default_params": dflt, "nchoices": n, "choices": list(range(n)), "n_particles": n,
         "simulator": cssm.lca if lca else cssm.race_model,
-        "simulation_transforms": [column_stack(zs, "z"), column_stack(vs, "v"),
+        "simulation_transforms": [_race_z(n, zs, variant), column_stack(vs, "v"),
                                   expand_dims(["t", "a"] + (["g", "b"] if lca else []))],
     }
 
@@ -176,9 +198,6 @@ def _registry() -> dict:
             "n_particles": 3, "simulator": cssm.racing_diffusion_model,
             "simulation_transforms": [column_stack(["v0", "v1", "v2"], "v"), expand_dims(["t", "A", "b"])],
         },
-        # race.py:14-352, lca.py:15-112
-        "race_2": _race(2), "race_3": _race(3), "race_4": _race(4),
-        "lca_3": _race(3, lca=True), "lca_4": _race(4, lca=True),
         # lba.py:94-222
         "lba2": _lba(2, [[0.0, 0.0, 0.0, 0.1], [1.0, 1.0, 1.0, 1.1]], [0.3, 0.5, 0.5, 0.5]),
         "lba3": _lba(3, [[0.0, 0.0, 0.0, 0.1, 0.1], [1.0, 1.0, 1.0, 1.1, 0.50]], [0.3, 0.5, 0.25, 0.5, 0.25]),
@@ -192,6 +211,18 @@ def _registry() -> dict:
             "n_particles": 3, "simulator": cssm.lba_angle, "simulation_transforms": _lba_angle_3_transforms(),
         },
     }
+    # race.py:14-480 (2-4 accumulators), lca.py:15-330 (3-4): five variants each
+    for variant, angle in (("", False), ("no_bias", False), ("no_z", False), ("no_bias", True), ("no_z", True)):
+        for n in (2, 3, 4):
+            cfg = _race(n, False, variant, angle)
+            reg[cfg["name"]] = cfg
+        for n in (3, 4):
+            cfg = _race(n, True, variant, angle)
+            reg[cfg["name"]] = cfg
+    # _modelconfig/__init__.py registers the weibull and full-DDM configurations under
+    # a second name as well
+    reg["weibull_cdf"] = reg["weibull"]
+    reg["full_ddm2"] = reg["full_ddm"]
     # ddm_par2.py:10-185, ddm_seq2.py:11-190
     for kind in ("par2", "seq2"):
         for variant in ("", "_no_bias", "_conflict_gamma_no_bias", "_angle_no_bias", "_weibull_no_bias"):

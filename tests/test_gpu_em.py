@@ -122,7 +122,7 @@ def test_ddm_weibull_in_register_vs_table(torch_gpu):
 
 
 @pytest.mark.parametrize("model", ["race", "lca"])
-@pytest.mark.parametrize("bname", ["constant", "weibull_cdf"])
+@pytest.mark.parametrize("bname", ["constant", "weibull_cdf", "angle"])
 def test_accumulator_replay_matches_oracle(torch_gpu, model, bname):
     import torch
 
@@ -138,7 +138,8 @@ def test_accumulator_replay_matches_oracle(torch_gpu, model, bname):
     t = rng.uniform(0, 2, K).astype(F32)
     g = rng.uniform(-1, 1, K).astype(F32)
     b = rng.uniform(-1, 1, K).astype(F32)
-    bp = {"a": a, "alpha": rng.uniform(0.3, 5, K).astype(F32), "beta": rng.uniform(0.3, 7, K).astype(F32)}
+    bp = {"a": a, "alpha": rng.uniform(0.3, 5, K).astype(F32), "beta": rng.uniform(0.3, 7, K).astype(F32),
+          "theta": rng.uniform(-0.1, 1.45, K).astype(F32)}
     bp = {k: bp[k] for k in E.BOUNDARIES[bname][1]}
     kw = dict(g=g, b=b) if model == "lca" else {}
     normals = rng.standard_normal((K * N, L_)).astype(F32)
@@ -146,12 +147,13 @@ def test_accumulator_replay_matches_oracle(torch_gpu, model, bname):
     r0, c0, s0 = E.accumulators(model, v, z, t, np.ones((K, 1), F32), np.full(K, 999, F32), n_samples=N,
                                 delta_t=DT, boundary=bname, bparams=bp, smooth_unif=True, normals=normals,
                                 uniforms=unif, **kw)
-    fun = {"constant": BF.constant, "weibull_cdf": BF.weibull_cdf}[bname]
-    rec = em.accumulator_records(v, z, t, np.ones((K, 1), F32), np.full(K, 999, F32), 20.0, DT, L.BOUNDARY_TABLE,
-                                 bp, **kw)
-    table = em.boundary_table(fun, bp, K, 20.0, DT)
+    fun = {"constant": BF.constant, "weibull_cdf": BF.weibull_cdf, "angle": BF.angle}[bname]
+    # constant and Weibull read the boundary table; the angle bound is evaluated in registers
+    code = L.BOUNDARY_ANGLE if bname == "angle" else L.BOUNDARY_TABLE
+    rec = em.accumulator_records(v, z, t, np.ones((K, 1), F32), np.full(K, 999, F32), 20.0, DT, code, bp, **kw)
+    table = em.boundary_table(fun, bp, K, 20.0, DT) if code == L.BOUNDARY_TABLE else None
     mdl = L.MODEL_RACE if model == "race" else L.MODEL_LCA
-    job = em.make_job(mdl, L.BOUNDARY_TABLE, P, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=1,
+    job = em.make_job(mdl, code, P, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=1,
                       smooth_unif=True, table=table, replay_normals=normals, replay_uniforms=unif)
     job.launch()
     torch.cuda.synchronize()

@@ -191,3 +191,69 @@ def test_legacy_ddm_through_the_simulator(torch_gpu):
     assert set(np.unique(out["choices"])) <= {-1, 1} and np.all(out["rts"] > 0.3)
     base = Simulator("ddm").simulate({"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.3}, n_samples=4000, random_state=0)
     assert np.mean(out["rts"]) < np.mean(base["rts"])  # bounds a apart instead of 2a
+
+
+# One model per simulator family, with two parameter rows that differ only in boundary
+# parameters (the reference's tests/test_metadata_boundary_trial0.py uses the same models).
+BOUNDARY_ROWS = {
+    "angle": {"theta": (0.0, 1.2)},
+    "weibull_cdf": {"alpha": (3.0, 1.0), "beta": (3.0, 1.0)},
+    "ddm_seq2_angle_no_bias": {"theta": (0.0, 1.0)},
+    "race_no_bias_angle_4": {"theta": (0.0, 1.0)},
+    "levy_angle": {"theta": (0.0, 1.0)},
+    "ornstein_angle": {"theta": (0.0, 1.0)},
+    "ddm_par2_angle_no_bias": {"theta": (0.0, 1.0)},
+}
+
+
+@pytest.mark.parametrize("model", list(BOUNDARY_ROWS))
+def test_full_metadata_boundary_describes_trial_0(torch_gpu, model):
+    """metadata['boundary'] is the boundary of the first parameter row, whatever the other
+    rows hold and whatever n_threads says, and equals the boundary function on the time grid."""
+    from ssmhip import config
+    from ssmhip.boundary_functions import time_grid
+    from ssmhip.simulator import simulator
+
+    print(f"[case] metadata boundary {model}", flush=True)
+    cfg = config.get(model)
+    row0 = dict(zip(cfg["params"], cfg["default_params"]))
+    row1 = dict(row0)
+    for p, (x0, x1) in BOUNDARY_ROWS[model].items():
+        row0[p], row1[p] = x0, x1
+
+    def bnd(rows, n_threads=1):
+        theta = {k: np.array([r[k] for r in rows], dtype=np.float32) for k in rows[0]}
+        out = simulator(theta, model=model, n_samples=2, random_state=42, n_threads=n_threads)
+        return np.asarray(out["metadata"]["boundary"])
+
+    b0, b1 = bnd([row0]), bnd([row1])
+    assert not np.allclose(b0, b1)
+    np.testing.assert_allclose(bnd([row0, row1]), b0, rtol=1e-6)
+    np.testing.assert_allclose(bnd([row1, row0]), b1, rtol=1e-6)
+    np.testing.assert_allclose(bnd([row0, row1], n_threads=4), b0, rtol=1e-6)
+    names = cfg["boundary_params"]
+    expect = cfg["boundary"](t=time_grid(20.0, 0.001), **{k: np.float32(row0[k]) for k in names})
+    np.testing.assert_allclose(b0, np.asarray(expect, np.float32), rtol=1e-6, atol=1e-7)
+
+
+def test_race_and_lca_variants_through_the_simulator(torch_gpu):
+    """Every race/LCA variant configuration runs at its default parameters; an angle
+    bound that collapses faster ends the race sooner."""
+    from ssmhip import config
+    from ssmhip.simulator import Simulator
+
+    for name in [m for m in config.list_models() if m.startswith(("race_", "lca_"))]:
+        print(f"[case] {name}", flush=True)
+        cfg = config.get(name)
+        theta = dict(zip(cfg["params"], cfg["default_params"]))
+        theta.update({f"v{i}": 1.0 for i in range(cfg["n_particles"])})
+        out = Simulator(name).simulate(theta, n_samples=500, random_state=1)
+        assert out["rts"].shape == (500, 1) and out["choices"].dtype == np.intc
+        assert set(np.unique(out["choices"])) <= set(cfg["choices"])
+        assert np.all(out["rts"] > 0) and out["metadata"]["possible_choices"] == cfg["choices"]
+    base = {"v0": 1.0, "v1": 0.8, "v2": 0.6, "v3": 0.4, "a": 2.0, "z": 0.2, "t": 0.1}
+    slow = Simulator("race_no_bias_angle_4").simulate({**base, "theta": 0.0}, n_samples=4000, random_state=2)
+    fast = Simulator("race_no_bias_angle_4").simulate({**base, "theta": 1.0}, n_samples=4000, random_state=2)
+    assert fast["rts"].mean() < slow["rts"].mean() - 0.2
+    const = Simulator("race_no_bias_4").simulate(base, n_samples=4000, random_state=2)
+    assert abs(const["rts"].mean() - slow["rts"].mean()) < 0.05

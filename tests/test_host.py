@@ -68,6 +68,50 @@ def test_model_configs_and_deadline_variant():
         config.get("nonexistent_model")
 
 
+def test_race_and_lca_variant_configs():
+    """race.py / lca.py variants: z per accumulator, one shared z (no_bias) or zeros (no_z),
+    constant or angle boundary; records carry the trial's own a and the angle slope."""
+    from ssmhip import config, em
+    from ssmhip import _lib as L
+    from ssmhip.boundary_functions import kernel_boundary_code
+    from ssmhip.simulator import make_boundary_dict
+
+    names = [m for m in config.list_models() if m.startswith(("race_", "lca_"))]
+    assert len(names) == 25
+    rng = np.random.default_rng(3)
+    for name in names:
+        c = config.get(name)
+        n = c["n_particles"]
+        assert c["n_params"] == len(c["params"]) == len(c["param_bounds"][0]) == len(c["default_params"])
+        assert c["choices"] == list(range(n)) and c["boundary_name"] == ("angle" if "angle" in name else "constant")
+        lo, hi = (np.asarray(b, F32) for b in c["param_bounds"])
+        rows = lo + (hi - lo) * rng.uniform(size=(3, lo.size)).astype(F32)
+        theta = {p: rows[:, j].copy() for j, p in enumerate(c["params"])}
+        theta["s"] = np.ones((3, n), F32)
+        theta["deadline"] = np.full(3, 999, F32)
+        for tr in c["simulation_transforms"]:
+            theta = tr(theta, c, 3)
+        assert theta["z"].shape == (3, n) and theta["v"].shape == (3, n)
+        if "no_z" in name:
+            assert np.all(theta["z"] == 0)
+        elif "no_bias" in name:
+            assert np.all(theta["z"] == rows[:, c["params"].index("z")][:, None])
+        else:
+            assert np.array_equal(theta["z"], rows[:, [c["params"].index(f"z{i}") for i in range(n)]])
+        bd = make_boundary_dict(c, theta)
+        code = kernel_boundary_code(bd["boundary_fun"])
+        kw = {"g": theta["g"], "b": theta["b"]} if name.startswith("lca") else {}
+        rec = em.accumulator_records(theta["v"], theta["z"], theta["t"], theta["s"], theta["deadline"], 20.0, 1e-3,
+                                     code, bd["boundary_params"], **kw)
+        assert np.array_equal(rec[:, L.P_A], rows[:, c["params"].index("a")])
+        if "angle" in name:
+            assert code == L.BOUNDARY_ANGLE
+            th = rows[:, c["params"].index("theta")]
+            assert np.allclose(rec[:, L.P_B1], -np.tan(th.astype(np.float64)), rtol=1e-5, atol=1e-6)
+    assert config.get("weibull_cdf")["params"] == config.get("weibull")["params"]
+    assert config.get("full_ddm2")["simulator"] is config.get("full_ddm")["simulator"]
+
+
 def test_batched_records_shape_checks():
     from ssmhip import batched
 
