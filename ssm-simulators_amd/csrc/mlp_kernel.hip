@@ -155,6 +155,42 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 // per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
 constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
 
+// tanh of output tile a of both row tiles; once a pair (2b, 2b+1) is done it is packed as
+// K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0.
+template <int NA>
+__device__ __forceinline__ void activate_tile(f32x4 (&acc)[2][8], bf16x8 (&Bn)[2][4], int a) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
+    if (a & 1) Bn[t][a >> 1] = pack_pair(acc[t][a - 1], acc[t][a]);
+    else if (a == NA - 1) Bn[t][a >> 1] = pack_pair(acc[t][a], f32x4{0.f, 0.f, 0.f, 0.f});
+  }
+}
+
+// One hidden layer for both row tiles: output tiles 0..NA-1 from K blocks B (KB of them).
+// The tanh of output tile a-1 is placed between the MFMAs of tile a, so within one wave
+// the transcendental work of one tile overlaps the matrix-core work of the next.
+template <int NA, int KB>
+__device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
+                                             const bf16x8 (&B)[2][4], f32x4 (&acc)[2][8], bf16x8 (&Bn)[2][4]) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const f32x4 bt = bias_tile(bias, bo, a, g);
+    f32x4 c0 = bt, c1 = bt;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+      const bf16x8 w = load_frag(frags, fo, a * KB + b, lane);
+      c0 = mfma(w, B[0][b], c0);
+      c1 = mfma(w, B[1][b], c1);
+    }
+    acc[0][a] = c0;
+    acc[1][a] = c1;
+    if (a > 0) activate_tile<NA>(acc, Bn, a - 1);
+  }
+  activate_tile<NA>(acc, Bn, NA - 1);
+}
+
 __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
                                                           const __bf16* __restrict__ gfrags,
                                                           const float* __restrict__ bias,
@@ -175,7 +211,7 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
     int lo = 0;
     asm volatile("" : "+v"(lo));
     const __bf16* frags = lfrag + lo;
-    bf16x8 B[2][4];
+    bf16x8 B[2][4], Bn[2][4];
     f32x4 acc[2][8];
     // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
 #pragma unroll
@@ -191,63 +227,15 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
     }
     // Output tile 7 of the 100-unit layers 1 and 2 (units 112..127) is all padding: its
     // weights and biases are zero, so it stays 0 and tanh(0) = 0 need not be computed.
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      if (a == 7) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) acc[t][a] = f32x4{0.f, 0.f, 0.f, 0.f};
-        continue;
-      }
-      const bf16x8 w = load_frag(frags, fF1, a, lane);
-      const f32x4 bt = bias_tile(bias, pB1, a, g);
-#pragma unroll
-      for (int t = 0; t < 2; ++t) acc[t][a] = mfma(w, B[t][0], bt);
-    }
-    // hidden layers 2 and 3
-#pragma unroll
-    for (int L = 2; L <= 3; ++L) {
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-#pragma unroll
-        for (int a = 0; a < 7; ++a)  // tile 7 of layers 1 and 2 is zero
-#pragma unroll
-          for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) B[t][b] = pack_pair(acc[t][2 * b], acc[t][2 * b + 1]);
-      }
-      const int fo = (L == 2) ? fF2 : fF3;
-      const int bo = (L == 2) ? pB2 : pB3;
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        if (L == 2 && a == 7) {
-          acc[0][a] = f32x4{0.f, 0.f, 0.f, 0.f};
-          acc[1][a] = f32x4{0.f, 0.f, 0.f, 0.f};
-          continue;
-        }
-        const f32x4 bt = bias_tile(bias, bo, a, g);
-        f32x4 c0 = bt, c1 = bt;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const bf16x8 w = load_frag(frags, fo, a * 4 + b, lane);
-          c0 = mfma(w, B[0][b], c0);
-          c1 = mfma(w, B[1][b], c1);
-        }
-        acc[0][a] = c0;
-        acc[1][a] = c1;
-      }
-    }
+    hidden_layer<7, 1>(frags, fF1, bias, pB1, lane, g, B, acc, Bn);
+    hidden_layer<7, 4>(frags, fF2, bias, pB2, lane, g, Bn, acc, B);
+    hidden_layer<8, 4>(frags, fF3, bias, pB3, lane, g, B, acc, Bn);
     // output layer
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) B[t][b] = pack_pair(acc[t][2 * b], acc[t][2 * b + 1]);
       f32x4 c = bias_tile(bias, pB4, 0, g);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), B[t][b], c);
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), Bn[t][b], c);
       const int64_t row = r0 + 16 * t + r;
       if (g == 0 && row < n) y[row] = c[0];
     }
