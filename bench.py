@@ -374,12 +374,13 @@ def surrogate_leg(args, lab, kde_evals_per_s):
     kde_at = np.empty_like(kll)
     kde_at[order] = kll
     kde_mae = float(np.mean(np.abs(kde_at - ref)))
-    # roof: the surrogate's tanh (exp2 + rcp on the transcendental unit) at full occupancy;
-    # the forward computes 352 tanh per row (112 + 112 + 128 with padding), MFMA runs beside it
-    tanh_rate = None if args.no_probe else roofline.probe_rate("tanh", iters=2048)
-    rec_roof = None if tanh_rate is None else {
-        "bound": "transcendental (tanh)", "achieved": evals / 1e9, "peak": tanh_rate / 352 / 1e9,
-        "unit": "G rows/s", "frac": evals / (tanh_rate / 352), "tanh_per_s": tanh_rate}
+    # roof: the surrogate's hidden unit u = 1 / (2^z + 1) (exp2 + rcp on the transcendental
+    # unit) at full occupancy; the forward computes 352 units per row (112 + 112 + 128 with
+    # padding), MFMA runs beside them
+    unit_rate = None if args.no_probe else roofline.probe_rate("hidden_unit", iters=2048)
+    rec_roof = None if unit_rate is None else {
+        "bound": "transcendental (hidden units)", "achieved": evals / 1e9, "peak": unit_rate / 352 / 1e9,
+        "unit": "G rows/s", "frac": evals / (unit_rate / 352), "hidden_units_per_s": unit_rate}
     rec = {"inference_roofline": rec_roof, "label_sets": args.surrogate_sets, "label_samples_per_set": args.surrogate_samples,
            "label_s": label_s, "train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
            "train_rows_per_s": train.shape[0] * args.surrogate_epochs / train_s, "huber_history": hist[-3:],

@@ -7,7 +7,10 @@ backward deltas) to bfloat16 (round to nearest even) and accumulates in
 float64, i.e. the arithmetic the MFMA kernels perform up to summation order;
 the tanh derivative 1 - h^2 then uses the bf16 activations, as the kernels do,
 and hidden-layer forward weights are rounded as bf16(2 log2(e) W) (the scale
-the kernels fold into their packed weights).
+the kernels fold into their packed weights). `carry="sigmoid"` models the
+inference kernel, which carries each hidden unit as u = (1 - tanh z) / 2 and
+folds 1 - 2u into the next layer: with bf16 rounding the next layer then sees
+1 - 2 bf16(u) in place of bf16(tanh z); in exact arithmetic the two agree.
 Pinned by: PyTorch autograd of the same nn.Sequential (float64) and central
 finite differences (tests/test_oracle_mlp.py).
 """
@@ -66,8 +69,10 @@ def standardise(x: np.ndarray, xnorm) -> np.ndarray:
     return ((x - xn[:7]).astype(np.float32) * xn[7:14]).astype(np.float32)
 
 
-def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None):
+def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None, carry: str = "tanh"):
     """Returns y [n] and the list of hidden activations."""
+    if carry not in ("tanh", "sigmoid"):
+        raise ValueError(carry)
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     h = r(standardise(x, xnorm))
     hs = [h]
@@ -83,7 +88,10 @@ def forward(p: np.ndarray, x: np.ndarray, bf16: bool = False, xnorm=None):
         if i < 3:
             a = np.tanh(z)
             hs.append(a)
-            h = r(a.astype(np.float32)) if bf16 else a
+            if bf16 and carry == "sigmoid":
+                h = 1.0 - 2.0 * r((0.5 * (1.0 - a)).astype(np.float32))
+            else:
+                h = r(a.astype(np.float32)) if bf16 else a
         else:
             return z[:, 0], hs
     raise AssertionError

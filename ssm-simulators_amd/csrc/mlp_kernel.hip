@@ -151,25 +151,34 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 // ---- forward (inference) --------------------------------------------------------------
 // x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. Two row tiles (32 rows) per wave.
 // xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
-// The forward fragments (78 KB of bf16) are staged once per block in LDS, so the
-// per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
+// The forward fragments (78 KB of bf16) and biases are staged once per block in LDS, so
+// the per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
 constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
 
-// tanh of output tile a of both row tiles; once a pair (2b, 2b+1) is done it is packed as
-// K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0.
+// Inference carries each hidden unit as u = 1 / (2^zs + 1) = (1 - tanh z) / 2, one
+// exp2, add and reciprocal, instead of tanh z = 1 - 2u (one FMA more per unit). The layer
+// that consumes the units takes the affine map: with packed weights P (bf16) it computes
+//   bias' + sum_k (-2 P_k) u_k,   bias' = bias + sum_k P_k,
+// which equals bias + sum_k P_k (1 - 2 u_k) = bias + sum_k P_k tanh z_k. -2 P is exact in
+// bf16, so the only change against carrying tanh is where bf16 rounds: u instead of tanh z.
+// The block prologue builds -2 P and bias' in LDS from the packed weights.
+
+// u of output tile a of both row tiles; once a pair (2b, 2b+1) is done it is packed as
+// K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0
+// (its weights in the next layer are zero, so any value would do).
 template <int NA>
 __device__ __forceinline__ void activate_tile(f32x4 (&acc)[2][8], bf16x8 (&Bn)[2][4], int a) {
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[t][a][i] = tanh_fast(acc[t][a][i]);
+    for (int i = 0; i < 4; ++i) acc[t][a][i] = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(acc[t][a][i]) + 1.0f);
     if (a & 1) Bn[t][a >> 1] = pack_pair(acc[t][a - 1], acc[t][a]);
     else if (a == NA - 1) Bn[t][a >> 1] = pack_pair(acc[t][a], f32x4{0.f, 0.f, 0.f, 0.f});
   }
 }
 
-// One hidden layer for both row tiles: output tiles 0..NA-1 from K blocks B (KB of them).
-// The tanh of output tile a-1 is placed between the MFMAs of tile a, so within one wave
+// One layer for both row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
+// activation of output tile a-1 is placed between the MFMAs of tile a, so within one wave
 // the transcendental work of one tile overlaps the matrix-core work of the next.
 template <int NA, int KB>
 __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
@@ -196,21 +205,51 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ xnorm, float* __restrict__ y) {
   __shared__ __attribute__((aligned(16))) __bf16 lfrag[kFwdFrag];
+  __shared__ __attribute__((aligned(16))) float lbias[kNBias];
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   {
+    // layer 1 fragments as packed; layers 2..4 (they consume u) as -2 P
     const int4* src = reinterpret_cast<const int4*>(gfrags);
     int4* dst = reinterpret_cast<int4*>(lfrag);
-    for (int i = threadIdx.x; i < kFwdFrag / 8; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kFwdFrag / 8; i += blockDim.x) {
+      if (i < fF2 / 8) {
+        dst[i] = src[i];
+      } else {
+        bf16x8 v = *reinterpret_cast<const bf16x8*>(src + i);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(-2.0f * (float)v[j]);
+        *reinterpret_cast<bf16x8*>(dst + i) = v;
+      }
+    }
+    for (int i = threadIdx.x; i < pB2; i += blockDim.x) lbias[i] = bias[i];
+    // bias' = bias + row sums of P, from MFMAs against all-ones columns: 17 output
+    // tiles (8 of layer 2, 8 of layer 3, the output tile), one per wave in turn
+    bf16x8 ones;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
+    for (int task = threadIdx.x >> 6; task < 17; task += blockDim.x >> 6) {
+      const int a = task & 7;
+      const int fo = task < 8 ? fF2 : (task < 16 ? fF3 : fF4);
+      const int bo = task < 8 ? pB2 : (task < 16 ? pB3 : pB4);
+      f32x4 c = bias_tile(bias, bo, a, g);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c = mfma(load_frag(gfrags, fo, a * 4 + b, lane), ones, c);
+      if (r == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lbias[bo + 16 * a + 4 * g + i] = c[i];
+      }
+    }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t r0 = wave * 32; r0 < n; r0 += nwaves * 32) {
-    // an offset the compiler cannot see through keeps the LDS fragment reads inside
-    // the row loop (hoisting them would need hundreds of registers)
+    // an offset the compiler cannot see through keeps the LDS fragment and bias reads
+    // inside the row loop (hoisting them would need hundreds of registers)
     int lo = 0;
     asm volatile("" : "+v"(lo));
     const __bf16* frags = lfrag + lo;
+    const float* lb = lbias + lo;
     bf16x8 B[2][4], Bn[2][4];
     f32x4 acc[2][8];
     // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
@@ -226,14 +265,15 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       B[t][0] = v;
     }
     // Output tile 7 of the 100-unit layers 1 and 2 (units 112..127) is all padding: its
-    // weights and biases are zero, so it stays 0 and tanh(0) = 0 need not be computed.
-    hidden_layer<7, 1>(frags, fF1, bias, pB1, lane, g, B, acc, Bn);
-    hidden_layer<7, 4>(frags, fF2, bias, pB2, lane, g, Bn, acc, B);
-    hidden_layer<8, 4>(frags, fF3, bias, pB3, lane, g, B, acc, Bn);
+    // weights and biases are zero and the next layer's weights for it are zero, so it
+    // need not be computed.
+    hidden_layer<7, 1>(frags, fF1, lb, pB1, lane, g, B, acc, Bn);
+    hidden_layer<7, 4>(frags, fF2, lb, pB2, lane, g, Bn, acc, B);
+    hidden_layer<8, 4>(frags, fF3, lb, pB3, lane, g, B, acc, Bn);
     // output layer
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      f32x4 c = bias_tile(bias, pB4, 0, g);
+      f32x4 c = bias_tile(lb, pB4, 0, g);
 #pragma unroll
       for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), Bn[t][b], c);
       const int64_t row = r0 + 16 * t + r;

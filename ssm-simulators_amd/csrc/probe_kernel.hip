@@ -6,7 +6,8 @@
 //   kind 2..8  single instructions, 8 independent chains per lane:
 //           2 v_mad_u64_u32, 3 v_xor_b32, 4 v_fma_f32, 5 v_log_f32,
 //           6 v_exp_f32, 7 v_sin_f32, 8 v_sqrt_f32
-//   kind 9  the surrogate's tanh: exp2, add, rcp, fma (+ one fma), 8 chains per lane
+//   kind 9  the surrogate's hidden unit u = 1 / (2^z + 1): exp2, add, rcp (+ one fma),
+//           8 chains per lane
 // Every lane writes one float so nothing is optimised away.
 #include <hip/hip_runtime.h>
 
@@ -87,10 +88,10 @@ __global__ void __launch_bounds__(256) probe_single(int64_t iters, float* out) {
       } else if constexpr (KIND == 7) {
         f[j] = __builtin_amdgcn_sinf(f[j]) + 0.5f;
       } else if constexpr (KIND == 9) {
-        // the surrogate's tanh on a pre-scaled argument: 1 - 2 / (2^z + 1), with one fma
-        // standing in for the bf16 repack of the result
+        // the surrogate's hidden unit on a pre-scaled argument, u = 1 / (2^z + 1) =
+        // (1 - tanh) / 2, with one fma standing in for the bf16 repack of the result
         const float z = __builtin_fmaf(f[j], 0.5f, 0.25f);
-        f[j] = __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(z) + 1.0f), 1.0f);
+        f[j] = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(z) + 1.0f);
       } else {
         f[j] = __builtin_amdgcn_sqrtf(f[j]) + 0.5f;
       }

@@ -1,4 +1,4 @@
-"""Measured compute roofs for the VALU-bound kernels (EM walk, KDE pairs, surrogate tanh).
+"""Measured compute roofs for the VALU-bound kernels (EM walk, KDE pairs, surrogate hidden units).
 
 The EM and KDE kernels do no MFMA work and move a few bytes per thousand
 instructions, so their roof is VALU issue. The probes in probe_kernel.hip run
@@ -12,7 +12,7 @@ from . import _device as D
 from . import _lib as L
 
 KINDS = {"em_block": 0, "kde_pair": 1, "mad_u64_u32": 2, "xor_b32": 3, "fma_f32": 4, "log_f32": 5, "exp_f32": 6,
-         "sin_f32": 7, "sqrt_f32": 8, "tanh": 9}
+         "sin_f32": 7, "sqrt_f32": 8, "hidden_unit": 9}
 # work units per lane-iteration: EM steps, KDE pairs, or instructions
 UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8, 9: 8}
 

@@ -97,3 +97,33 @@ def test_bf16_rounding_is_nearest_even():
     assert r[1] == 1.0  # halfway between 1.0 and 1.0078125 -> even
     assert r[2] == 1.0078125
     assert np.allclose(r, torch.tensor(v).bfloat16().double().numpy())
+
+
+def test_sigmoid_carry_rounding_matches_an_independent_torch_restatement():
+    """carry="sigmoid": each hidden unit crosses to the next layer as 1 - 2 bf16(u),
+    u = sigmoid(-2z); rebuilt here with torch's own bfloat16 rounding and sigmoid."""
+    p = _params(7)
+    x = np.random.default_rng(8).normal(size=(256, 7)).astype(np.float32)
+    y_plain, _ = mlp_ref.forward(p, x)
+    assert np.array_equal(mlp_ref.forward(p, x, carry="sigmoid")[0], y_plain)  # no rounding: same network
+    y_sig, _ = mlp_ref.forward(p, x, bf16=True, carry="sigmoid")
+    y_tanh, _ = mlp_ref.forward(p, x, bf16=True)
+
+    def bf(t):
+        return t.float().bfloat16().double()
+
+    s = mlp_ref.TANH_SCALE
+    h = bf(torch.from_numpy(x))
+    for i, (W, b) in enumerate(mlp_ref.unflatten(p)):
+        Wt = torch.from_numpy(W.astype(np.float32))
+        Wr = bf(Wt * np.float32(s)) / s if i < 3 else bf(Wt)
+        z = h @ Wr.T + torch.from_numpy(b.astype(np.float64))
+        if i == 3:
+            break
+        h = 1.0 - 2.0 * bf(torch.sigmoid(-2.0 * z))
+    assert np.max(np.abs(z[:, 0].numpy() - y_sig)) < 1e-9
+    # both roundings stay within the bf16 quantisation of the float64 network
+    assert np.max(np.abs(y_sig - y_plain)) < 0.05 and np.max(np.abs(y_tanh - y_plain)) < 0.05
+    assert np.max(np.abs(y_sig - y_tanh)) > 0  # the rounding point does move the result
+    with pytest.raises(ValueError):
+        mlp_ref.forward(p, x, carry="relu")
