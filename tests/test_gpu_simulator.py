@@ -257,3 +257,17 @@ def test_race_and_lca_variants_through_the_simulator(torch_gpu):
     assert fast["rts"].mean() < slow["rts"].mean() - 0.2
     const = Simulator("race_no_bias_4").simulate(base, n_samples=4000, random_state=2)
     assert abs(const["rts"].mean() - slow["rts"].mean()) < 0.05
+
+
+def test_zero_samples_give_empty_outputs(torch_gpu):
+    """n_samples = 0 launches nothing and returns empty arrays of the reference's shapes."""
+    from ssmhip.simulator import Simulator
+
+    for name, theta in (("angle", {"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.2, "theta": 0.3}),
+                        ("race_no_bias_3", {"v0": 1.0, "v1": 0.5, "v2": 0.2, "a": 2.0, "z": 0.3, "t": 0.1}),
+                        ("lba2", {"A": 0.3, "b": 0.6, "v0": 0.9, "v1": 0.5}),
+                        ("ddm_seq2_no_bias", {"vh": 0.5, "vl1": 0.5, "vl2": -0.5, "a": 1.0, "t": 0.2})):
+        print(f"[case] zero samples {name}", flush=True)
+        out = Simulator(name).simulate(theta, n_samples=0, random_state=3)
+        assert out["rts"].shape[0] == 0 and out["rts"].dtype == np.float32
+        assert out["choices"].shape == out["rts"].shape and out["choices"].dtype == np.intc
