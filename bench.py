@@ -171,11 +171,16 @@ def hssm_leg(args):
     v = np.clip(post[:, :1] + 0.5 * x[None, :], -3, 3).astype(np.float32)
     args_ = [v] + [post[:, j:j + 1] for j in range(1, 5)]
     fn = H.get_simulator_fun_internal("angle")
-    H.rng_fn([a[:10] for a in args_], None, np.random.default_rng(0), fn, 2)  # warm-up (module load)
+    # steady state: one call of the full size first (module load, device buffers), then the
+    # median of three timed calls
+    H.rng_fn(args_, None, np.random.default_rng(0), fn, 2)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = H.rng_fn(args_, None, np.random.default_rng(1), fn, 2)
-    wall = time.perf_counter() - t0
+    walls = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        out = H.rng_fn(args_, None, np.random.default_rng(1 + rep), fn, 2)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
     rows = draws * trials
     assert out.shape == (draws, trials, 2)
     theta = np.column_stack([np.broadcast_to(a, (draws, trials)).reshape(-1) for a in args_]).astype(np.float32)

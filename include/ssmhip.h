@@ -60,8 +60,10 @@ extern "C" {
 
 #define SSM_MAX_PARTICLES 16
 
-/* Per-set parameter record: float32[SSM_SET_STRIDE] per parameter set. */
+/* Per-set parameter record: float32[SSM_SET_STRIDE] per parameter set (the
+ * single-particle diffusions may use compact records, see set_stride).      */
 #define SSM_SET_STRIDE 64
+#define SSM_SET_STRIDE_SINGLE 16
 #define SSM_P_T0 0           /* non-decision time t                          */
 #define SSM_P_DEADLINE 1     /* deadline (999 when the model has none)       */
 #define SSM_P_DEADLINE_TMP 2 /* min(max_t, deadline - t), float32            */
@@ -96,7 +98,10 @@ typedef struct ssm_em_problem {
   int32_t smooth_unif;   /* 1: add the reference's uniform RT jitter          */
   int32_t out_layout;    /* 0: out[offsets[k] + n]; 1: out[n * n_sets + k]    */
   int32_t choice_bytes;  /* 1: int8 choices; 4: int32 (np.intc) choices       */
-  int32_t pad0;
+  int32_t set_stride;    /* floats per set record; 0 = SSM_SET_STRIDE. DDM,
+                            full DDM, Ornstein, Levy and legacy DDM read slots
+                            0..15 only and accept SSM_SET_STRIDE_SINGLE (16);
+                            the other models need at least SSM_SET_STRIDE   */
   uint64_t seed;         /* Philox key                                        */
   int64_t replay_stride; /* 0: native Philox; >0: replay buffer row length    */
 } ssm_em_problem;
@@ -129,7 +134,7 @@ int32_t ssm_device_cu_count(void);
  * cssm.racing_diffusion_model (src/cssm/race_models.pyx:925-1260) and
  * cssm.ddm_flexbound_par2 / ddm_flexbound_seq2 (src/cssm/parallel_models.pyx,
  * src/cssm/sequential_models.pyx), batched over ragged sets.
- *   set_params_dev   float32[n_sets][SSM_SET_STRIDE]
+ *   set_params_dev   float32[n_sets][set_stride] (set_stride 0: SSM_SET_STRIDE)
  *   offsets_dev      int64[n_sets + 1], offsets[0] = 0, non-decreasing
  *   table_dev        float32[n_sets][n_steps] when boundary == TABLE, else NULL
  *   replay_normals_dev float32[n_total][replay_stride] (replay mode) or NULL;

@@ -107,7 +107,7 @@ using namespace ssmhip;
 
 extern "C" {
 
-int32_t ssm_abi_version(void) { return 2; }
+int32_t ssm_abi_version(void) { return 3; }
 
 const char* ssm_last_error(void) { return g_last_error.c_str(); }
 

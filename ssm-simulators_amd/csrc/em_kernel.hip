@@ -38,6 +38,7 @@ struct EmArgs {
   unsigned long long* ws;  // [0] queue head, [1] executed steps, [2] lane-step slots issued
   int64_t total;
   int64_t replay_stride;
+  int32_t set_stride;  // floats per set record
   int32_t n_sets;
   int32_t n_steps;
   int32_t n_particles;
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             n = (int)(gi - A.qoff[jq]);
             k = A.order ? A.order[jq] : jq;
             g = A.order ? A.offsets[k] + n : gi;  // the trajectory's index in the set-major outputs
-            const float* sp = A.set_params + (int64_t)k * SSM_SET_STRIDE;
+            const float* sp = A.set_params + (int64_t)k * A.set_stride;
             t0 = sp[SSM_P_T0];
             deadline = sp[SSM_P_DEADLINE];
             dl_tmp = sp[SSM_P_DEADLINE_TMP];
@@ -713,6 +714,11 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
   if (P->out_layout != 0 && P->out_layout != 1) return fail(SSM_EINVAL, "out_layout must be 0 or 1");
   if (P->choice_bytes != 1 && P->choice_bytes != 4) return fail(SSM_EINVAL, "choice_bytes must be 1 or 4");
   if (P->replay_stride < 0) return fail(SSM_EINVAL, "replay_stride must be >= 0");
+  const bool two_stage = P->model == SSM_MODEL_PAR2 || P->model == SSM_MODEL_SEQ2;
+  const int32_t min_stride = (diffusion && !two_stage) ? SSM_SET_STRIDE_SINGLE : SSM_SET_STRIDE;
+  if (P->set_stride != 0 && (P->set_stride < min_stride || P->set_stride > (1 << 16)))
+    return fail(SSM_EINVAL, (diffusion && !two_stage) ? "set_stride must be 0 or in [16, 65536] for this model"
+                                                      : "set_stride must be 0 or in [64, 65536] for this model");
   if (P->out_layout == 1 && P->n_sets > 0 && P->n_total % P->n_sets != 0)
     return fail(SSM_EINVAL, "out_layout 1 needs the same n_samples for every set");
   if (P->n_total > 0) {
@@ -752,6 +758,7 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
 
   EmArgs A;
   A.set_params = set_params;
+  A.set_stride = P->set_stride ? P->set_stride : SSM_SET_STRIDE;
   A.offsets = offsets;
   A.order = (L && L->set_order) ? L->set_order : nullptr;
   A.qoff = A.order ? L->queue_offsets : offsets;

@@ -26,6 +26,7 @@ MODEL_PAR2, MODEL_SEQ2, MODEL_DDM_LEGACY = 7, 8, 9
 BOUNDARY_CONSTANT, BOUNDARY_ANGLE, BOUNDARY_WEIBULL, BOUNDARY_TABLE = 0, 1, 2, 3
 MAX_PARTICLES = 16
 SET_STRIDE = 64
+SET_STRIDE_SINGLE = 16  # compact records of the single-particle diffusions (slots 0..15)
 P_T0, P_DEADLINE, P_DEADLINE_TMP, P_A, P_B1, P_B2, P_Z, P_V, P_SQRT_ST, P_G, P_BINH, P_SZ, P_SV, P_ST, P_ALPHA, P_MAX_T = range(16)
 P_VJ, P_ZJ, P_SSTJ = 16, 32, 48
 EM_WORKSPACE_WORDS = 4
@@ -43,7 +44,7 @@ class EmProblem(C.Structure):
         ("smooth_unif", C.c_int32),
         ("out_layout", C.c_int32),
         ("choice_bytes", C.c_int32),
-        ("pad0", C.c_int32),
+        ("set_stride", C.c_int32),
         ("seed", C.c_uint64),
         ("replay_stride", C.c_int64),
     ]

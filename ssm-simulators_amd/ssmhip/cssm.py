@@ -105,7 +105,8 @@ def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, 
     code = kernel_boundary_code(boundary_fun)
     max_t = float(np.float32(max_t))
     delta_t = float(np.float32(delta_t))
-    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra)
+    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra,
+                      stride=L.SET_STRIDE_SINGLE)
     rts, choices = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t, _seed(random_state),
                         smooth_unif, boundary_fun, boundary_params, replay_normals, replay_uniforms)
     minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)

@@ -78,11 +78,14 @@ def levy_scale32(s, alpha, delta_t) -> np.ndarray:
 
 
 def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None, sv=None, st=None,
-                g=None, alpha=None) -> np.ndarray:
+                g=None, alpha=None, stride: int = L.SET_STRIDE) -> np.ndarray:
     """Records of the single-particle diffusions: DDM; with sz/sv/st the full DDM
     (ddm_models.pyx:2177-2360); with g the Ornstein–Uhlenbeck leak (ornstein_models.pyx:47-200);
     with alpha the Levy flight (levy_models.pyx:47-220), whose noise scale s dt^(1/alpha)
-    replaces sqrt(dt) s."""
+    replaces sqrt(dt) s. stride = L.SET_STRIDE_SINGLE gives the compact 16-float records these
+    models accept (a quarter of the bytes to build and upload)."""
+    if stride not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
+        raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     n = v.shape[0]
     # the single-particle slots all lie in the first 16 floats: fill them contiguously, then
@@ -106,6 +109,8 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     a_b = np.asarray(bparams.get("a", a), F32).reshape(n, -1)[:, 0]
     rec[:, L.P_A] = a_b
     fill_boundary(rec, code, bparams)
+    if stride == L.SET_STRIDE_SINGLE:
+        return rec
     full = new_records(n)
     full[:, :16] = rec
     return full
@@ -270,6 +275,9 @@ def make_job(
 ) -> EmJob:
     t = D.require_gpu()
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    if records.ndim != 2 or records.shape[1] not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
+        raise ValueError(f"records must have {L.SET_STRIDE} (or, single-particle diffusions, "
+                         f"{L.SET_STRIDE_SINGLE}) floats per set")
     n_sets = records.shape[0]
     if offsets.shape != (n_sets + 1,):
         raise ValueError("offsets must have n_sets + 1 entries")
@@ -312,7 +320,7 @@ def make_job(
         smooth_unif=int(bool(smooth_unif)),
         out_layout=out_layout,
         choice_bytes=choice_bytes,
-        pad0=0,
+        set_stride=0 if records.shape[1] == L.SET_STRIDE else int(records.shape[1]),
         seed=int(seed) & 0xFFFFFFFFFFFFFFFF,
         replay_stride=stride,
     )

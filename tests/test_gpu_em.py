@@ -373,7 +373,8 @@ def test_same_seed_same_output_different_seed_different(torch_gpu):
 
 
 # ----------------------------------------------------------------------------- full DDM and Ornstein
-def _run_gpu_diffusion(model, v, a, z, t, bp, bname, N, normals=None, unif=None, smooth=False, seed=1, **extra):
+def _run_gpu_diffusion(model, v, a, z, t, bp, bname, N, normals=None, unif=None, smooth=False, seed=1,
+                       stride=None, **extra):
     import torch
 
     L, em = _ssm()
@@ -382,7 +383,8 @@ def _run_gpu_diffusion(model, v, a, z, t, bp, bname, N, normals=None, unif=None,
     K = v.shape[0]
     fun = {"constant": BF.constant, "angle": BF.angle}[bname]
     code = BF.kernel_boundary_code(fun)
-    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, code, bp, **extra)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), 20.0, DT, code, bp, **extra,
+                         stride=stride or L.SET_STRIDE)
     job = em.make_job(model, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=seed,
                       smooth_unif=smooth, replay_normals=normals, replay_uniforms=unif)
     job.launch()
@@ -686,3 +688,39 @@ def test_legacy_ddm_replay_matches_oracle(torch_gpu):
                                  smooth_unif=True, normals=normals, uniforms=unif, legacy=True)
     r1, c1, steps = _run_gpu_diffusion(L.MODEL_DDM_LEGACY, v, a, z, t, {"a": a}, "constant", N, normals, unif, True)
     assert np.array_equal(c0, c1) and np.array_equal(r0, r1) and steps == int(s0.sum())
+
+
+@pytest.mark.parametrize("model", ["ddm", "full_ddm", "ornstein", "levy", "legacy"])
+def test_compact_records_give_the_same_walks(torch_gpu, model):
+    """16-float records (set_stride 16) of the single-particle diffusions: same results, bit
+    for bit, as the 64-float records; the other models refuse them."""
+    import ctypes as C
+
+    L, em = _ssm()
+    from ssmhip import _device as D
+
+    print(f"[case] compact records {model}", flush=True)
+    K, N = 24, 500
+    v, a, z, t, bp = _ddm_case("angle", K, 41)
+    mdl = {"ddm": L.MODEL_DDM, "full_ddm": L.MODEL_FULL_DDM, "ornstein": L.MODEL_ORNSTEIN, "levy": L.MODEL_LEVY,
+           "legacy": L.MODEL_DDM_LEGACY}[model]
+    extra = {"full_ddm": _variability(K, 42), "ornstein": {"g": np.full(K, 0.3, F32)},
+             "levy": {"alpha": np.full(K, 1.6, F32)}}.get(model, {})
+    if model == "legacy":
+        z = np.full(K, 0.5, F32)
+        bp = {"a": a}
+    bname = "constant" if model == "legacy" else "angle"
+    full = _run_gpu_diffusion(mdl, v, a, z, t, bp, bname, N, smooth=True, seed=5, **extra)
+    compact = _run_gpu_diffusion(mdl, v, a, z, t, bp, bname, N, smooth=True, seed=5, stride=L.SET_STRIDE_SINGLE,
+                                 **extra)
+    assert np.array_equal(full[0], compact[0]) and np.array_equal(full[1], compact[1]) and full[2] == compact[2]
+    if model == "ddm":
+        # a race with 16-float records is refused before anything is launched
+        rec = em.accumulator_records(np.ones((2, 2), F32), np.full((2, 2), 0.3, F32), np.zeros(2, F32),
+                                     np.ones((2, 2), F32), np.full(2, 999, F32), 20.0, DT, L.BOUNDARY_CONSTANT,
+                                     {"a": np.ones(2, F32)})
+        job = em.make_job(L.MODEL_RACE, L.BOUNDARY_CONSTANT, 2, rec, em.uniform_offsets(2, 10), delta_t=DT,
+                          max_t=20.0, seed=1, smooth_unif=False)
+        job.prob.set_stride = L.SET_STRIDE_SINGLE
+        with pytest.raises(ValueError, match="set_stride"):
+            job.launch()

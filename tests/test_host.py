@@ -112,6 +112,24 @@ def test_race_and_lca_variant_configs():
     assert config.get("full_ddm2")["simulator"] is config.get("full_ddm")["simulator"]
 
 
+def test_compact_single_particle_records():
+    """ddm_records(stride=16) holds exactly the first 16 floats of the full records."""
+    from ssmhip import _lib as L
+    from ssmhip import em
+
+    rng = np.random.default_rng(12)
+    n = 9
+    args = [rng.uniform(0.1, 0.9, n).astype(F32) for _ in range(4)] + [np.full(n, 999, F32), np.ones(n, F32)]
+    bp = {"a": rng.uniform(0.5, 2, n).astype(F32), "theta": rng.uniform(-0.1, 1.3, n).astype(F32)}
+    full = em.ddm_records(*args, 20.0, 1e-3, L.BOUNDARY_ANGLE, bp, alpha=np.full(n, 1.5, F32))
+    compact = em.ddm_records(*args, 20.0, 1e-3, L.BOUNDARY_ANGLE, bp, alpha=np.full(n, 1.5, F32),
+                             stride=L.SET_STRIDE_SINGLE)
+    assert full.shape == (n, L.SET_STRIDE) and compact.shape == (n, L.SET_STRIDE_SINGLE)
+    assert np.array_equal(full[:, :16], compact) and not np.any(full[:, 16:])
+    with pytest.raises(ValueError):
+        em.ddm_records(*args, 20.0, 1e-3, L.BOUNDARY_ANGLE, bp, stride=32)
+
+
 def test_batched_records_shape_checks():
     from ssmhip import batched
 
