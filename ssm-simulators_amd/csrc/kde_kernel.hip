@@ -342,12 +342,16 @@ constexpr int kTile = 2048;
 // Pass 1: block (group g, chunk c) sums the kernel over samples
 // [c * SSM_KDE_CHUNK, (c + 1) * SSM_KDE_CHUNK) of group g for every point of g.
 // Splitting groups along their samples balances the very unequal
-// (points x samples) work of the groups across the chip. A group with few points
-// (<= 512, <= 256) splits the block's threads into 2 or 4 teams that cover the same
-// points against different quarters/halves of each sample tile, so every lane holds
-// live points; the teams' sums are added in LDS in a fixed order.
-// NQ of a thread's kPtsPerThread points against the samples [lo, hi) of the LDS tile, four
-// samples per step in tile order (the per-point summation order does not depend on NQ)
+// (points x samples) work of the groups across the chip. Inside a block each of the four
+// waves takes every point of the group (up to kRows rows of 64 per pass) against its own
+// quarter of each sample tile, so all waves carry the same number of point rows and none
+// waits at the tile barriers for another; only the last row of a group has idle lanes.
+// The four waves' sums are added in LDS in a fixed order.
+constexpr int kWaves = kEvalThreads / 64;
+constexpr int kRows = 2 * kPtsPerThread;  // point rows per wave per pass (512 points)
+
+// NQ of a group of kPtsPerThread point rows against the samples [lo, hi) of the LDS tile,
+// four samples per step in tile order (the per-point summation order does not depend on NQ)
 template <int NQ>
 __device__ __forceinline__ void kde_pairs(const float* tile, int lo, int hi, const float (&x)[kPtsPerThread],
                                           float (&acc)[kPtsPerThread]) {
@@ -366,7 +370,7 @@ __device__ __forceinline__ void kde_pairs(const float* tile, int lo, int hi, con
 
 __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs A) {
   __shared__ __attribute__((aligned(16))) float tile[kTile];
-  __shared__ float red[3 * kEvalThreads];
+  __shared__ float red[(kWaves - 1) * kRows * 64];
   const int g = blockIdx.x;
   const int chunk = blockIdx.y;
   const int cnt = A.grp_count[g];
@@ -382,62 +386,62 @@ __global__ void __launch_bounds__(kEvalThreads) kde_eval_partial_kernel(EvalArgs
   // 2^(64 - log2(e) (x - s)^2 / (2 h^2)) = 2^64 exp(-(x - s)^2 / (2 h^2)).
   const float sc = 0.84932180028801904f / h;
   const float ctr = smp[0];
-  const int teams = cnt <= kEvalThreads ? 4 : (cnt <= 2 * kEvalThreads ? 2 : 1);
-  const int tpt = kEvalThreads / teams;  // threads per team
-  const int team = threadIdx.x / tpt, lt = threadIdx.x % tpt;
-  const int seg = kTile / teams;         // samples of each tile per team (a multiple of 4)
-  for (int pbase = 0; pbase < cnt; pbase += tpt * kPtsPerThread) {
-    float x[kPtsPerThread], acc[kPtsPerThread];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int seg = kTile / kWaves;  // samples of each tile per wave (a multiple of 4)
+  for (int pbase = 0; pbase < cnt; pbase += kRows * 64) {
+    // rows of 64 points this pass: the same for every wave of the block
+    const int rows = min(kRows, (cnt - pbase + 63) >> 6);
+    float x[kRows / kPtsPerThread][kPtsPerThread], acc[kRows / kPtsPerThread][kPtsPerThread];
 #pragma unroll
-    for (int q = 0; q < kPtsPerThread; ++q) {
-      const int pi = pbase + q * tpt + lt;
-      const float rt = pi < cnt ? A.pts_rt[pb + pi] : 1.f;
-      x[q] = (logf(fmaxf(rt, A.eps)) - ctr) * sc;
-      acc[q] = 0.f;
-    }
+    for (int r4 = 0; r4 < kRows / kPtsPerThread; ++r4)
+#pragma unroll
+      for (int q = 0; q < kPtsPerThread; ++q) {
+        const int pi = pbase + (r4 * kPtsPerThread + q) * 64 + lane;
+        const float rt = pi < cnt ? A.pts_rt[pb + pi] : 1.f;
+        x[r4][q] = (logf(fmaxf(rt, A.eps)) - ctr) * sc;
+        acc[r4][q] = 0.f;
+      }
     for (int t0 = s0; t0 < s1; t0 += kTile) {
       const int tn = (s1 - t0) < kTile ? (s1 - t0) : kTile;
       __syncthreads();
       for (int i = threadIdx.x; i < kTile; i += kEvalThreads) tile[i] = i < tn ? (smp[t0 + i] - ctr) * sc : 1e30f;
       __syncthreads();
       const int tn4 = (tn + 3) & ~3;
-      const int lo = team * seg;
+      const int lo = wave * seg;
       const int hi = lo + seg < tn4 ? lo + seg : tn4;
-      // point row q of this wave holds points pbase + q tpt + (wave's lanes): rows lying wholly
-      // past the group's last point are skipped (wave-uniform), so a pass that is only partly
-      // filled costs at most 63 idle lanes per team instead of up to 3/4 of its point slots
-      const int first = pbase + ((lt >> 6) << 6);  // this wave's lane-0 point of row 0
-      int nq = first < cnt ? (cnt - first + tpt - 1) / tpt : 0;
-      nq = nq < kPtsPerThread ? nq : kPtsPerThread;
-      static_assert(kPtsPerThread == 4, "kde_pairs dispatch below covers 1..4 rows");
-      switch (nq) {
-        case 4: kde_pairs<4>(tile, lo, hi, x, acc); break;
-        case 3: kde_pairs<3>(tile, lo, hi, x, acc); break;
-        case 2: kde_pairs<2>(tile, lo, hi, x, acc); break;
-        case 1: kde_pairs<1>(tile, lo, hi, x, acc); break;
-        default: break;
-      }
-    }
-    if (teams > 1) {
-      __syncthreads();
-      if (team > 0) {
 #pragma unroll
-        for (int q = 0; q < kPtsPerThread; ++q) red[((team - 1) * tpt + lt) * kPtsPerThread + q] = acc[q];
-      }
-      __syncthreads();
-      if (team == 0) {
-        for (int m = 1; m < teams; ++m) {
-#pragma unroll
-          for (int q = 0; q < kPtsPerThread; ++q) acc[q] += red[((m - 1) * tpt + lt) * kPtsPerThread + q];
+      for (int r4 = 0; r4 < kRows / kPtsPerThread; ++r4) {
+        const int nq = min(kPtsPerThread, rows - r4 * kPtsPerThread);  // wave-uniform
+        static_assert(kPtsPerThread == 4, "kde_pairs dispatch below covers 1..4 rows");
+        switch (nq) {
+          case 4: kde_pairs<4>(tile, lo, hi, x[r4], acc[r4]); break;
+          case 3: kde_pairs<3>(tile, lo, hi, x[r4], acc[r4]); break;
+          case 2: kde_pairs<2>(tile, lo, hi, x[r4], acc[r4]); break;
+          case 1: kde_pairs<1>(tile, lo, hi, x[r4], acc[r4]); break;
+          default: break;
         }
       }
     }
-    if (team == 0) {
+    // waves 1..3 hand their sums to wave 0, which adds them in wave order
+    __syncthreads();
+    if (wave > 0) {
 #pragma unroll
-      for (int q = 0; q < kPtsPerThread; ++q) {
-        const int pi = pbase + q * tpt + lt;
-        if (pi < cnt) A.partial[(pb + pi) * A.n_chunks + chunk] = acc[q];
-      }
+      for (int r4 = 0; r4 < kRows / kPtsPerThread; ++r4)
+#pragma unroll
+        for (int q = 0; q < kPtsPerThread; ++q)
+          red[((wave - 1) * kRows + r4 * kPtsPerThread + q) * 64 + lane] = acc[r4][q];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int r4 = 0; r4 < kRows / kPtsPerThread; ++r4)
+#pragma unroll
+        for (int q = 0; q < kPtsPerThread; ++q) {
+          float v = acc[r4][q];
+          for (int m = 1; m < kWaves; ++m) v += red[((m - 1) * kRows + r4 * kPtsPerThread + q) * 64 + lane];
+          const int pi = pbase + (r4 * kPtsPerThread + q) * 64 + lane;
+          if (pi < cnt) A.partial[(pb + pi) * A.n_chunks + chunk] = v;
+        }
     }
   }
 }
