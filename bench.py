@@ -105,7 +105,7 @@ def accumulator_leg(args):
     """LCA-4 and race-4/Weibull, 1024 sets x 20,000 samples each, against the CPU restatement."""
     from oracle import cpu_baseline
     from oracle import em_numpy as E
-    from ssmhip import batched
+    from ssmhip import batched, roofline
 
     out = {}
     for model, kind in (("lca_4", 2), ("race_4_weibull", 1)):
@@ -119,6 +119,12 @@ def accumulator_leg(args):
                "em_steps_per_s": steps / (ms / 1e3),
                "launch": [lp.block_size, lp.blocks_per_cu, lp.refill_min, lp.queue_block],
                "lane_utilisation": steps / max(1, sim.lane_slots())}
+        if not args.no_probe:
+            # roof: the same model's step (bound test, one Philox block, 4-accumulator update)
+            # branch-free at full occupancy
+            peak = roofline.probe_rate("lca4_step" if model == "lca_4" else "race4_weibull_step", iters=2048)
+            rec["roofline"] = {"bound": "valu", "achieved": rec["em_steps_per_s"] / 1e9, "peak": peak / 1e9,
+                               "unit": "G EM-steps/s", "frac": rec["em_steps_per_s"] / peak}
         if not args.no_cpu:
             idx = np.arange(0, theta.shape[0], 32)
             n_cpu = 20000
@@ -146,6 +152,7 @@ def accumulator_leg(args):
             rec["gpu_over_cpu"] = rec["trajectories_per_s"] / rec["cpu_baseline"]["value"]
         out[model] = rec
         log(f"accumulators {model}: {rec['trajectories_per_s']:.3e} traj/s"
+            + (f", {rec['roofline']['frac'] * 100:.0f}% of the step roof" if "roofline" in rec else "")
             + (f", {rec['gpu_over_cpu']:.0f}x CPU" if "gpu_over_cpu" in rec else ""))
     return out
 

@@ -8,6 +8,9 @@
 //           6 v_exp_f32, 7 v_sin_f32, 8 v_sqrt_f32
 //   kind 9  the surrogate's hidden unit u = 1 / (2^z + 1): exp2, add, rcp (+ one fma),
 //           8 chains per lane
+//   kind 10 LCA step, 4 accumulators, constant bound: bound test, Philox4x32-10 + 2
+//           Box–Muller pairs, the leaky, inhibited update with its float32 sum
+//   kind 11 race step, 4 accumulators, Weibull bound: the same with the race update
 // Every lane writes one float so nothing is optimised away.
 #include <hip/hip_runtime.h>
 
@@ -34,6 +37,58 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
     }
   }
   out[gid] = y + t;
+}
+
+template <bool LCA>
+__global__ void __launch_bounds__(256) probe_acc_step(int64_t iters, float* out) {
+  // one accumulator-model step per iteration, as the simulator's accumulator loop does it
+  // (bound test on every accumulator, four normals from one Philox block, update), without
+  // walk termination or refill: a finished walk restarts in place
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float dt = 0.001f, a = 2.0f, leak = 0.2f, inh = 0.1f, alpha = 2.0f, binv = 1.0f / 3.0f;
+  float x[4], vraw[4], vdt[4], sst[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    x[j] = 0.05f * (float)((gid + j) & 7);
+    vraw[j] = 0.4f + 0.1f * (float)j;
+    vdt[j] = vraw[j] * dt;
+    sst[j] = 0.0316f;
+  }
+  float t = 0.f;
+  int ix = 0;
+  for (int64_t it = 0; it < iters; ++it) {
+    float b = a;
+    if constexpr (!LCA) {
+      const float u = ((float)ix * dt) * binv;
+      const float p = __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(u));
+      b = a * __builtin_amdgcn_exp2f(-1.4426950408889634f * p);
+    }
+    bool go = ix < 20001 && t <= 20.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) go = go && !(x[j] > b);
+    float nz[4];
+    philox_normals4((uint32_t)gid, 7u, (uint32_t)ix, 0u, 0x1234u, 0x5678u, nz);
+    float xn[4];
+    if constexpr (LCA) {
+      float sum = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sum += x[j];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float red = (-x[j]) + sum;
+        const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
+        xn[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xn[j] = fmaxf(0.0f, x[j] + (vdt[j] + sst[j] * nz[j]));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = go ? xn[j] : 0.5f * x[j];
+    t = go ? t + dt : 0.f;
+    ix = go ? ix + 1 : 0;
+  }
+  out[gid] = x[0] + x[1] + x[2] + x[3] + t;
 }
 
 __global__ void __launch_bounds__(256) probe_kde_pair(int64_t iters, float* out) {
@@ -118,6 +173,8 @@ int32_t probe_enqueue(int32_t kind, int64_t iters, int64_t n_threads, float* out
     case 7: hipLaunchKernelGGL(probe_single<7>, grid, block, 0, stream, iters, out); break;
     case 8: hipLaunchKernelGGL(probe_single<8>, grid, block, 0, stream, iters, out); break;
     case 9: hipLaunchKernelGGL(probe_single<9>, grid, block, 0, stream, iters, out); break;
+    case 10: hipLaunchKernelGGL(probe_acc_step<true>, grid, block, 0, stream, iters, out); break;
+    case 11: hipLaunchKernelGGL(probe_acc_step<false>, grid, block, 0, stream, iters, out); break;
     default: return fail(SSM_EINVAL, "unknown probe kind");
   }
   SSM_HIP(hipGetLastError());

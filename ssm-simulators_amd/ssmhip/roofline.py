@@ -12,9 +12,9 @@ from . import _device as D
 from . import _lib as L
 
 KINDS = {"em_block": 0, "kde_pair": 1, "mad_u64_u32": 2, "xor_b32": 3, "fma_f32": 4, "log_f32": 5, "exp_f32": 6,
-         "sin_f32": 7, "sqrt_f32": 8, "hidden_unit": 9}
+         "sin_f32": 7, "sqrt_f32": 8, "hidden_unit": 9, "lca4_step": 10, "race4_weibull_step": 11}
 # work units per lane-iteration: EM steps, KDE pairs, or instructions
-UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8, 9: 8}
+UNITS = {0: 4, 1: 16, 2: 8, 3: 8, 4: 8, 5: 8, 6: 8, 7: 8, 8: 8, 9: 8, 10: 1, 11: 1}
 
 # MI355X's published FP32 vector peak, 157.3 TFLOP/s, as FMA lane-operations/s (an FMA is
 # 2 FLOP); the EM kernel was measured issuing 48e12 lane-level VALU instructions/s, so a
