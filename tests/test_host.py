@@ -189,3 +189,15 @@ def test_angle_slope_array_call_equals_the_per_trial_scalar_calls():
     th[:3] = np.float32([0.0, 1.3, -0.1])
     scalar = np.array([(-np.sin(x) / np.cos(x)) for x in th], dtype=np.float32)
     assert np.array_equal(em.angle_slope32(th), scalar)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says why, before any
+    device work."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    bench = Path(__file__).resolve().parents[1] / "bench.py"
+    r = subprocess.run([sys.executable, str(bench), "--gpus", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
