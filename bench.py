@@ -463,17 +463,40 @@ def main() -> int:
     return 0
 
 
-def em_traffic_bytes():
-    """HBM bytes per headline launch from the committed counter summary (two rocprofv3 --pmc runs)."""
+def em_counter_value(key: str):
+    """A figure of the committed counter summary of the headline launch (rocprofv3 --pmc runs):
+    traffic_bytes_per_launch, valu_lane_instructions_per_step."""
     path = REPO / "profiles" / "em_angle_batched4096_counters.md"
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                if line.startswith(key + ":"):
+                    return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def published_valu_view(steps_per_s: float):
+    """The headline kernel against MI355X's published FP32 vector peak (157.3 TFLOP/s = 78.65e12
+    FMA lane-instructions/s): lane-level VALU instructions per useful EM step from the counter
+    summary (SQ_INSTS_VALU x 64 / steps) times the EM steps/s of this run."""
+    from ssmhip import roofline
+
+    ips = em_counter_value("valu_lane_instructions_per_step")
+    if ips is None:
+        return None
+    achieved = ips * steps_per_s
+    return {"unit": "T VALU lane-instructions/s", "achieved": achieved / 1e12,
+            "peak": roofline.PEAK_VALU_LANE_OPS / 1e12, "frac": achieved / roofline.PEAK_VALU_LANE_OPS,
+            "basis": f"{ips:.2f} lane-level VALU instructions per useful EM step "
+                     "(profiles/em_angle_batched4096_counters.md) x EM steps/s; peak = 157.3 TFLOP/s / 2"}
+
+
+def em_traffic_bytes():
+    """HBM bytes per headline launch from the committed counter summary (two rocprofv3 --pmc runs)."""
+    v = em_counter_value("traffic_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def headline_leg(args, torch, batched, roofline) -> dict:
@@ -572,6 +595,7 @@ def headline_leg(args, torch, batched, roofline) -> dict:
                              f"bytes {int(traj) * 5 + int(args.n_sets) * 264}",
             "peak_basis": "measured: branch-free Philox4x32-10 + Box-Muller + angle-step loop at full occupancy "
                           "(VALU issue bound; no MFMA, ~5 B of HBM traffic per trajectory)",
+            "published_peak": published_valu_view(steps_per_s),
         },
     }
     if args.copy_calibration_mib:
