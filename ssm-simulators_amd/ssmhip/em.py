@@ -88,9 +88,10 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
         raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
     v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
     n = v.shape[0]
-    # the single-particle slots all lie in the first 16 floats: fill them contiguously, then
-    # place them in the 64-float records with one strided copy
-    rec = np.zeros((n, 16), F32)
+    # the single-particle slots all lie in the first 16 floats: each slot is written as one
+    # contiguous row of a slot-major (16, n) array (rec is its transposed view), then the
+    # records are laid out by one transposing copy (and, for 64-float records, one more)
+    rec = np.zeros((16, n), F32).T
     for slot, val in ((L.P_SZ, sz), (L.P_SV, sv), (L.P_ST, st), (L.P_G, g)):
         if val is not None:
             rec[:, slot] = np.asarray(val, F32).reshape(n, -1)[:, 0]
@@ -110,7 +111,7 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     rec[:, L.P_A] = a_b
     fill_boundary(rec, code, bparams)
     if stride == L.SET_STRIDE_SINGLE:
-        return rec
+        return np.ascontiguousarray(rec)
     full = new_records(n)
     full[:, :16] = rec
     return full
