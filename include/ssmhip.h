@@ -121,6 +121,8 @@ typedef struct ssm_launch_config {
 } ssm_launch_config;
 
 /* ---- library ---------------------------------------------------------------*/
+/* 3: ssm_em_problem.set_stride (compact 16-float records for the single-particle
+ *    diffusions; 0 keeps SSM_SET_STRIDE, the layout of version 2).            */
 int32_t ssm_abi_version(void);
 const char* ssm_last_error(void);
 /* Number of CUs of the current device (0 and an error when no GPU).          */
