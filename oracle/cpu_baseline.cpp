@@ -136,7 +136,12 @@ int64_t cpu_em_simulate(int model, int P, int n_sets, const int64_t* offsets, co
       int choice = 0;
       if (model == 0) {
         const float vdt = v[k] * dt, s = sst[k];
-        float y = (float)((double)(-b0) + (double)z[k] * 2.0 * (double)b0);  // 2.0 is a C double
+        // replay mode restates the sequential loop, whose start (ddm_models.pyx:709, int 2) is
+        // float arithmetic; the native mode restates the parallel branch, whose start (:798,
+        // double literal 2.0) is formed in double and rounded once
+        const float y_seq = (-b0) + ((z[k] * 2.0f) * b0);
+        const float y_par = (float)((double)(-b0) + (double)z[k] * 2.0 * (double)b0);
+        float y = replay_normals ? y_seq : y_par;
         for (;;) {
           if (ix >= n_steps) break;
           const float bb = row[ix];

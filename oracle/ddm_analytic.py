@@ -1,7 +1,7 @@
 """Closed forms for the constant-bound Wiener first-passage problem (test oracle).
 
 Process dX = v dt + dW on (-a, a), X0 = (2z - 1) a (the ddm_flexbound
-convention, src/cssm/ddm_models.pyx:719). Written from the standard results:
+convention, src/cssm/ddm_models.pyx:709). Written from the standard results:
   * absorption probability and mean exit time (Cox & Miller 1965, ch. 5),
   * first-passage density series of Navarro & Fuss (2009), with the
     large-time and small-time expansions.

@@ -1,7 +1,7 @@
 """Float32 NumPy transcription of the reference's sequential EM loops (test oracle).
 
 Follows, operation by operation in float32:
-  ddm_flexbound  src/cssm/ddm_models.pyx:696-744  (start :719, walk :721-733,
+  ddm_flexbound  src/cssm/ddm_models.pyx:696-744  (start :709, walk :719-733,
                  smooth_unif :735-739, rt :741, sign :742, deadline :744)
   ddm            src/cssm/ddm_models.pyx:453-482 (legacy bounds 0 and a)
   full_ddm       src/cssm/ddm_models.pyx:2284-2358 (variability of v, z, t)
@@ -90,6 +90,21 @@ def smooth_unif32(t: np.ndarray, dl_tmp: np.ndarray, dt: np.float32, u: np.ndarr
     half = (u64 * 0.5 * np.float64(dt)).astype(F32)
     mid = ((0.5 - u64) * np.float64(dt)).astype(F32)
     return np.where(t == F32(0.0), half, np.where(t < dl_tmp, mid, F32(0.0))).astype(F32)
+
+
+def ddm_start32(z, b0, double_literal=False):
+    """Start point -b0 + 2 z b0 of the two-bound diffusions, in the reference's C precision.
+
+    ddm_flexbound (src/cssm/ddm_models.pyx:709) writes `(-1) * b0 + (z * 2 * b0)` with an
+    int 2 and float operands, so C evaluates it in float: fl(-b0 + fl(fl(2 z) b0)), two
+    roundings (ornstein_models.pyx:165, levy_models.pyx:183 and the two-stage models use the
+    same form). full_ddm (:2305) and the parallel branch (:798) write `z * 2.0 * b0` with a
+    double literal: the sum is formed in double and rounded to float once."""
+    z = np.asarray(z, F32)
+    b0 = np.asarray(b0, F32)
+    if double_literal:
+        return ((-b0).astype(np.float64) + (z.astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
+    return ((-b0) + ((z * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
 
 
 def stable_cms(u, w, alpha):
@@ -185,17 +200,15 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     total = K * N
     lane_set = np.repeat(np.arange(K), N)
     b0 = B[lane_set, 0]
-    # :719 in C: (-1) * b0 is float, z * 2.0 * b0 double (2.0 is a double literal); the sum is
-    # double, rounded to float on assignment
     if levy:
         alpha_k = np.asarray(alpha, F32).reshape(-1)
         sqrt_st_k = levy_scale32(s, alpha_k, delta_t)
     if legacy:
         y = (z[lane_set] * b0).astype(F32)
-    elif ou or levy:
-        y = ((-b0) + ((z[lane_set] * F32(2)).astype(F32) * b0).astype(F32)).astype(F32)
+    elif full:
+        y = ddm_start32(z[lane_set], b0, double_literal=True)
     else:
-        y = ((-b0).astype(np.float64) + (z[lane_set].astype(np.float64) * 2.0) * b0.astype(np.float64)).astype(F32)
+        y = ddm_start32(z[lane_set], b0)
     t_lane = t[lane_set].copy()
     vdt_lane = vdt_k[lane_set].copy()
     draw_off = 0

@@ -96,6 +96,52 @@ __device__ __forceinline__ float step_increment(float y, float vdt, float v, flo
   else return vdt + sst * nz;
 }
 
+// ---- float32 forms of the reference's double-precision expressions ----------------
+// A few reference lines mix float operands with C double literals (0.5, 2.0) or libc's
+// double fmax, so C evaluates them in double and rounds to float on assignment. The
+// operands are floats, so the exact value is a short sum of float products; it is formed
+// here without FP64 by error-free transformations and rounded once. This equals the double
+// expression whenever its double intermediates are exact, which holds for the operand
+// ranges of these lines except in cases where rounding to double and then to float lands
+// differently from rounding once (relative probability ~2^-29 per value).
+
+// Knuth's TwoSum: s + e == a + b exactly (needs -ffp-contract=off, as this file is built).
+__device__ __forceinline__ float two_sum(float a, float b, float& e) {
+  const float s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+  return s;
+}
+
+// y + 2 (u - 0.5) s, the full DDM's start and non-decision-time jitter
+// (ddm_models.pyx:2295-2302, float y, u, s and double literals).
+__device__ __forceinline__ float add_unit_shift(float y, float u, float s) {
+  float wl;
+  const float wh = two_sum(u + u, -1.0f, wl);                 // 2u - 1 == wh + wl exactly
+  const float ph = wh * s;
+  const float pl = __builtin_fmaf(wh, s, -ph) + wl * s;       // wh s - ph exactly, plus wl s
+  float e;
+  const float r = two_sum(y, ph, e);
+  return r + (e + pl);
+}
+
+// compute_smooth_unif (_utils.pyx:668-695): u 0.5 dt at t == 0 (u 0.5 is exact and the float
+// product of two floats is the rounded exact product, as in double), (0.5 - u) dt before the
+// deadline (one rounding of the exact value by the FMA), else 0.
+__device__ __forceinline__ float smooth_unif_shift(float u, float t, float dl_tmp, float dt) {
+  if (t == 0.f) return (u * 0.5f) * dt;
+  if (t < dl_tmp) return __builtin_fmaf(-u, dt, 0.5f * dt);
+  return 0.f;
+}
+
+// a + b + c rounded once (parallel_models.pyx:538: fmax(t_h, t_l) + t + smooth_u is a double sum)
+__device__ __forceinline__ float sum3_rounded(float a, float b, float c) {
+  float e1, e2;
+  const float s1 = two_sum(a, b, e1);
+  const float s2 = two_sum(s1, c, e2);
+  return s2 + (e1 + e2);
+}
+
 // the single-particle diffusions that share the DDM walk
 template <int MODEL>
 constexpr bool kDiffusion = MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM || MODEL == SSM_MODEL_ORNSTEIN ||
@@ -211,15 +257,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             u32x4 r = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, 0u, 1u}, A.k0, A.k1);
             u = u32_to_unit(r.x);
           }
-          // compute_smooth_unif, _utils.pyx:668-695
-          // the 0.5 literals are C doubles: formed in double, rounded to float once
-          if (t_end == 0.f) su = (float)((double)u * 0.5 * (double)dt);
-          else if (t_end < dl_tmp) su = (float)((0.5 - (double)u) * (double)dt);
-          else su = 0.f;
+          su = smooth_unif_shift(u, t_end, dl_tmp, dt);
         }
         float rt;
         if constexpr (MODEL == SSM_MODEL_PAR2)
-          rt = (float)(((double)t_end + (double)t0) + (double)su);  // parallel_models.pyx:538: fmax is a double
+          rt = sum3_rounded(t_end, t0, su);
         else
           rt = t + t0 + su;
         if (rt >= deadline || deadline <= 0.f) rt = -999.f;  // enforce_deadline, _utils.pyx:697-710
@@ -329,10 +371,17 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               sst[0] = sp[SSM_P_SQRT_ST];
               if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) {
                 x[0] = z * b0;  // ddm_models.pyx:457: start z a, in float
-              } else if constexpr (MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_LEVY || kTwoStage<MODEL>) {
-                // ornstein_models.pyx:165, levy_models.pyx:183, parallel_models.pyx:452,
-                // sequential_models.pyx:460: the literal 2 is an int, so this stays in float
+              } else if constexpr (MODEL == SSM_MODEL_FULL_DDM) {
+                // ddm_models.pyx:2305: the literal 2.0 is a C double, so the sum is the exact
+                // -b0 + 2 z b0 rounded to float once: one FMA (2 z is exact)
+                x[0] = __builtin_fmaf(z * 2.0f, b0, -b0);
+              } else {
+                // ddm_models.pyx:709 (and ornstein_models.pyx:165, levy_models.pyx:183,
+                // parallel_models.pyx:452, sequential_models.pyx:460): the literal 2 is an int,
+                // so C evaluates (-b0) + ((z 2) b0) in float with two roundings
                 x[0] = (-b0) + ((z * 2.0f) * b0);
+              }
+              if constexpr (MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_LEVY || kTwoStage<MODEL>) {
                 leak = sp[SSM_P_G];
                 vraw[0] = sp[SSM_P_V];
                 if constexpr (MODEL == SSM_MODEL_LEVY) {
@@ -354,15 +403,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                     uu[2] = u32_to_unit(r.z); uu[3] = u32_to_unit(r.w);
                   }
                 }
-              } else {
-                // ddm_models.pyx:719 / :2291: the literal 2.0 is a C double, so the sum is
-                // formed in double and rounded to float once
-                x[0] = (float)((double)(-b0) + (double)z * 2.0 * (double)b0);
               }
               if constexpr (MODEL == SSM_MODEL_FULL_DDM) {
                 // trial-to-trial variability, ddm_models.pyx:2294-2313: drift v + sv g,
                 // non-decision time t + 2 (u_t - 1/2) st, start + 2 (u_z - 1/2) sz; the
-                // uniform terms are formed in double (0.5 is a C double)
+                // uniform terms are double expressions in the reference (0.5 is a C double)
                 float gv, ut, uz;
                 if constexpr (REPLAY) {
                   gv = A.replay_normals[g * A.replay_stride];
@@ -377,8 +422,8 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   uz = u32_to_unit(r.w);
                 }
                 vdt[0] = (sp[SSM_P_V] + sp[SSM_P_SV] * gv) * dt;
-                t0 = (float)((double)t0 + (2.0 * ((double)ut - 0.5)) * (double)sp[SSM_P_ST]);
-                x[0] = (float)((double)x[0] + (2.0 * ((double)uz - 0.5)) * (double)sp[SSM_P_SZ]);
+                t0 = add_unit_shift(t0, ut, sp[SSM_P_ST]);
+                x[0] = add_unit_shift(x[0], uz, sp[SSM_P_SZ]);
               }
             } else {
               leak = sp[SSM_P_G];

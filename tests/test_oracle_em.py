@@ -59,6 +59,53 @@ def test_ddm_oracle_matches_wiener_closed_form(v, a, z):
     assert abs(np.mean(rts) - mt) < 4 * se_t + 0.003
 
 
+def test_ddm_start_point_follows_the_reference_c_precision():
+    # ddm_models.pyx:709 `(-1) * b0 + (z * 2 * b0)`: float operands and an int 2, so each C
+    # operation rounds to float: fl(-b0 + fl(fl(2 z) b0)). full_ddm (:2305) writes 2.0, a
+    # double: the exact -b0 + 2 z b0 rounded once. Both against exact rationals.
+    from fractions import Fraction as Q
+
+    from em_cases import round32
+
+    rng = np.random.default_rng(7)
+    z = rng.uniform(0.1, 0.9, 300).astype(F32)
+    b0 = rng.uniform(0.3, 3.0, 300).astype(F32)
+    z[:4] = [0.5, 0.1, 0.9, 0.3]
+    b0[:4] = [1.5, 1.5, 0.7, 2.9]
+    seq = E.ddm_start32(z, b0)
+    dbl = E.ddm_start32(z, b0, double_literal=True)
+    for i in range(z.shape[0]):
+        zq, bq = Q(float(z[i])), Q(float(b0[i]))
+        prod = round32(Q(float(round32(2 * zq))) * bq)
+        assert seq[i] == round32(-bq + Q(float(prod))), i
+        assert dbl[i] == round32(-bq + 2 * zq * bq), i
+    # the forms differ on about half of such pairs, so a mix-up fails the loop above
+    assert 0.3 < np.mean(seq != dbl) < 0.8
+    assert seq[0] == dbl[0] == 0.0
+
+
+def test_ddm_oracle_walk_starts_from_the_float_start_point():
+    # v = 0, s = 0: y stays at its start, so the walk ends at the first boundary index with
+    # b[ix] < |y0|. On sets where that index depends on the start's rounding, the plain DDM
+    # oracle must end where the float start fl(-b0 + fl(2 z b0)) (exact rationals) puts it.
+    from fractions import Fraction as Q
+
+    from em_cases import round32, start_sensitive_ddm_cases
+
+    v, a, z, th, _ = start_sensitive_ddm_cases(2000)
+    assert v.shape[0] >= 50
+    K = v.shape[0]
+    _, ch, steps = E.ddm_flexbound(v, a, z, np.zeros(K, F32), np.full(K, 999, F32), np.zeros(K, F32), n_samples=1,
+                                   delta_t=DT, boundary="angle", bparams={"a": a, "theta": th},
+                                   normals=np.zeros((K, 1), F32))
+    table = E.boundary_table("angle", {"a": a, "theta": th}, K, 20.0, DT)
+    for i in range(K):
+        zq, bq = Q(float(z[i])), Q(float(a[i]))
+        y0 = round32(-bq + Q(float(round32(Q(float(round32(2 * zq))) * bq))))
+        assert steps[i] == int(np.argmax(table[i] < abs(y0))), i
+        assert ch[i] == np.sign(y0), i
+
+
 def test_ddm_oracle_noise_free_crossing_step():
     # s = 0: y_k = y0 + k*v*dt (float32 sums); the walk stops at the first k with y_k > a
     v, a, z = 0.8, 1.0, 0.5
