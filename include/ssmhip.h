@@ -241,6 +241,28 @@ int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin
                      const int64_t* offsets_dev, float max_t, float lb, float eps, int64_t max_samples,
                      float* workspace_dev, float* out_ll_dev, void* stream);
 
+/* ssm_kde_eval_noalloc has no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that evaluate likelihoods call after call, harnesses that forbid
+ * hipMalloc). It computes exactly what ssm_kde_eval computes, with the same
+ * kernels, but its partial sums live in a fixed device array of
+ * SSM_KDE_EVAL_SCRATCH_FLOATS floats (64 MiB) that every GPU the library runs
+ * on holds until the process ends. It allocates nothing.
+ *   n_points  bounds every point index of the group table (the length of
+ *             pts_rt_dev); n_points * ceil(max_samples / SSM_KDE_CHUNK) must
+ *             not exceed SSM_KDE_EVAL_SCRATCH_FLOATS, else SSM_EINVAL.
+ * All calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (other streams, other host threads) race and must be serialised by the
+ * caller. Every partial sum a call reads is written earlier in the same call,
+ * so nothing needs clearing between calls.                                  */
+#define SSM_KDE_EVAL_SCRATCH_FLOATS (16 * 1024 * 1024)
+int32_t ssm_kde_eval_noalloc(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev,
+                             const int32_t* grp_count_dev, const float* pts_rt_dev, const float* samples_dev,
+                             const int32_t* kde_count_dev, const int32_t* choice_total_dev,
+                             const float* bandwidth_dev, const int64_t* kde_start_dev, const int64_t* offsets_dev,
+                             float max_t, float lb, float eps, int64_t max_samples, int64_t n_points,
+                             float* out_ll_dev, void* stream);
+
 /* ssm_lan_assemble replaces the row assembly and shuffle of
  * ResampleMixtureStrategy.generate (resample_mixture_strategy.py:129-214):
  * rows_dev[n_sets*m][n_theta + 3] = (theta_dev[set][:], rt, choice value, ll),

@@ -75,6 +75,11 @@ int32_t filter_workspace_bytes(int32_t n_sets, int64_t n_total, size_t* bytes);
 int32_t filter_stats_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice_values, const float* rts,
                              const void* choices, int32_t choice_bytes, const int64_t* offsets, int64_t n_total,
                              void* ws, size_t ws_bytes, double* stats, hipStream_t stream);
+int32_t kde_eval_noalloc_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin,
+                                 const int32_t* grp_count, const float* pts_rt, const float* samples,
+                                 const int32_t* kde_count, const int32_t* choice_total, const float* bandwidth,
+                                 const int64_t* kde_start, const int64_t* offsets, float max_t, float lb, float eps,
+                                 int64_t max_samples, int64_t n_points, float* out_ll, hipStream_t stream);
 int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
                              const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
                              const int32_t* choice_values, float* rows, hipStream_t stream);
@@ -222,6 +227,17 @@ int32_t ssm_kde_eval(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin
   return kde_eval_enqueue(n_sets, n_choices, grp_begin_dev, grp_count_dev, pts_rt_dev, samples_dev, kde_count_dev,
                           choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, max_t, lb, eps, max_samples,
                           workspace_dev, out_ll_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_kde_eval_noalloc(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin_dev,
+                             const int32_t* grp_count_dev, const float* pts_rt_dev, const float* samples_dev,
+                             const int32_t* kde_count_dev, const int32_t* choice_total_dev,
+                             const float* bandwidth_dev, const int64_t* kde_start_dev, const int64_t* offsets_dev,
+                             float max_t, float lb, float eps, int64_t max_samples, int64_t n_points,
+                             float* out_ll_dev, void* stream) {
+  return kde_eval_noalloc_enqueue(n_sets, n_choices, grp_begin_dev, grp_count_dev, pts_rt_dev, samples_dev,
+                                  kde_count_dev, choice_total_dev, bandwidth_dev, kde_start_dev, offsets_dev, max_t,
+                                  lb, eps, max_samples, n_points, out_ll_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,

@@ -22,6 +22,9 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <mutex>
+#include <vector>
+
 #include "ssmhip_internal.h"
 
 namespace ssmhip {
@@ -597,6 +600,52 @@ int32_t kde_eval_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_b
   hipLaunchKernelGGL(kde_eval_finish_kernel, dim3(n_sets * n_choices), dim3(kEvalThreads), 0, stream, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
+}
+
+// ---- evaluation without allocation ----------------------------------------------------
+// Partial sums of ssm_kde_eval_noalloc: a fixed array in device memory (one copy per GPU,
+// held until the process ends). Host code never passes its name to a kernel: the name is a
+// host-side stand-in, so the device address is looked up once per device with
+// hipGetSymbolAddress (with that device current) and kept.
+static __device__ __align__(256) float scratch[SSM_KDE_EVAL_SCRATCH_FLOATS];
+
+static int32_t scratch_for_current_device(float** out) {
+  static std::mutex mu;
+  static std::vector<float*> per_device;
+  int dev = 0;
+  SSM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+  if (!per_device[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(SSM_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    per_device[dev] = (float*)p;
+  }
+  *out = per_device[dev];
+  return SSM_OK;
+}
+
+int32_t kde_eval_noalloc_enqueue(int32_t n_sets, int32_t n_choices, const int64_t* grp_begin,
+                                 const int32_t* grp_count, const float* pts_rt, const float* samples,
+                                 const int32_t* kde_count, const int32_t* choice_total, const float* bandwidth,
+                                 const int64_t* kde_start, const int64_t* offsets, float max_t, float lb, float eps,
+                                 int64_t max_samples, int64_t n_points, float* out_ll, hipStream_t stream) {
+  if (n_sets < 0 || max_samples < 0 || n_points < 0) return fail(SSM_EINVAL, "negative sizes");
+  const int64_t nch = max_samples > 0 ? (max_samples + SSM_KDE_CHUNK - 1) / SSM_KDE_CHUNK : 1;
+  if (nch > 65535) return fail(SSM_EINVAL, "max_samples too large");
+  if (n_points > (int64_t)SSM_KDE_EVAL_SCRATCH_FLOATS / nch)
+    return fail(SSM_EINVAL, "n_points * ceil(max_samples / 8192) = " + std::to_string(n_points * nch) +
+                                " partial sums exceed the no-allocation scratch of " +
+                                std::to_string((int64_t)SSM_KDE_EVAL_SCRATCH_FLOATS) +
+                                " floats; evaluate fewer points per call or use ssm_kde_eval with a workspace");
+  if (n_sets == 0) return SSM_OK;
+  float* ws = nullptr;
+  const int32_t rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  return kde_eval_enqueue(n_sets, n_choices, grp_begin, grp_count, pts_rt, samples, kde_count, choice_total,
+                          bandwidth, kde_start, offsets, max_t, lb, eps, max_samples, ws, out_ll, stream);
 }
 
 int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,

@@ -31,6 +31,7 @@ P_T0, P_DEADLINE, P_DEADLINE_TMP, P_A, P_B1, P_B2, P_Z, P_V, P_SQRT_ST, P_G, P_B
 P_VJ, P_ZJ, P_SSTJ = 16, 32, 48
 EM_WORKSPACE_WORDS = 4
 KDE_CHUNK = 8192
+KDE_EVAL_SCRATCH_FLOATS = 16 * 1024 * 1024  # SSM_KDE_EVAL_SCRATCH_FLOATS
 
 class EmProblem(C.Structure):
     _fields_ = [
@@ -97,6 +98,11 @@ _SIGS = {
         C.c_int32,
         [C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float,
          C.c_int64, _VP, _VP, _VP],
+    ),
+    "ssm_kde_eval_noalloc": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float,
+         C.c_int64, C.c_int64, _VP, _VP],
     ),
     "ssm_lan_assemble": (
         C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]

@@ -53,3 +53,16 @@ def test_em_rejects_bad_descriptor_before_touching_the_gpu():
     assert "n_particles" in _lib.last_error()
     with pytest.raises(ValueError, match="n_particles"):
         _lib.check(_lib.SSM_EINVAL)
+
+
+def test_kde_eval_noalloc_rejects_sizes_beyond_its_scratch_before_touching_the_gpu():
+    from ssmhip import _lib
+
+    lib = _lib.load()
+    nch = 13  # ceil(1e5 / 8192) partial sums per point
+    rc = lib.ssm_kde_eval_noalloc(1, 2, *([None] * 9), C.c_float(20.0), C.c_float(-66.774), C.c_float(1e-4),
+                                  100_000, _lib.KDE_EVAL_SCRATCH_FLOATS // nch + 1, None, None)
+    assert rc == _lib.SSM_EINVAL
+    assert "no-allocation scratch" in _lib.last_error()
+    assert lib.ssm_kde_eval_noalloc(1, 2, *([None] * 9), C.c_float(20.0), C.c_float(-66.774), C.c_float(1e-4),
+                                    100_000, -1, None, None) == _lib.SSM_EINVAL
