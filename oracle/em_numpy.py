@@ -527,3 +527,28 @@ def lba(v, a, z, t, deadline, sd, *, n_samples, tan_theta=None, uniforms=None, n
     rts = (np.min(x, axis=1) + np.asarray(t, F32).reshape(-1)[lane_set]).astype(F32)
     rts[rts >= np.asarray(deadline, F32).reshape(-1)[lane_set]] = F32(-999.0)
     return rts, choices
+
+
+def lba64(v, a, z, t, deadline, sd, *, n_samples, tan_theta=None, uniforms=None, normals=None):
+    """lba_vanilla / lba_angle in the reference's own precision (src/cssm/lba_models.pyx:101-115,
+    :212-224): NumPy float64 throughout, zs = U(0, z) as z u, vs = |v + sd g|, x = (a - zs) / vs
+    (angle: / (vs + tan(theta)) with float64 tan of the float32 theta), choice = first argmin,
+    rt = min x + t stored to float32 (one rounding), rt >= deadline -> -999. The draws u, g are
+    given (float32[n_total, nact]) so that lba() and the kernel can be held against it."""
+    f8 = np.float64
+    v = np.asarray(v, F32)
+    K, P = v.shape
+    N = int(n_samples)
+    ls = np.repeat(np.arange(K), N)
+    a_ = np.asarray(a, F32).reshape(K, -1)[:, 0].astype(f8)[ls, None]
+    z_ = np.asarray(z, F32).reshape(K, -1)[:, 0].astype(f8)[ls, None]
+    sd_ = np.broadcast_to(np.asarray(sd, F32).reshape(K, -1), (K, P)).astype(f8)[ls]
+    zs = np.asarray(uniforms, F32).astype(f8) * z_
+    vs = np.abs(v.astype(f8)[ls] + sd_ * np.asarray(normals, F32).astype(f8))
+    if tan_theta is not None:
+        vs = vs + np.asarray(tan_theta, f8).reshape(-1)[ls, None]
+    x = (a_ - zs) / vs
+    choices = np.argmin(x, axis=1).astype(np.int32)
+    rts = (np.min(x, axis=1) + np.asarray(t, F32).reshape(-1).astype(f8)[ls]).astype(F32)
+    rts[rts >= np.asarray(deadline, F32).reshape(-1)[ls]] = F32(-999.0)
+    return rts, choices

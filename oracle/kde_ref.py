@@ -4,6 +4,8 @@ Follows ssms/support_utils/kde_class.py:
   fit   _attach_data_from_simulator :438-504 (split by possible choice, drop
         -999, p_c = count_c / N incl. omissions), bandwidth_silverman :508-553,
         fit on log(rt) of the float32 rts (:174-176)
+  sample counts  kde_sample :387-399 (n_c = round(n p_c) on the float64 product, +-1 on the
+        first largest count when the counts do not add up to n)
   eval  __kde_eval_ :251-329 with KernelDensity.score_samples
         (sklearn/neighbors/_kde.py:248-284): exact Gaussian log-sum, no tree
         approximation (rtol = atol = 0 in the reference makes it exact).
@@ -48,6 +50,18 @@ def logsumexp_gauss(x: np.ndarray, samples: np.ndarray, h: float) -> np.ndarray:
     return m + np.log(np.exp(e - m[:, None]).sum(axis=1))
 
 
+def n_by_choice(n_samples: int, p) -> list[int]:
+    """round(n * p_c) per choice, Python's round of the float64 product (half to even on that
+    value); then, if the counts do not add up to n, the first largest count moves by one
+    (kde_class.py:387-399)."""
+    n_by = [round(n_samples * float(pc)) for pc in p]
+    if sum(n_by) > n_samples:
+        n_by[int(np.argmax(n_by))] -= 1
+    elif sum(n_by) < n_samples:
+        n_by[int(np.argmax(n_by))] += 1
+    return n_by
+
+
 class KDEFit:
     def __init__(self, rts, choices, possible_choices):
         rts = np.asarray(rts, np.float32).reshape(-1)
@@ -62,6 +76,10 @@ class KDEFit:
             lr = np.log(keep).astype(np.float64)  # np.log of float32 data, as the reference
             self.logs.append(lr)
             self.h.append(bandwidth_silverman(lr) if lr.size else None)
+
+    def n_by_choice(self, n_samples: int) -> list[int]:
+        """Per-choice KDE sample counts of kde_sample (kde_class.py:387-399)."""
+        return n_by_choice(n_samples, self.p)
 
     def eval(self, rts, choices, max_t=20.0, lb=-66.774, eps=10e-5):
         rts = np.asarray(rts, np.float64).reshape(-1)

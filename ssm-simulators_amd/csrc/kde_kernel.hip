@@ -255,13 +255,14 @@ __global__ void __launch_bounds__(kSampleThreads) kde_sample_kernel(SampleArgs A
   if (threadIdx.x == 0 && A.n_by_override) {
     for (int c = 0; c < C; ++c) n_by[c] = A.n_by_override[set * C + c];
   } else if (threadIdx.x == 0) {
-    // n_c = round(n_kde * p_c), half to even (Python round), computed exactly in integers
+    // n_c = round(n_kde * p_c) as kde_class.py:387-392 forms it: p_c = count / N is a Python
+    // float (kde_class.py:496), the product a float64, and round() goes half to even on that
+    // float64 value. Two scalar float64 operations per choice reproduce its rounding exactly
+    // (an exact-rational rule differs on ties where the float64 product is not exactly .5).
     int sum = 0, arg = 0;
     for (int c = 0; c < C; ++c) {
-      const int64_t q = (int64_t)A.n_kde * A.choice_total[set * C + c];
-      int64_t f = N > 0 ? q / N : 0;
-      const int64_t r2 = N > 0 ? 2 * (q - f * N) : 0;
-      if (r2 > N || (r2 == N && N > 0 && (f & 1))) ++f;
+      const double p = N > 0 ? (double)A.choice_total[set * C + c] / (double)N : 0.0;
+      const int64_t f = (int64_t)rint((double)A.n_kde * p);
       n_by[c] = (int)f;
       sum += (int)f;
       if (n_by[c] > n_by[arg]) arg = c;  // np.argmax: first maximum

@@ -6,8 +6,9 @@ wavefront tolerates before it refills them, and the largest index range a
 wavefront claims from the global queue per atomic — by timing candidate
 shapes on a job the caller supplies (BatchedSimulation passes a bounded
 sample of its own sets, so the sample keeps the workload's mix of short and
-long walks). Results are cached per (device, model, boundary, particles) in memory
-and in a JSON file (SSMHIP_TUNE_CACHE, default ~/.cache/ssmhip/tune.json).
+long walks). Results are cached per (device, model, boundary, particles, replay mode,
+grid length, power-of-two classes of the trajectory and set counts) in memory and in a JSON
+file (SSMHIP_TUNE_CACHE, default ~/.cache/ssmhip/tune.json).
 """
 
 from __future__ import annotations
@@ -35,10 +36,16 @@ def _cache_path() -> Path:
 
 
 def _key(job) -> str:
+    """Cache key: device, model, boundary, particles, replay mode, the grid length (n_steps) and
+    power-of-two classes of the trajectory and set counts, so that a shape tuned on a small job
+    is not reused for a batch a hundred times larger."""
     t = D.torch()
     name = t.cuda.get_device_name() if t.cuda.is_available() else "cpu"
     p = job.prob
-    return f"{name}|m{p.model}|b{p.boundary}|p{p.n_particles}|r{int(p.replay_stride > 0)}|q"
+    n_cls = max(0, int(p.n_total)).bit_length()
+    k_cls = max(0, int(p.n_sets)).bit_length()
+    return (f"{name}|m{p.model}|b{p.boundary}|p{p.n_particles}|r{int(p.replay_stride > 0)}|q"
+            f"|s{p.n_steps}|n{n_cls}|k{k_cls}")
 
 
 def _load_file() -> dict:

@@ -352,7 +352,8 @@ def lca(v, z, g, b, t, s, deadline, delta_t=0.001, max_t=20, n_samples=2000, n_t
                         b=b, replay_normals=replay_normals, replay_uniforms=replay_uniforms)
 
 
-def _lba(name, v, a, z, deadline, sd, t, nact, n_samples, n_trials, max_t, tan_theta, random_state, replay):
+def _lba(name, v, a, z, deadline, sd, t, nact, n_samples, n_trials, max_t, tan_theta, random_state, replay,
+         theta=None):
     import ctypes as C  # noqa: PLC0415
 
     from . import _device as D  # noqa: PLC0415
@@ -381,6 +382,8 @@ def _lba(name, v, a, z, deadline, sd, t, nact, n_samples, n_trials, max_t, tan_t
     choices = ch[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
     minimal = _minimal_meta(name, list(np.arange(0, nact, 1)), n_samples, n_trials)
     params = {"a": a, "z": z, "deadline": deadline, "sd": sd, "t": t}
+    if theta is not None:  # lba_models.pyx:231
+        params = {"a": a, "z": z, "theta": theta, "deadline": deadline, "sd": sd, "t": t}
     meta = _full_meta(minimal, params, {"max_t": max_t, "n_threads": 1},
                       extra={f"v_{i}": v[:, i] for i in range(nact)})
     return {"rts": rts, "choices": choices, "metadata": meta}
@@ -398,4 +401,4 @@ def lba_angle(v, a, z, theta, deadline, sd, t, nact=3, n_samples=2000, n_trials=
     """LBA with an angle-collapsing threshold, closed form (lba_models.pyx:144-250)."""
     th = np.asarray(theta, np.float32).reshape(int(n_trials), -1)[:, 0]
     return _lba("lba_angle", v, a, z, deadline, sd, t, int(nact), int(n_samples), int(n_trials), max_t,
-                np.tan(th).astype(np.float32), random_state, replay)
+                np.tan(th).astype(np.float32), random_state, replay, theta=theta)

@@ -2,9 +2,11 @@
 
 from __future__ import annotations
 
+import math
 from fractions import Fraction
 
 import numpy as np
+from scipy import stats
 
 from oracle import em_numpy as E
 
@@ -58,3 +60,69 @@ def start_sensitive_ddm_cases(n_search: int = 4000, seed: int = 5):
                                uniforms=np.full((n, 3), 0.5, F32), **common)
     d = s0 != s1
     return v[d], a[d], z[d], theta[d], E.ddm_start32(z[d], a[d])
+
+
+def quantile_check(x, y, qs=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)):
+    """The q-quantiles of two samples agree within 3 Monte-Carlo standard errors (order-statistic
+    intervals of +-sqrt(q(1-q)/n) in probability)."""
+    for q in qs:
+        def se(s):
+            d = math.sqrt(q * (1 - q) / s.size)
+            lo, hi = np.quantile(s, [max(q - d, 0), min(q + d, 1)])
+            return (hi - lo) / 2
+
+        tol = 3 * math.hypot(se(x), se(y))
+        assert abs(np.quantile(x, q) - np.quantile(y, q)) <= tol + 1e-6, (q, np.quantile(x, q), np.quantile(y, q), tol)
+
+
+def compare_samples(r_gpu, c_gpu, r_cpu, c_cpu, choices):
+    """Per choice: probabilities within 3 MC SE; with >= 200 draws each, two-sample KS p > 0.01 and
+    the 0.1..0.9 rt quantiles within 3 MC SE (tests/test_parallel_equivalence.py:39-108 of the
+    reference uses the same criteria)."""
+    n1, n2 = r_gpu.size, r_cpu.size
+    for c in choices:
+        p1, p2 = np.mean(c_gpu == c), np.mean(c_cpu == c)
+        se = math.sqrt(p1 * (1 - p1) / n1 + p2 * (1 - p2) / n2)
+        assert abs(p1 - p2) <= 3 * se + 1e-9, (c, p1, p2, se)
+        x, y = r_gpu[c_gpu == c], r_cpu[c_cpu == c]
+        if min(x.size, y.size) >= 200:
+            assert stats.ks_2samp(x, y).pvalue > 0.01, c
+            quantile_check(x, y)
+
+
+def lba_float32_deviation_ok(r32, c32, r64, c64, v, sd, t, uniforms, normals, n_samples, tan32=None, tan64=None,
+                             a=None, z=None):
+    """Whether float32 LBA results (the kernel, em_numpy.lba) stay within the rounding error of a
+    float32 evaluation of the reference's float64 form (em_numpy.lba64) on the same draws.
+
+    x = (a - u z) / (|v + sd g| + tan theta), each float32 operation rounding with unit roundoff
+    e = 2^-24: the numerator is off by at most e (|u z| + |a - u z|), the denominator by
+    e (|sd g| + |v + sd g| + |den|) + |tan32 - tan64|, the quotient by e more, so x carries the
+    relative error num_err / |num| + den_err / |den| + e, which grows where the numerator or the
+    denominator cancels; rt = x + t adds e |rt|. Checked with a factor 2 of slack. Returns the
+    share of differing choices and whether every rt of a matching choice lies within its bound."""
+    K, P = np.asarray(v).shape
+    ls = np.repeat(np.arange(K), int(n_samples))
+    f8 = np.float64
+    e = 2.0 ** -24
+    v64 = np.asarray(v, f8)[ls]
+    sdg = np.broadcast_to(np.asarray(sd, f8).reshape(K, -1), (K, P))[ls] * np.asarray(normals, f8)
+    vs = np.abs(v64 + sdg)
+    den = vs.copy()
+    dtan = np.zeros_like(den)
+    if tan64 is not None:
+        t64 = np.asarray(tan64, f8).reshape(-1)[ls, None]
+        den = den + t64
+        dtan = np.abs(np.asarray(tan32, np.float32).reshape(-1).astype(f8)[ls, None] - t64)
+    a64 = np.asarray(a, f8).reshape(K, -1)[:, 0][ls, None]
+    zs = np.asarray(uniforms, f8) * np.asarray(z, f8).reshape(K, -1)[:, 0][ls, None]
+    num = a64 - zs
+    rel = (e * (np.abs(zs) + np.abs(num)) / np.abs(num) + (e * (np.abs(sdg) + vs + np.abs(den)) + dtan) / np.abs(den)
+           + e)
+    j = np.asarray(c64, np.int64)
+    rows = np.arange(j.size)
+    x64 = num[rows, j] / den[rows, j]
+    bound = 2 * (np.abs(x64) * rel[rows, j] + e * np.abs(np.asarray(r64, f8)))
+    ok = (np.asarray(c32) == j) & (np.asarray(r32) != -999.0) & (np.asarray(r64) != -999.0)
+    err = np.abs(np.asarray(r32, f8) - np.asarray(r64, f8))
+    return float(np.mean(np.asarray(c32) != j)), bool(np.all(err[ok] <= bound[ok]))

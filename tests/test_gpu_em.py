@@ -14,6 +14,8 @@ import numpy as np
 import pytest
 from scipy import stats
 
+from em_cases import compare_samples as _compare_samples
+from em_cases import quantile_check as _quantile_check
 from oracle import cpu_baseline, ddm_analytic
 from oracle import em_numpy as E
 from oracle import philox as PH
@@ -300,29 +302,6 @@ def test_replay_is_independent_of_queue_order_on_ragged_sets(torch_gpu):
 
 
 # ----------------------------------------------------------------------------- native RNG statistics
-def _quantile_check(x, y, qs=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)):
-    for q in qs:
-        def se(s):
-            d = math.sqrt(q * (1 - q) / s.size)
-            lo, hi = np.quantile(s, [max(q - d, 0), min(q + d, 1)])
-            return (hi - lo) / 2
-
-        tol = 3 * math.hypot(se(x), se(y))
-        assert abs(np.quantile(x, q) - np.quantile(y, q)) <= tol + 1e-6, (q, np.quantile(x, q), np.quantile(y, q), tol)
-
-
-def _compare_samples(r_gpu, c_gpu, r_cpu, c_cpu, choices):
-    n1, n2 = r_gpu.size, r_cpu.size
-    for c in choices:
-        p1, p2 = np.mean(c_gpu == c), np.mean(c_cpu == c)
-        se = math.sqrt(p1 * (1 - p1) / n1 + p2 * (1 - p2) / n2)
-        assert abs(p1 - p2) <= 3 * se + 1e-9, (c, p1, p2, se)
-        x, y = r_gpu[c_gpu == c], r_cpu[c_cpu == c]
-        if min(x.size, y.size) >= 200:
-            assert stats.ks_2samp(x, y).pvalue > 0.01, c
-            _quantile_check(x, y)
-
-
 def test_ddm_native_single_set_1e5_matches_cpu_restatement_and_closed_form(torch_gpu):
     import torch
 
@@ -355,6 +334,48 @@ def test_ddm_native_single_set_1e5_matches_cpu_restatement_and_closed_form(torch
         exp_ = pdf_mass * obs.sum()
         chi2 = np.sum((obs - exp_) ** 2 / exp_)
         assert stats.chi2.sf(chi2, df=19) > 1e-3, (c, chi2)
+
+
+@pytest.mark.parametrize("bname", ["angle", "weibull_cdf"])
+def test_ddm_native_in_register_bounds_match_cpu_restatement(torch_gpu, bname):
+    """Native Philox walks with the angle or Weibull bound evaluated in registers, against the
+    C++/OpenMP restatement of the parallel branch (MT19937 + Ziggurat walking a NumPy boundary
+    table) at 1e5 samples per set. Set 0 has opening bounds (theta < 0) or a slow collapse and a
+    small drift, so a share of its walks runs to max_t."""
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    N, K = 100_000, 3
+    v = np.array([0.1, 1.5, -0.8], F32)
+    z = np.array([0.5, 0.4, 0.6], F32)
+    t = np.array([0.3, 0.1, 0.5], F32)
+    if bname == "angle":
+        a = np.array([2.0, 1.2, 1.5], F32)
+        bp = {"a": a, "theta": np.array([-0.1, 1.2, 0.4], F32)}
+    else:
+        a = np.array([5.0, 1.2, 1.5], F32)
+        bp = {"a": a, "alpha": np.array([0.5, 3.0, 1.5], F32), "beta": np.array([200.0, 1.0, 3.0], F32)}
+    print(f"[case] ddm native {bname} vs cpu restatement", flush=True)
+    fun = {"angle": BF.angle, "weibull_cdf": BF.weibull_cdf}[bname]
+    code = BF.kernel_boundary_code(fun)
+    assert code != L.BOUNDARY_TABLE
+    dl = np.full(K, 999.0, F32)
+    rec = em.ddm_records(v, a, z, t, dl, np.ones(K, F32), 20.0, DT, code, bp)
+    job = em.make_job(L.MODEL_DDM, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=20.0, seed=4,
+                      smooth_unif=False)
+    job.launch()
+    torch.cuda.synchronize()
+    rg, cg = job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy()
+    table = E.boundary_table(bname, bp, K, 20.0, DT)
+    dl_tmp = np.minimum(F32(20.0), (dl - t).astype(F32)).astype(F32)
+    rc_, cc_, _ = cpu_baseline.em_simulate(0, v, z, np.full(K, E.sqrt_dt32(DT)), t, dl, dl_tmp, table,
+                                           np.arange(K + 1, dtype=np.int64) * N, dt=DT, seed=31)
+    assert np.mean(rc_[:N] > 20.0) > 0.01 and np.mean(rg[:N] > 20.0) > 0.01  # walks that reach max_t
+    for k in range(K):
+        s_ = slice(k * N, (k + 1) * N)
+        _compare_samples(rg[s_], cg[s_], rc_[s_], cc_[s_], (-1, 1))
 
 
 @pytest.mark.parametrize("model", ["race", "lca"])

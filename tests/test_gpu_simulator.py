@@ -125,6 +125,53 @@ def test_graph_replay_matches_eager_launch(torch_gpu):
     sim.close()
 
 
+def test_config2_shaped_autotuned_graph_replay_matches_oracle(torch_gpu):
+    """1024 ragged angle sets drawn from the batched angle workload's parameter bounds, in one
+    launch whose shape the autotuner picks on the batch itself, captured in a HIP graph and
+    replayed twice (with unrelated eager work between): every trajectory of every set equals
+    the oracle's in replay mode (per-set n_i = floor(10^U(0, log10 40)) keeps the oracle cheap)."""
+    import torch
+
+    from oracle import em_numpy as E
+    from ssmhip import batched
+
+    K, NMAX, STRIDE = 1024, 40, 257
+    theta, ns = batched.sample_angle_ragged(K, seed=12, lo=1, hi=NMAX)
+    ns = np.minimum(ns, NMAX)
+    offs = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    total = int(offs[-1])
+    print(f"[case] config-2-shaped graph replay: {K} sets, {total} trajectories", flush=True)
+    rng = np.random.default_rng(13)
+    normals = rng.standard_normal((total, STRIDE)).astype(np.float32)
+    unif = rng.uniform(size=total).astype(np.float32)
+    sim = batched.BatchedSimulation("angle", theta, ns, seed=1, smooth_unif=True, replay_normals=normals,
+                                    replay_uniforms=unif, launch="auto", tune_sample=None)
+    # the oracle on a uniform NMAX-sample layout; lanes beyond a set's n_i are padding
+    pad_n = np.zeros((K * NMAX, STRIDE), np.float32)
+    pad_u = np.zeros(K * NMAX, np.float32)
+    valid = np.zeros(K * NMAX, bool)
+    for k in range(K):
+        lo = k * NMAX
+        pad_n[lo : lo + ns[k]] = normals[offs[k] : offs[k + 1]]
+        pad_u[lo : lo + ns[k]] = unif[offs[k] : offs[k + 1]]
+        valid[lo : lo + ns[k]] = True
+    v, a, z, t, th = (theta[:, i] for i in range(5))
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, np.float32), np.ones(K, np.float32), n_samples=NMAX,
+                                 delta_t=0.001, boundary="angle", bparams={"a": a, "theta": th}, smooth_unif=True,
+                                 normals=pad_n, uniforms=pad_u)
+    r0, c0 = r0[valid], c0[valid]
+    for rep in range(2):
+        sim.rts.fill_(0)
+        sim.choices.fill_(9)
+        sim.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(sim.choices.cpu().numpy().astype(np.int32), c0), rep
+        assert np.array_equal(sim.rts.cpu().numpy(), r0), rep
+        assert sim.steps() == int(s0[valid].sum())
+        torch.zeros(1000, device="cuda").sum()  # unrelated eager work between replays
+    sim.close()
+
+
 def test_full_ddm_and_ornstein_through_the_simulator(torch_gpu):
     from ssmhip.simulator import Simulator
 

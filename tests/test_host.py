@@ -201,3 +201,18 @@ def test_bench_refuses_more_than_one_gpu():
     bench = Path(__file__).resolve().parents[1] / "bench.py"
     r = subprocess.run([sys.executable, str(bench), "--gpus", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_autotune_cache_key_separates_batch_sizes():
+    from types import SimpleNamespace
+
+    from ssmhip import autotune
+
+    def job(n_sets, n_total, n_steps=20001):
+        return SimpleNamespace(prob=SimpleNamespace(model=0, boundary=1, n_particles=1, replay_stride=0,
+                                                    n_sets=n_sets, n_total=n_total, n_steps=n_steps))
+
+    small, big = autotune._key(job(64, 1_300_000)), autotune._key(job(4096, 86_000_000))
+    assert small != big
+    assert autotune._key(job(64, 1_300_000)) == autotune._key(job(100, 1_500_000))  # same size classes
+    assert autotune._key(job(64, 1_300_000, n_steps=5001)) != small

@@ -161,6 +161,46 @@ def test_lba_closed_form_choice_probability():
     assert np.all(rts > 0)
 
 
+@pytest.mark.parametrize("angle", [False, True])
+def test_lba_float32_transcription_stays_within_ulps_of_the_float64_reference_form(angle):
+    # the kernel computes the LBA in float32 (lba()); the reference in NumPy float64 (lba64()).
+    # On the same draws: identical choices and rts within the float32 error bound
+    # (em_cases.lba_float32_deviation_ok; vanilla: at most 5 ulps measured; angle: more where
+    # vs + tan(theta) cancels and magnifies the float32 rounding of tan(theta)).
+    from em_cases import lba_float32_deviation_ok
+
+    K, N, P = 32, 20000, 3
+    rng = np.random.default_rng(12)
+    v = rng.uniform(0.2, 2.5, (K, P)).astype(F32)
+    a = rng.uniform(0.5, 2.0, K).astype(F32)
+    z = (a * rng.uniform(0.05, 0.95, K)).astype(F32)
+    t = rng.uniform(0.0, 1.0, K).astype(F32)
+    sd = np.full(K, 0.1, F32)
+    th = rng.uniform(-0.1, 0.6, K).astype(F32) if angle else None
+    u = rng.uniform(size=(K * N, P)).astype(F32)
+    g = rng.standard_normal((K * N, P)).astype(F32)
+    kw = dict(n_samples=N, uniforms=u, normals=g)
+    tan32 = None if th is None else np.tan(th).astype(F32)
+    tan64 = None if th is None else np.tan(th.astype(np.float64))
+    r32, c32 = E.lba(v, a, z, t, np.full(K, 999, F32), sd, tan_theta=tan32, **kw)
+    r64, c64 = E.lba64(v, a, z, t, np.full(K, 999, F32), sd, tan_theta=tan64, **kw)
+    share, within = lba_float32_deviation_ok(r32, c32, r64, c64, v, sd, t, u, g, N, tan32, tan64, a=a, z=z)
+    assert share < 1e-5 and within
+    if not angle:
+        assert np.max(np.abs(r32.view(np.int32).astype(np.int64) - r64.view(np.int32))) <= 8
+
+
+def test_lba64_choice_probability_closed_form():
+    # start-point range 0 and v >> sd: choice 0 iff v0 + sd g0 > v1 + sd g1, P = Phi((v0 - v1)/(sd sqrt 2))
+    n = 200000
+    rng = np.random.default_rng(14)
+    v = np.array([[1.0, 0.9]], F32)
+    rts, ch = E.lba64(v, [1.0], [0.0], [0.0], [999.0], [0.1], n_samples=n,
+                      uniforms=rng.uniform(size=(n, 2)).astype(F32), normals=rng.standard_normal((n, 2)).astype(F32))
+    p0 = stats.norm.cdf(0.1 / (0.1 * math.sqrt(2)))
+    assert abs(np.mean(ch == 0) - p0) < 4 * math.sqrt(p0 * (1 - p0) / n)
+
+
 def _ddm_cpu_args(v, a, z, t, s, bname, bparams, K):
     table = E.boundary_table(bname, bparams, K, 20.0, DT)
     dl = np.minimum(F32(20.0), (F32(999.0) - t).astype(F32)).astype(F32)

@@ -60,3 +60,30 @@ def test_floor_and_no_base_data():
     assert out[2] == -66.774  # far tail below the floor
     assert abs(out[3] - (-math.log(4 * 20.0))) < 1e-12  # choice without base data
     assert fit.p[1] == 1.0 and fit.logs[1].size == 3  # omission counted in p, dropped from the data
+
+
+def test_sample_counts_round_the_float64_product():
+    # kde_class.py:387-399: p_c = count / N as a Python float, round(n p_c) on the float64 product.
+    # Where n count / N is exactly k + 1/2 in rationals, the float64 product can land just below
+    # or above it; then the count follows the float, not the rational. Checked against the
+    # float64 product formed from exact rationals (Fraction) and rounded by hand.
+    from fractions import Fraction as Q
+
+    def by_hand(n, c, N):
+        p = Q(c / N)  # the float64 p_c, exactly
+        prod = Q(float(n * p))  # n p_c rounded once to float64 (n * p exact, then float())
+        f = prod.numerator // prod.denominator
+        rem = prod - f
+        return f + (1 if rem > Q(1, 2) or (rem == Q(1, 2) and f % 2 == 1) else 0)
+
+    cases = [(320, 23), (320, 49), (320, 51), (1600, 1), (7, 3), (4000, 2999)]
+    for N, c in cases:
+        hand = [by_hand(800, c, N), by_hand(800, N - c, N)]
+        first_max = int(np.argmax(hand))
+        hand[first_max] += 800 - sum(hand)  # the +-1 fix-up (the sum is off by at most one)
+        assert kde_ref.n_by_choice(800, [c / N, (N - c) / N]) == hand, (N, c)
+    # the rational half-to-even rule differs here: it would give 58 + 742 and 122 + 678
+    assert kde_ref.n_by_choice(800, [23 / 320, 297 / 320]) == [57, 743]
+    assert kde_ref.n_by_choice(800, [49 / 320, 271 / 320]) == [123, 677]
+    # counts that add up to more than n: the first largest loses one (np.argmax)
+    assert kde_ref.n_by_choice(3, [0.5, 0.5]) == [1, 2]
