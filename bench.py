@@ -157,6 +157,46 @@ def accumulator_leg(args):
     return out
 
 
+def ddm_leg(args):
+    """Plain DDM through the drop-in front end: Simulator('ddm').simulate with v=1, a=1.5, z=0.5,
+    t=0.2, 100,000 samples, seed 0 (dt 1e-3, max_t 20, smooth_unif, full metadata), end to end:
+    parameter preprocessing, the launch, the copy of rts and choices to the host, the trajectory
+    of trial 0 and the boundary of the metadata. Median of 5 calls after one warm-up call."""
+    import torch
+
+    from oracle import cpu_baseline
+    from oracle import em_numpy as E
+    from ssmhip.simulator import Simulator
+
+    theta = {"v": 1.0, "a": 1.5, "z": 0.5, "t": 0.2}
+    sim = Simulator("ddm")
+    sim.simulate(theta, n_samples=100_000, random_state=0)
+    times = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = sim.simulate(theta, n_samples=100_000, random_state=0)
+        times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    p_up = float(np.mean(out["choices"] == 1))
+    rec = {"workload": "Simulator('ddm').simulate, v=1, a=1.5, z=0.5, t=0.2, 1e5 samples, seed 0",
+           "seconds_end_to_end": sec, "trajectories_per_s": 100_000 / sec, "p_upper": p_up,
+           "mean_rt": float(np.mean(out["rts"]))}
+    if not args.no_cpu:
+        table = E.boundary_table("constant", {"a": [1.5]}, 1, 20.0, 0.001)
+        t0 = time.perf_counter()
+        cpu_baseline.em_simulate(0, [1.0], [0.5], [E.sqrt_dt32(0.001)], [0.2], [999.0], [20.0], table,
+                                 np.array([0, 100_000], np.int64), dt=0.001, seed=0, smooth_unif=True,
+                                 n_threads=cpu_baseline.n_cores())
+        csec = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": 100_000 / csec, "unit": "trajectories/s", "cores": cpu_baseline.n_cores(),
+                               "kind": "port", "sample": f"the whole workload, 1e5 trajectories in {csec:.3f} s"}
+        rec["gpu_over_cpu"] = rec["trajectories_per_s"] / rec["cpu_baseline"]["value"]
+    log(f"ddm (1 set x 1e5, end to end): {sec * 1e3:.2f} ms, {rec['trajectories_per_s']:.3e} traj/s, "
+        f"P(upper) {p_up:.4f}" + (f", {rec['gpu_over_cpu']:.0f}x CPU" if "gpu_over_cpu" in rec else ""))
+    return rec
+
+
 def hssm_leg(args):
     """HSSM posterior-predictive draws: 1000 posterior draws x 1000 data trials of the angle model
     (per-trial drift), one sample each, through hssm_support.rng_fn; the device-only rate of the
@@ -242,7 +282,11 @@ def lan_leg(args):
         rows = lab.run(timed=True)
         tot.append(lab.times)
     torch.cuda.synchronize()
-    st = min(tot, key=lambda x: x.total_ms)
+    # the mean of the timed passes, stage by stage
+    from ssmhip.lan import StageTimes
+
+    st = StageTimes(*(float(np.mean([getattr(x, f) for x in tot])) for f in
+                      ("simulate_ms", "fit_ms", "sample_ms", "eval_ms", "assemble_ms")))
     n_rows = int(rows.shape[0])
     cnt = lab.bank.kde_count.cpu().numpy().astype(np.int64)
     gcount = lab.points.grp_count.cpu().numpy().astype(np.int64)
@@ -296,7 +340,7 @@ def lan_leg(args):
     return rec, lab, rows
 
 
-def surrogate_leg(args, lab, kde_evals_per_s):
+def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
     """Surrogate: label rows of --surrogate-sets angle sets, train, then 4M-row inference against
     the GPU KDE and accuracy against the analytic DDM density."""
     import torch
@@ -380,8 +424,20 @@ def surrogate_leg(args, lab, kde_evals_per_s):
     xs_parts = xs
     xs = np.concatenate(xs)
     ref = np.concatenate(ref)
-    pred = net(torch.from_numpy(xs).cuda()).cpu().numpy()
+    dxs = torch.from_numpy(xs).cuda()
+    pred = net(dxs).cpu().numpy()
     mae = float(np.mean(np.abs(pred - ref)))
+    # the same network trained on the LAN leg's own rows (1024 sets x 1e5 simulations, 800 KDE +
+    # 200 uniform rows per set: the labelling workload as configured, 1,024,000 rows)
+    mae_c4 = None
+    if config4_rows is not None:
+        n4 = config4_rows.shape[0]
+        perm4 = torch.randperm(n4, device="cuda", generator=torch.Generator(device="cuda").manual_seed(1))
+        net4 = MLPSurrogate(seed=0)
+        net4.fit(config4_rows.index_select(0, perm4[n4 // 50:]), epochs=args.surrogate_epochs, batch=8192, lr=3e-3,
+                 lr_final=5e-5, val_rows=config4_rows.index_select(0, perm4[: n4 // 50]))
+        mae_c4 = float(np.mean(np.abs(net4(dxs).cpu().numpy() - ref)))
+        del net4, perm4
     # the KDE labels' own error at the same points (simulated KDE per parameter set, 1e5 draws)
     from ssmhip.kde import KDEBank, PointBatch
 
@@ -407,7 +463,8 @@ def surrogate_leg(args, lab, kde_evals_per_s):
            "val_mae_vs_kde_above_floor": net.val_mae, "val_mae_vs_kde_all": net.val_mae_all, "inference_rows": N, "ms_per_inference": ms, "evals_per_s": evals,
            "over_gpu_kde": evals / kde_evals_per_s,
            "mfma_tflops_algorithmic": flops / 1e12, "mfma_frac_of_bf16_peak": flops / 2.5e15,
-           "mae_vs_analytic_theta0": mae, "kde_mae_vs_analytic_theta0": kde_mae, "analytic_rows": int(ref.size)}
+           "mae_vs_analytic_theta0": mae, "kde_mae_vs_analytic_theta0": kde_mae, "analytic_rows": int(ref.size),
+           "mae_vs_analytic_theta0_trained_on_1024x1e5_rows": mae_c4}
     rec["inference_published_peak"] = {
         "mfma_bf16": {"unit": "TFLOP/s", "achieved": flops / 1e12, "peak": 2500.0, "frac": flops / 2.5e15,
                       "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 32.7% of SIMD cycles "
@@ -417,7 +474,7 @@ def surrogate_leg(args, lab, kde_evals_per_s):
                            "basis": "708 v_exp_f32 + v_rcp_f32 per row (counters)"},
     }
     log(f"surrogate: {evals:.3e} evals/s ({rec['over_gpu_kde']:.0f}x GPU KDE), MAE vs analytic {mae:.3f} nats "
-        f"(KDE labels: {kde_mae:.3f})")
+        f"(KDE labels: {kde_mae:.3f}; trained on the 1024 x 1e5 rows: {mae_c4})")
     return rec
 
 
@@ -434,7 +491,7 @@ def main() -> int:
     ap.add_argument("--queue-order", choices=("expected", "index"), default="expected",
                     help="visiting order of the sets: longest expected walks first, or set index order")
     ap.add_argument("--headline-only", action="store_true", help="run the headline leg only")
-    ap.add_argument("--legs", type=str, default="headline,accumulators,hssm,lan,surrogate",
+    ap.add_argument("--legs", type=str, default="headline,ddm,accumulators,hssm,lan,surrogate",
                     help="comma list of legs to run (surrogate needs lan); without 'headline' the "
                          "JSON line carries no headline value (for profiling one leg)")
     ap.add_argument("--surrogate-epochs", type=int, default=150)
@@ -456,15 +513,17 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     legs = {"headline"} if args.headline_only else set(args.legs.split(","))
-    unknown = legs - {"headline", "accumulators", "hssm", "lan", "surrogate"}
+    unknown = legs - {"headline", "ddm", "accumulators", "hssm", "lan", "surrogate"}
     if unknown or ("surrogate" in legs and "lan" not in legs):
-        print(f"bad --legs {args.legs!r}: legs are headline, accumulators, hssm, lan, surrogate (surrogate needs "
+        print(f"bad --legs {args.legs!r}: legs are headline, ddm, accumulators, hssm, lan, surrogate (surrogate needs "
               f"lan)", file=sys.stderr)
         return 2
     out = {"metric": "angle-DDM trajectories/s", "value": None, "unit": "trajectories/s", "n_gpus": 1,
            "legs": sorted(legs)}
     if "headline" in legs:
         out = headline_leg(args, torch, batched, roofline)
+    if "ddm" in legs:
+        out["ddm"] = ddm_leg(args)
     if "accumulators" in legs:
         out["accumulators"] = accumulator_leg(args)
     if "hssm" in legs:
@@ -473,8 +532,8 @@ def main() -> int:
         c4, lab, rows = lan_leg(args)
         out["lan_rows"] = c4
         if "surrogate" in legs:
-            del rows
-            out["surrogate"] = surrogate_leg(args, lab, c4["kde_evals_per_s"])
+            out["surrogate"] = surrogate_leg(args, lab, c4["kde_evals_per_s"], rows)
+        del rows
     print(json.dumps(out), flush=True)
     return 0
 

@@ -166,6 +166,20 @@ int32_t ssm_em_simulate(const ssm_em_problem* prob, const ssm_launch_config* lau
                         const float* replay_uniforms_dev, float* rts_dev,
                         void* choices_dev, uint64_t* workspace_dev, void* stream);
 
+/* Replaces the trajectory recording of the sequential loops (metadata['trajectory']:
+ * ddm_models.pyx:714-716, :726-728 and the other single-walk diffusions; race_models.pyx:
+ * 433-436, :449-452 (race), :841-844, :865-868 (lca), :1223-1226, :1250-1254 (racing
+ * diffusion)): walks trajectory 0 of set 0 of the same problem (same draws, so the same walk
+ * as in ssm_em_simulate) and writes its positions traj_dev[ix][j] (float32[traj_rows]
+ * [n_particles]) for ix = 0 (the start) up to its last step; rows after it are left as the
+ * caller filled them (the reference fills -999). Set 0 must hold at least one sample.
+ * Two launches on `stream` (workspace reset, one wave). SSM_EUNSUPPORTED for the two-stage
+ * models, whose reference trajectories also record both low walks.                    */
+int32_t ssm_em_trajectory(const ssm_em_problem* prob, const float* set_params_dev, const int64_t* offsets_dev,
+                          const float* table_dev, const float* replay_normals_dev,
+                          const float* replay_uniforms_dev, float* traj_dev, int32_t traj_rows,
+                          uint64_t* workspace_dev, void* stream);
+
 /* Capture the two launches of ssm_em_simulate into a HIP graph executable.
  * The pointers are baked into the graph; replay with ssm_graph_launch.      */
 int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config* launch,

@@ -146,6 +146,21 @@ def levy_scale32(s, alpha, delta_t):
     return (s64 * np.power(np.float64(F32(delta_t)), 1.0 / a64)).astype(F32)
 
 
+class _Trajectory:
+    """Positions of lane 0 (trial 0, sample 0): the start in row 0, then the position after the
+    step that took its step index to ix in row ix; -999 elsewhere (the reference's traj buffer)."""
+
+    def __init__(self, on, n_rows, n_particles, start):
+        self.on = bool(on)
+        self.rows = np.full((n_rows, n_particles), F32(-999.0), F32) if self.on else None
+        if self.on:
+            self.rows[0] = np.asarray(start, F32).reshape(-1)
+
+    def after_step(self, moved, ix, pos0):
+        if self.on and moved.size and moved[0] == 0 and ix[0] < self.rows.shape[0]:
+            self.rows[ix[0]] = np.asarray(pos0, F32).reshape(-1)
+
+
 class _Normals:
     def __init__(self, normals=None, rng=None, n_lanes=0):
         self.normals = None if normals is None else np.asarray(normals, F32)
@@ -169,8 +184,11 @@ def _finish(rts, t_el, t0, dl_tmp, deadline, dt, smooth, uniforms):
 
 def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
                   bparams=None, smooth_unif=False, normals=None, uniforms=None, rng=None, table=None,
-                  sz=None, sv=None, st=None, g=None, alpha=None, legacy=False):
+                  sz=None, sv=None, st=None, g=None, alpha=None, legacy=False, record0=False):
     """Returns (rts float32[n_total], choices int32[n_total], steps int64[n_total]), set-major.
+    With record0: a 4th value, the trajectory of trial 0, sample 0 as the sequential loop
+    records it (ddm_models.pyx:714-716, :726-728): float32[len(t_s), 1], the start in row 0,
+    the position after step ix in row ix, -999 after the walk's end.
 
     With sz/sv/st: full_ddm (src/cssm/ddm_models.pyx:2284-2358): per trajectory the drift
     increment (v + sv gv) dt (float), t_tmp = t + 2 (u_t - 0.5) st and the start shift
@@ -232,6 +250,7 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
     tt = np.zeros(total, F32)
     ix = np.zeros(total, np.int64)
     src = _Normals(normals, rng)
+    traj = _Trajectory(record0 and total > 0, nst, 1, y[:1])
     active = np.arange(total)
     while active.size:
         ks = lane_set[active]
@@ -259,6 +278,7 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
         y[active] = (y[active] + inc).astype(F32)
         tt[active] = (tt[active] + dt).astype(F32)
         ix[active] += 1
+        traj.after_step(active, ix, y[:1])
     if uniforms is None:
         u = np.zeros(total, F32)
     elif full:
@@ -269,13 +289,15 @@ def ddm_flexbound(v, a, z, t, deadline, s, *, n_samples, max_t=20.0, delta_t=0.0
         u = (rng.uniform(0.0, 1.0, total)).astype(F32)
     rts = _finish(None, tt, t_lane, dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
     choices = ((y > 0).astype(np.int32) - (y < 0).astype(np.int32)).astype(np.int32)
-    return rts, choices, ix
+    return (rts, choices, ix, traj.rows) if record0 else (rts, choices, ix)
 
 
 def accumulators(model, v, z, t, s, deadline, *, n_samples, max_t=20.0, delta_t=0.001, boundary="constant",
                  bparams=None, g=None, b=None, smooth_unif=False, normals=None, uniforms=None, rng=None,
-                 table=None):
-    """race_model (model='race') or lca (model='lca'); returns (rts, choices, steps), set-major."""
+                 table=None, record0=False):
+    """race_model (model='race') or lca (model='lca'); returns (rts, choices, steps), set-major.
+    With record0: a 4th value, the trajectory of trial 0, sample 0 (race_models.pyx:433-452,
+    :841-868): float32[len(t_s), P], the starts in row 0, positions after step ix in row ix."""
     v = np.asarray(v, F32)
     K, P = v.shape
     z = np.broadcast_to(np.asarray(z, F32), (K, P))
@@ -298,6 +320,7 @@ def accumulators(model, v, z, t, s, deadline, *, n_samples, max_t=20.0, delta_t=
     tt = np.zeros(total, F32)
     ix = np.zeros(total, np.int64)
     src = _Normals(normals, rng)
+    traj = _Trajectory(record0 and total > 0, nst, P, x[0])
     active = np.arange(total)
     while active.size:
         ks = lane_set[active]
@@ -327,23 +350,25 @@ def accumulators(model, v, z, t, s, deadline, *, n_samples, max_t=20.0, delta_t=
         x[active] = np.maximum(F32(0.0), xn)
         tt[active] = (tt[active] + dt).astype(F32)
         ix[active] += 1
+        traj.after_step(active, ix, x[0])
     u = np.zeros(total, F32) if uniforms is None else np.asarray(uniforms, F32).reshape(-1)
     if smooth_unif and uniforms is None:
         u = (rng.uniform(0.0, 1.0, total)).astype(F32)
     rts = _finish(None, tt, t[lane_set], dl_tmp_k[lane_set], deadline[lane_set], dt, smooth_unif, u)
     choices = np.argmax(x, axis=1).astype(np.int32)
-    return rts, choices, ix
+    return (rts, choices, ix, traj.rows) if record0 else (rts, choices, ix)
 
 
 def racing_diffusion(v, A, b, t, s, deadline, *, n_samples, max_t=20.0, delta_t=0.001, smooth_unif=False,
-                     normals=None, uniforms=None, rng=None):
+                     normals=None, uniforms=None, rng=None, record0=False):
     """racing_diffusion_model, sequential path (src/cssm/race_models.pyx:1150-1280):
     accumulators start at u_j A (:1207-1208); each step moves them in order by
     (v_j dt) + (sqrt_st g) and the first to reach b wins at once, before the rest move and
     before t advances (:1218-1237); rt = t + t0 + smooth_u, and past the deadline or without
     a winner (-999, -1) (:1248-1255). Replay: normals[i, (step * P + j) % stride];
     uniforms[i] = (u_0 .. u_{P-1}, u_smooth). Returns (rts, choices, steps) set-major, steps
-    counting the winning step."""
+    counting the winning step; with record0 also the trajectory of trial 0, sample 0 (starts in
+    row 0, positions after every step without a winner, race_models.pyx:1223-1254)."""
     v = np.asarray(v, F32)
     K, P = v.shape
     A = np.asarray(A, F32).reshape(K, -1)[:, 0]
@@ -367,6 +392,7 @@ def racing_diffusion(v, A, b, t, s, deadline, *, n_samples, max_t=20.0, delta_t=
     ix = np.zeros(total, np.int64)
     win = np.full(total, -1, np.int64)
     src = _Normals(normals, rng)
+    traj = _Trajectory(record0 and total > 0, n_steps(max_t, delta_t), P, x[0])
     active = np.arange(total)
     while active.size:
         ks = lane_set[active]
@@ -387,6 +413,7 @@ def racing_diffusion(v, A, b, t, s, deadline, *, n_samples, max_t=20.0, delta_t=
         cont = active[~found]
         tt[cont] = (tt[cont] + dt).astype(F32)
         ix[cont] += 1
+        traj.after_step(cont, ix, x[0])
         active = cont
     su = smooth_unif32(tt, dl_tmp_k[lane_set], dt, uu[:, P]) if smooth_unif else np.zeros(total, F32)
     rt = ((tt + t[lane_set]).astype(F32) + su).astype(F32)
@@ -395,7 +422,7 @@ def racing_diffusion(v, A, b, t, s, deadline, *, n_samples, max_t=20.0, delta_t=
     bad = (rt == F32(-999.0)) | (win < 0)
     rt[bad] = F32(-999.0)
     ch = np.where(bad, -1, win).astype(np.int32)
-    return rt, ch, ix + (win >= 0)
+    return (rt, ch, ix + (win >= 0), traj.rows) if record0 else (rt, ch, ix + (win >= 0))
 
 
 def _walk(y, tt, ix, moving, vdt, sst, B, lane_set, dl_tmp, src, draw_base, dt):

@@ -43,6 +43,9 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
 int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const float* set_params,
                    const int64_t* offsets, const float* table, const float* rn, const float* ru,
                    float* rts, void* choices, uint64_t* ws, hipStream_t stream);
+int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
+                              const float* table, const float* rn, const float* ru, float* traj, int32_t traj_rows,
+                              uint64_t* ws, hipStream_t stream);
 int32_t lba_enqueue(int32_t nact, int32_t n_sets, int64_t n_total, int32_t out_layout, int32_t choice_bytes,
                     uint64_t seed, const float* params, const int64_t* offsets, const float* replay,
                     float* rts, void* choices, hipStream_t stream);
@@ -132,6 +135,13 @@ int32_t ssm_em_simulate(const ssm_em_problem* prob, const ssm_launch_config* lau
   if (!workspace_dev) return fail(SSM_EINVAL, "workspace is NULL");
   return em_enqueue(prob, launch, set_params_dev, offsets_dev, table_dev, replay_normals_dev, replay_uniforms_dev,
                     rts_dev, choices_dev, workspace_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_em_trajectory(const ssm_em_problem* prob, const float* set_params_dev, const int64_t* offsets_dev,
+                          const float* table_dev, const float* replay_normals_dev, const float* replay_uniforms_dev,
+                          float* traj_dev, int32_t traj_rows, uint64_t* workspace_dev, void* stream) {
+  return em_trajectory_enqueue(prob, set_params_dev, offsets_dev, table_dev, replay_normals_dev, replay_uniforms_dev,
+                               traj_dev, traj_rows, workspace_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config* launch,

@@ -35,6 +35,7 @@ struct EmArgs {
   const float* replay_uniforms;
   float* rts;
   void* choices;
+  float* traj;        // trajectory recording: positions of walk (set 0, sample 0), [traj_rows][n_particles]
   unsigned long long* ws;  // [0] queue head, [1] executed steps, [2] lane-step slots issued
   int64_t total;
   int64_t replay_stride;
@@ -47,6 +48,7 @@ struct EmArgs {
   int32_t choice_bytes;
   int32_t refill_min;
   int32_t queue_block;  // largest index range a wave claims from the global queue at once
+  int32_t traj_rows;
   float dt;
   uint32_t k0, k1;
 };
@@ -170,6 +172,11 @@ template <bool REPLAY>
 constexpr int kSubBlocks = REPLAY ? 2 : 8;
 template <bool REPLAY, int MAXP>
 constexpr int kAccSteps = (REPLAY || MAXP > 4) ? 1 : 8;
+// Trajectory recording (RECORD) runs one walk; its kernels keep the shortest loop bodies.
+template <bool REPLAY, bool RECORD>
+constexpr int kSubBlocksR = RECORD ? 1 : kSubBlocks<REPLAY>;
+template <bool REPLAY, int MAXP, bool RECORD>
+constexpr int kAccStepsR = RECORD ? 1 : kAccSteps<REPLAY, MAXP>;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -196,7 +203,11 @@ __device__ __forceinline__ float stable_cms(uint32_t bu, uint32_t bw, float alph
   return x == x ? x : 0.0f;
 }
 
-template <int MODEL, int BOUND, bool REPLAY, int MAXP>
+// RECORD: the launch runs walk (set 0, sample 0) only and stores its positions, start included,
+// after every step as the reference's sequential loops do for metadata['trajectory']
+// (ddm_models.pyx:714-716, :726-728; race_models.pyx:433-436, :449-452, :1223-1226, :1250-1254);
+// it writes no rts or choices.
+template <int MODEL, int BOUND, bool REPLAY, int MAXP, bool RECORD>
 __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   const int P = kDiffusion<MODEL> ? 1 : A.n_particles;
   const float dt = A.dt;
@@ -287,7 +298,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             if (j < P && x[j] > best) { best = x[j]; choice = j; }
         }
         const int64_t o = A.out_layout == 0 ? g : (int64_t)n * A.n_sets + k;
-        if (o < A.total) {  // layout 1 assumes uniform offsets; never write past the buffers
+        if (!RECORD && o < A.total) {  // layout 1 assumes uniform offsets; never write past the buffers
           A.rts[o] = rt;
           if constexpr (MODEL == SSM_MODEL_PAR2) {  // rts_high, rts_low (parallel_models.pyx:539-540)
             A.rts[A.total + o] = t_h + t0;
@@ -465,6 +476,13 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             t = 0.f;
             ix = 0;
             state = kRunning;
+            if constexpr (RECORD) {
+              if (g == 0 && A.traj_rows > 0) {
+#pragma unroll
+                for (int j = 0; j < MAXP; ++j)
+                  if (j < P) A.traj[j] = x[j];
+              }
+            }
           }
         }
       }
@@ -481,7 +499,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
       // every walk starts block-aligned, so the draws do not depend on the grouping
       bool alive = state == kRunning;
 #pragma unroll
-      for (int sb = 0; sb < kSubBlocks<REPLAY>; ++sb) {
+      for (int sb = 0; sb < kSubBlocksR<REPLAY, RECORD>; ++sb) {
         if (sb > 0) {
           if (__ballot(alive) == 0) break;  // wave-uniform: no lane left to step
           ++iters;
@@ -522,6 +540,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   ++ix;
+                  if constexpr (RECORD) {
+                    if (g == 0 && ix < A.traj_rows) A.traj[ix] = x[0];
+                  }
                 } else {
                   alive = false;
                 }
@@ -540,6 +561,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   ++ix;
+                  if constexpr (RECORD) {
+                    if (g == 0 && ix < A.traj_rows) A.traj[ix] = x[0];
+                  }
                 } else {
                   alive = false;
                 }
@@ -595,7 +619,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     } else {
       // up to kAccSteps steps per loop iteration (the loop's own work is paid once for them)
 #pragma unroll
-      for (int sb = 0; sb < kAccSteps<REPLAY, MAXP>; ++sb) {
+      for (int sb = 0; sb < kAccStepsR<REPLAY, MAXP, RECORD>; ++sb) {
         if (sb > 0) {
           if (__ballot(state == kRunning) == 0) break;  // wave-uniform: no lane left to step
           ++iters;
@@ -675,6 +699,13 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             if (MODEL != SSM_MODEL_RDM || win < 0) {
               t += dt;
               ++ix;
+              if constexpr (RECORD) {
+                if (g == 0 && ix < A.traj_rows) {
+#pragma unroll
+                  for (int j = 0; j < MAXP; ++j)
+                    if (j < P) A.traj[(int64_t)ix * P + j] = x[j];
+                }
+              }
             }
           }
         }
@@ -698,45 +729,60 @@ __global__ void ws_reset_kernel(unsigned long long* ws) {
 // --------------------------------------------------------------------------------
 using KernelFn = void (*)(EmArgs);
 
-template <int MODEL, int BOUND, bool REPLAY>
+template <int MODEL, int BOUND, bool REPLAY, bool RECORD>
 static KernelFn pick_p(int p) {
   if constexpr (kDiffusion<MODEL>) {
-    return em_kernel<MODEL, BOUND, REPLAY, 1>;
+    return em_kernel<MODEL, BOUND, REPLAY, 1, RECORD>;
   } else {
-    if (p <= 4) return em_kernel<MODEL, BOUND, REPLAY, 4>;
-    if (p <= 8) return em_kernel<MODEL, BOUND, REPLAY, 8>;
-    return em_kernel<MODEL, BOUND, REPLAY, 16>;
+    if (p <= 4) return em_kernel<MODEL, BOUND, REPLAY, 4, RECORD>;
+    if (p <= 8) return em_kernel<MODEL, BOUND, REPLAY, 8, RECORD>;
+    return em_kernel<MODEL, BOUND, REPLAY, 16, RECORD>;
   }
 }
 
-template <int MODEL, bool REPLAY>
+template <int MODEL, bool REPLAY, bool RECORD>
 static KernelFn pick_bound(int bound, int p) {
   switch (bound) {
-    case SSM_BOUNDARY_CONSTANT: return pick_p<MODEL, SSM_BOUNDARY_CONSTANT, REPLAY>(p);
-    case SSM_BOUNDARY_ANGLE: return pick_p<MODEL, SSM_BOUNDARY_ANGLE, REPLAY>(p);
-    case SSM_BOUNDARY_WEIBULL: return pick_p<MODEL, SSM_BOUNDARY_WEIBULL, REPLAY>(p);
-    default: return pick_p<MODEL, SSM_BOUNDARY_TABLE, REPLAY>(p);
+    case SSM_BOUNDARY_CONSTANT: return pick_p<MODEL, SSM_BOUNDARY_CONSTANT, REPLAY, RECORD>(p);
+    case SSM_BOUNDARY_ANGLE: return pick_p<MODEL, SSM_BOUNDARY_ANGLE, REPLAY, RECORD>(p);
+    case SSM_BOUNDARY_WEIBULL: return pick_p<MODEL, SSM_BOUNDARY_WEIBULL, REPLAY, RECORD>(p);
+    default: return pick_p<MODEL, SSM_BOUNDARY_TABLE, REPLAY, RECORD>(p);
   }
+}
+
+template <int MODEL, bool RECORD>
+static KernelFn pick_mode(int bound, int p, bool replay) {
+  return replay ? pick_bound<MODEL, true, RECORD>(bound, p) : pick_bound<MODEL, false, RECORD>(bound, p);
 }
 
 static KernelFn pick_kernel(int model, int bound, int p, bool replay) {
-  if (model == SSM_MODEL_DDM) return replay ? pick_bound<SSM_MODEL_DDM, true>(bound, p) : pick_bound<SSM_MODEL_DDM, false>(bound, p);
-  if (model == SSM_MODEL_FULL_DDM)
-    return replay ? pick_bound<SSM_MODEL_FULL_DDM, true>(bound, p) : pick_bound<SSM_MODEL_FULL_DDM, false>(bound, p);
-  if (model == SSM_MODEL_ORNSTEIN)
-    return replay ? pick_bound<SSM_MODEL_ORNSTEIN, true>(bound, p) : pick_bound<SSM_MODEL_ORNSTEIN, false>(bound, p);
-  if (model == SSM_MODEL_LEVY)
-    return replay ? pick_bound<SSM_MODEL_LEVY, true>(bound, p) : pick_bound<SSM_MODEL_LEVY, false>(bound, p);
-  if (model == SSM_MODEL_DDM_LEGACY)
-    return replay ? pick_bound<SSM_MODEL_DDM_LEGACY, true>(bound, p) : pick_bound<SSM_MODEL_DDM_LEGACY, false>(bound, p);
-  if (model == SSM_MODEL_PAR2)
-    return replay ? pick_bound<SSM_MODEL_PAR2, true>(bound, p) : pick_bound<SSM_MODEL_PAR2, false>(bound, p);
-  if (model == SSM_MODEL_SEQ2)
-    return replay ? pick_bound<SSM_MODEL_SEQ2, true>(bound, p) : pick_bound<SSM_MODEL_SEQ2, false>(bound, p);
-  if (model == SSM_MODEL_RDM)
-    return replay ? pick_bound<SSM_MODEL_RDM, true>(bound, p) : pick_bound<SSM_MODEL_RDM, false>(bound, p);
-  if (model == SSM_MODEL_RACE) return replay ? pick_bound<SSM_MODEL_RACE, true>(bound, p) : pick_bound<SSM_MODEL_RACE, false>(bound, p);
-  return replay ? pick_bound<SSM_MODEL_LCA, true>(bound, p) : pick_bound<SSM_MODEL_LCA, false>(bound, p);
+  switch (model) {
+    case SSM_MODEL_DDM: return pick_mode<SSM_MODEL_DDM, false>(bound, p, replay);
+    case SSM_MODEL_FULL_DDM: return pick_mode<SSM_MODEL_FULL_DDM, false>(bound, p, replay);
+    case SSM_MODEL_ORNSTEIN: return pick_mode<SSM_MODEL_ORNSTEIN, false>(bound, p, replay);
+    case SSM_MODEL_LEVY: return pick_mode<SSM_MODEL_LEVY, false>(bound, p, replay);
+    case SSM_MODEL_DDM_LEGACY: return pick_mode<SSM_MODEL_DDM_LEGACY, false>(bound, p, replay);
+    case SSM_MODEL_PAR2: return pick_mode<SSM_MODEL_PAR2, false>(bound, p, replay);
+    case SSM_MODEL_SEQ2: return pick_mode<SSM_MODEL_SEQ2, false>(bound, p, replay);
+    case SSM_MODEL_RDM: return pick_mode<SSM_MODEL_RDM, false>(bound, p, replay);
+    case SSM_MODEL_RACE: return pick_mode<SSM_MODEL_RACE, false>(bound, p, replay);
+    default: return pick_mode<SSM_MODEL_LCA, false>(bound, p, replay);
+  }
+}
+
+// trajectory-recording kernels: every model but the two-stage ones
+static KernelFn pick_record_kernel(int model, int bound, int p, bool replay) {
+  switch (model) {
+    case SSM_MODEL_DDM: return pick_mode<SSM_MODEL_DDM, true>(bound, p, replay);
+    case SSM_MODEL_FULL_DDM: return pick_mode<SSM_MODEL_FULL_DDM, true>(bound, p, replay);
+    case SSM_MODEL_ORNSTEIN: return pick_mode<SSM_MODEL_ORNSTEIN, true>(bound, p, replay);
+    case SSM_MODEL_LEVY: return pick_mode<SSM_MODEL_LEVY, true>(bound, p, replay);
+    case SSM_MODEL_DDM_LEGACY: return pick_mode<SSM_MODEL_DDM_LEGACY, true>(bound, p, replay);
+    case SSM_MODEL_RDM: return pick_mode<SSM_MODEL_RDM, true>(bound, p, replay);
+    case SSM_MODEL_RACE: return pick_mode<SSM_MODEL_RACE, true>(bound, p, replay);
+    case SSM_MODEL_LCA: return pick_mode<SSM_MODEL_LCA, true>(bound, p, replay);
+    default: return nullptr;
+  }
 }
 
 int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
@@ -812,6 +858,8 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   A.replay_uniforms = ru;
   A.rts = rts;
   A.choices = choices;
+  A.traj = nullptr;
+  A.traj_rows = 0;
   A.ws = (unsigned long long*)ws;
   A.total = P->n_total;
   A.replay_stride = P->replay_stride;
@@ -835,6 +883,58 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   if (grid < 1) grid = 1;
   KernelFn fn = pick_kernel(P->model, P->boundary, P->n_particles, P->replay_stride > 0);
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, stream, A);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+// Walk (set 0, sample 0) of a problem once more, one wave of one block, storing its positions:
+// traj[ix][j] for ix = 0 (start) up to the last step, at most traj_rows rows; later rows are left
+// as the caller filled them. The draws are the problem's own (the Philox counter of native mode
+// depends only on (sample, set); replay mode reads row 0), so the walk is the one the full launch
+// produced for that trajectory.
+int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
+                              const float* table, const float* rn, const float* ru, float* traj, int32_t traj_rows,
+                              uint64_t* ws, hipStream_t stream) {
+  if (!traj) return fail(SSM_EINVAL, "traj is NULL");
+  if (traj_rows < 1) return fail(SSM_EINVAL, "traj_rows must be >= 1");
+  // rts / choices are never written by the recording kernels: the trajectory buffer stands in
+  // for them in the shared validation
+  int32_t rc = em_validate(P, set_params, offsets, table, rn, ru, traj, traj, ws);
+  if (rc) return rc;
+  if (!ws) return fail(SSM_EINVAL, "workspace is NULL");
+  if (P->n_total < 1 || P->n_sets < 1) return fail(SSM_EINVAL, "the problem has no trajectory to record");
+  KernelFn fn = pick_record_kernel(P->model, P->boundary, P->n_particles, P->replay_stride > 0);
+  if (!fn) return fail(SSM_EUNSUPPORTED, "trajectory recording is not available for the two-stage models");
+  hipLaunchKernelGGL(ws_reset_kernel, dim3(1), dim3(64), 0, stream, (unsigned long long*)ws);
+  SSM_HIP(hipGetLastError());
+  EmArgs A;
+  A.set_params = set_params;
+  A.set_stride = P->set_stride ? P->set_stride : SSM_SET_STRIDE;
+  A.offsets = offsets;
+  A.order = nullptr;
+  A.qoff = offsets;
+  A.table = table;
+  A.replay_normals = rn;
+  A.replay_uniforms = ru;
+  A.rts = nullptr;
+  A.choices = nullptr;
+  A.traj = traj;
+  A.traj_rows = traj_rows;
+  A.ws = (unsigned long long*)ws;
+  A.total = 1;  // queue index 0 only: set 0, sample 0 (set 0 must hold a sample)
+  A.replay_stride = P->replay_stride;
+  A.n_sets = 1;
+  A.n_steps = P->n_steps;
+  A.n_particles = P->n_particles;
+  A.smooth_unif = P->smooth_unif;
+  A.out_layout = 0;
+  A.choice_bytes = P->choice_bytes;
+  A.refill_min = 1;
+  A.queue_block = 1;
+  A.dt = P->dt;
+  A.k0 = (uint32_t)P->seed;
+  A.k1 = (uint32_t)(P->seed >> 32);
+  hipLaunchKernelGGL(fn, dim3(1), dim3(64), 0, stream, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

@@ -78,6 +78,7 @@ _SIGS = {
         C.c_int32,
         [C.POINTER(EmProblem), C.POINTER(LaunchConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
+    "ssm_em_trajectory": (C.c_int32, [C.POINTER(EmProblem), _VP, _VP, _VP, _VP, _VP, _VP, C.c_int32, _VP, _VP]),
     "ssm_em_graph_create": (
         C.c_int32,
         [C.POINTER(EmProblem), C.POINTER(LaunchConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(_VP)],

@@ -85,7 +85,16 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
     total = n_samples * n_trials
     rts = job.rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
     choices = job.choices[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
-    return rts, choices
+    return rts, choices, job
+
+
+def _trajectory(job, n_particles, max_t, delta_t, n_samples, n_trials) -> np.ndarray:
+    """metadata['trajectory']: float32[len(t_s), n_particles], the positions of trial 0, sample 0
+    after every step (ddm_models.pyx:714-728, race_models.pyx:433-452), -999 after its end."""
+    rows = time_grid(max_t, delta_t).shape[0]
+    if n_samples < 1 or n_trials < 1:
+        return np.full((rows, n_particles), -999.0, dtype=np.float32)
+    return job.trajectory(rows)
 
 
 def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, n_trials, boundary_fun,
@@ -107,12 +116,13 @@ def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, 
     delta_t = float(np.float32(delta_t))
     rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra,
                       stride=L.SET_STRIDE_SINGLE)
-    rts, choices = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t, _seed(random_state),
-                        smooth_unif, boundary_fun, boundary_params, replay_normals, replay_uniforms)
+    rts, choices, job = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t,
+                             _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
+                             replay_uniforms)
     minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
-    traj = np.full((time_grid(max_t, delta_t).shape[0], 1), -999.0, dtype=np.float32)
+    traj = _trajectory(job, 1, max_t, delta_t, int(n_samples), int(n_trials))
     full = _full_meta(
         minimal,
         params,
@@ -209,13 +219,13 @@ def racing_diffusion_model(v, b, A, t, s, deadline, delta_t=0.001, max_t=20, n_s
     delta_t = float(np.float32(delta_t))
     rec = accumulator_records(v, col(A), col(t), col(s), deadline, max_t, delta_t, L.BOUNDARY_CONSTANT,
                               {"a": col(b)})
-    rts, choices = _run(L.MODEL_RDM, L.BOUNDARY_CONSTANT, n_particles, rec, int(n_samples), n_trials, max_t,
-                        delta_t, _seed(random_state), smooth_unif, constant, {"a": col(b)}, replay_normals,
-                        replay_uniforms)
+    rts, choices, job = _run(L.MODEL_RDM, L.BOUNDARY_CONSTANT, n_particles, rec, int(n_samples), n_trials, max_t,
+                             delta_t, _seed(random_state), smooth_unif, constant, {"a": col(b)}, replay_normals,
+                             replay_uniforms)
     minimal = _minimal_meta("rdm_simulator", list(np.arange(0, n_particles, 1)), int(n_samples), n_trials)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
-    traj = np.full((time_grid(max_t, delta_t).shape[0], n_particles), -999.0, dtype=np.float32)
+    traj = _trajectory(job, n_particles, max_t, delta_t, int(n_samples), n_trials)
     full = _full_meta(minimal, {"v": v, "b": b, "A": A, "t": t, "deadline": deadline, "s": s},
                       {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads}, traj=traj,
                       extra={f"v{i}": v[:, i] for i in range(n_particles)})
@@ -255,6 +265,9 @@ def _two_stage(name, model, vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, delta_t,
     minimal = _minimal_meta(name, [0, 1, 2, 3], n_samples, n_trials, boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal, **extra}
+    # the reference's two-stage trajectories hold the high walk and both low walks (columns 0-2,
+    # sequential_models.pyx:457-579); the kernel walks only the low walk that the high walk
+    # selects, so these stay at the -999 fill
     traj = np.full((time_grid(max_t, delta_t).shape[0], 3), -999.0, dtype=np.float32)
     full = _full_meta(
         minimal,
@@ -311,9 +324,9 @@ def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n
     max_t = float(np.float32(max_t))
     delta_t = float(np.float32(delta_t))
     rec = accumulator_records(v, z, t, s, deadline, max_t, delta_t, code, boundary_params, g=g, b=b)
-    rts, choices = _run(model, code, n_particles, rec, int(n_samples), int(n_trials), max_t, delta_t,
-                        _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
-                        replay_uniforms)
+    rts, choices, job = _run(model, code, n_particles, rec, int(n_samples), int(n_trials), max_t, delta_t,
+                             _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
+                             replay_uniforms)
     minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
@@ -324,7 +337,7 @@ def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n
     extra = {f"v{i}": v[:, i] for i in range(n_particles)}
     zz = np.asarray(z, np.float32)
     extra.update({f"z{i}": zz[:, i] for i in range(zz.shape[1])})
-    traj = np.full((time_grid(max_t, delta_t).shape[0], n_particles), -999.0, dtype=np.float32)
+    traj = _trajectory(job, n_particles, max_t, delta_t, int(n_samples), int(n_trials))
     full = _full_meta(
         minimal, params, {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads},
         boundary_fun=boundary_fun,

@@ -237,6 +237,65 @@ class MLPSurrogate:
             self.val_mae = float(err[above].mean().item()) if bool(above.any()) else self.val_mae_all
         return history
 
+    # ---- inference over theta ---------------------------------------------------
+    def fit_theta(self, rt, choice, *, lower=None, upper=None, n_starts: int = 16, steps: int = 300,
+                  lr: float = 0.05, lr_final: float = 0.002, seed: int = 0):
+        """Maximum-likelihood parameters of one data set under the surrogate.
+
+        Gradient ascent on sum_i loglik(theta, rt_i, choice_i) over the 5 parameter inputs:
+        d loglik / d theta comes from the hand-written backward kernel through the autograd
+        function (`differentiable`), summed over the data rows by autograd. Adam runs
+        `n_starts` starts at once, drawn uniformly in [lower, upper] (default: the training
+        box of the parameter inputs), with an exponential learning-rate decay lr -> lr_final.
+        After every step the parameters are projected back into the box.
+
+        A trial with rt <= t (the non-decision time input, column `t_index`) is impossible under
+        the model: it contributes the KDE labels' floor (FLOOR, -66.774) instead of the
+        network's value, as the KDE labels such rows. Without that rule the fit can drift into a
+        corner of the box where the network, trained on floor-clipped targets, extrapolates.
+
+        Returns (theta_hat [5], loglik_hat, thetas [n_starts, 5], logliks [n_starts]) as NumPy
+        arrays; theta_hat is the start with the largest final log-likelihood."""
+        t = D.torch()
+        if lower is None or upper is None:
+            if self.bounds is None:
+                raise ValueError("fit_theta needs lower/upper or a trained surrogate (its training bounds)")
+        lo = np.asarray(self.bounds[0][:5] if lower is None else lower, np.float32)
+        hi = np.asarray(self.bounds[1][:5] if upper is None else upper, np.float32)
+        rt = t.as_tensor(np.asarray(rt, np.float32).reshape(-1), device="cuda")
+        ch = t.as_tensor(np.asarray(choice, np.float32).reshape(-1), device="cuda")
+        if rt.numel() == 0 or rt.numel() != ch.numel():
+            raise ValueError("rt and choice must be non-empty and of the same length")
+        K, N = int(n_starts), int(rt.numel())
+        rng = np.random.default_rng(seed)
+        lo_t, hi_t = t.as_tensor(lo, device="cuda"), t.as_tensor(hi, device="cuda")
+        theta = t.tensor(rng.uniform(lo, hi, (K, 5)).astype(np.float32), device="cuda", requires_grad=True)
+        opt = t.optim.Adam([theta], lr=lr)
+        gamma = (lr_final / lr) ** (1.0 / max(1, steps))
+        sched = t.optim.lr_scheduler.ExponentialLR(opt, gamma=gamma)
+        data = t.stack([rt, ch], 1)[None].expand(K, N, 2)
+
+        def rows(th):
+            return t.cat([th[:, None, :].expand(K, N, 5), data], 2).reshape(K * N, IN)
+
+        def total(y, th):
+            possible = rt[None, :] > th[:, self.t_index:self.t_index + 1]
+            return t.where(possible, y.view(K, N), t.full_like(y.view(K, N), FLOOR)).sum(1)
+
+        for _ in range(int(steps)):
+            ll = total(self.differentiable(rows(theta)), theta)
+            opt.zero_grad(set_to_none=True)
+            (-ll.sum()).backward()
+            opt.step()
+            sched.step()
+            with t.no_grad():
+                theta.copy_(t.minimum(t.maximum(theta, lo_t), hi_t))
+        with t.no_grad():
+            ll = total(self(rows(theta)), theta)
+        best = int(t.argmax(ll).item())
+        thetas, lls = theta.detach().cpu().numpy(), ll.cpu().numpy()
+        return thetas[best], float(lls[best]), thetas, lls
+
     def set_input_normalisation(self, x, rt_mode: str = "log_minus_t", t_index: int = 3) -> None:
         """Standardise the inputs: the network sees (x' - mean) / std (std floored at 1e-6).
         x' is x with rt replaced by log(max(u, 1e-4)), u = rt ("log": the KDE labels live on

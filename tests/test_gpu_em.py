@@ -775,3 +775,65 @@ def test_compact_records_give_the_same_walks(torch_gpu, model):
         job.prob.set_stride = L.SET_STRIDE_SINGLE
         with pytest.raises(ValueError, match="set_stride"):
             job.launch()
+
+
+# ----------------------------------------------------------------------------- metadata['trajectory']
+def test_trajectory_metadata_matches_oracle_in_replay_mode(torch_gpu):
+    """The drop-ins' metadata['trajectory'] (trial 0, sample 0, every step) equals the oracle's
+    recording of the same walk, for the DDM with an angle bound, the full DDM, race and LCA."""
+    from ssmhip import boundary_functions as BF
+    from ssmhip import cssm
+
+    K, N, L_ = 3, 20, 257
+    rng = np.random.default_rng(71)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=(K * N, 3)).astype(F32)
+    v = np.array([0.6, -0.4, 1.0], F32)
+    a = np.array([1.2, 1.0, 1.5], F32)
+    z = np.array([0.45, 0.55, 0.5], F32)
+    t = np.array([0.2, 0.3, 0.1], F32)
+    dl, s1 = np.full(K, 999, F32), np.ones(K, F32)
+    th = np.array([0.2, 0.5, 0.1], F32)
+    print("[case] trajectory metadata, replay", flush=True)
+    out = cssm.ddm_flexbound(v, a, z, t, dl, s1, n_samples=N, n_trials=K, boundary_fun=BF.angle,
+                             boundary_params={"a": a, "theta": th}, replay_normals=normals,
+                             replay_uniforms=unif[:, 0].copy())
+    *_, tr = E.ddm_flexbound(v, a, z, t, dl, s1, n_samples=N, boundary="angle", bparams={"a": a, "theta": th},
+                             normals=normals, uniforms=unif[:, 0].copy(), record0=True)
+    assert np.array_equal(out["metadata"]["trajectory"], tr)
+    var = {"sz": np.full(K, 0.1, F32), "sv": np.full(K, 0.5, F32), "st": np.full(K, 0.05, F32)}
+    out = cssm.full_ddm(v, a, z, t, var["sz"], var["sv"], var["st"], dl, s1, n_samples=N, n_trials=K,
+                        replay_normals=normals, replay_uniforms=unif)
+    *_, tr = E.ddm_flexbound(v, a, z, t, dl, s1, n_samples=N, normals=normals, uniforms=unif, record0=True, **var)
+    assert np.array_equal(out["metadata"]["trajectory"], tr)
+    P = 3
+    vr = rng.uniform(0.5, 2.0, (K, P)).astype(F32)
+    zr = rng.uniform(0.0, 0.4, (K, P)).astype(F32)
+    ar = np.array([1.5, 2.0, 1.2], F32)
+    for model in ("race", "lca"):
+        kw = dict(g=np.full((K, 1), 0.3, F32), b=np.full((K, 1), 0.2, F32)) if model == "lca" else {}
+        fn = cssm.race_model if model == "race" else cssm.lca
+        out = fn(v=vr, z=zr, t=t[:, None], s=np.ones((K, 1), F32), deadline=dl, n_samples=N, n_trials=K,
+                 boundary_fun=BF.constant, boundary_params={"a": ar}, replay_normals=normals, **kw)
+        *_, tr = E.accumulators(model, vr, zr, t, np.ones((K, 1), F32), dl, n_samples=N, bparams={"a": ar},
+                                normals=normals, record0=True, **kw)
+        assert np.array_equal(out["metadata"]["trajectory"], tr), model
+
+
+def test_trajectory_metadata_in_native_mode_is_walk_0(torch_gpu):
+    """Native draws: the trajectory ends where trial 0, sample 0 ended (its sign is the choice,
+    its length the rt's step count) and stays inside the bounds until then."""
+    from ssmhip import cssm
+
+    out = cssm.ddm_flexbound(np.array([0.8], F32), np.array([1.0], F32), np.array([0.5], F32), np.array([0.3], F32),
+                             np.array([999.0], F32), np.array([1.0], F32), n_samples=1000, n_trials=1,
+                             random_state=3)
+    tr = out["metadata"]["trajectory"][:, 0]
+    n = int(np.sum(tr != -999.0)) - 1
+    assert n >= 1 and np.all(tr[n + 1:] == -999.0)
+    assert np.all(np.abs(tr[:n]) <= 1.0) and abs(tr[n]) > 1.0
+    assert np.sign(tr[n]) == out["choices"][0, 0, 0]
+    t_walk = F32(0)
+    for _ in range(n):
+        t_walk = F32(t_walk + F32(DT))
+    assert out["rts"][0, 0, 0] == F32(t_walk + F32(0.3))  # smooth_unif off in the drop-in's default

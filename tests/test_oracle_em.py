@@ -621,3 +621,52 @@ def test_legacy_ddm_is_the_symmetric_walk_shifted_by_half_the_bound():
     r1, c1, s1 = E.ddm_flexbound(v, a / 2, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, delta_t=DT,
                                  normals=nz)
     assert np.mean(s0 == s1) > 0.99 and np.mean(c0 == c1) > 0.99
+
+
+@pytest.mark.parametrize("bname", ["constant", "angle"])
+def test_oracle_trajectory_of_trial0_sample0_is_its_walk(bname):
+    # the recorded positions are walk (0, 0) itself: the start in row 0 (ddm_start32), one row
+    # per step it took, every row but the last inside the bounds of its step index, the last
+    # one outside (or the clock past max_t), its sign the choice; -999 after it
+    K, N, L_ = 3, 40, 257
+    rng = np.random.default_rng(8)
+    v = np.array([0.8, -0.5, 1.2], F32)
+    a = np.array([1.1, 0.9, 1.4], F32)
+    z = np.array([0.4, 0.6, 0.5], F32)
+    bp = {"a": a, "theta": np.array([0.3, 0.1, 0.2], F32)} if bname == "angle" else {"a": a}
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    r, c, steps, tr = E.ddm_flexbound(v, a, z, np.zeros(K, F32), np.full(K, 999, F32), np.ones(K, F32),
+                                      n_samples=N, delta_t=DT, boundary=bname, bparams=bp, normals=normals,
+                                      record0=True)
+    n = int(steps[0])
+    B = E.boundary_table(bname, bp, K, 20.0, DT)[0]
+    assert tr.shape == (E.n_steps(20.0, DT), 1)
+    assert tr[0, 0] == E.ddm_start32(z[:1], B[:1])[0]
+    assert np.all(tr[: n + 1, 0] != -999.0) and np.all(tr[n + 1:, 0] == -999.0)
+    assert np.all(np.abs(tr[:n, 0]) <= B[:n])
+    assert abs(tr[n, 0]) > B[n]
+    assert np.sign(tr[n, 0]) == c[0]
+    sst = E.sqrt_dt32(DT)
+    for i in range(n):  # step i: y + ((v dt) + (sqrt_st g_i)) in float32
+        inc = F32(F32(v[0] * F32(DT)) + F32(sst * normals[0, i % L_]))
+        assert tr[i + 1, 0] == F32(tr[i, 0] + inc)
+
+
+@pytest.mark.parametrize("model", ["race", "lca"])
+def test_oracle_accumulator_trajectory_of_trial0_sample0(model):
+    K, P, N, L_ = 2, 3, 30, 311
+    rng = np.random.default_rng(9)
+    v = rng.uniform(0.5, 2.0, (K, P)).astype(F32)
+    z = rng.uniform(0.0, 0.4, (K, P)).astype(F32)
+    a = np.array([1.5, 2.0], F32)
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    kw = dict(g=np.full(K, 0.3, F32), b=np.full(K, 0.2, F32)) if model == "lca" else {}
+    r, c, steps, tr = E.accumulators(model, v, z, np.zeros(K, F32), np.ones((K, 1), F32), np.full(K, 999, F32),
+                                     n_samples=N, delta_t=DT, bparams={"a": a}, normals=normals, record0=True, **kw)
+    n = int(steps[0])
+    assert tr.shape == (E.n_steps(20.0, DT), P)
+    assert np.array_equal(tr[0], (z[0] * a[0]).astype(F32))
+    assert np.all(tr[: n + 1] != -999.0) and np.all(tr[n + 1:] == -999.0)
+    assert np.all(tr[:n].max(axis=1) <= a[0]) and tr[n].max() > a[0]
+    assert int(np.argmax(tr[n])) == c[0]
+    assert np.all(tr[: n + 1] >= 0.0)
