@@ -122,7 +122,9 @@ typedef struct ssm_launch_config {
 
 /* ---- library ---------------------------------------------------------------*/
 /* 3: ssm_em_problem.set_stride (compact 16-float records for the single-particle
- *    diffusions; 0 keeps SSM_SET_STRIDE, the layout of version 2).            */
+ *    diffusions; 0 keeps SSM_SET_STRIDE, the layout of version 2).
+ * 4: ssm_lan_assemble takes the negative-RT rows (n_neg, n_choices, neg_ll);
+ *    ssm_copy_rows, ssm_em_trajectory, ssm_kde_eval_noalloc.                  */
 int32_t ssm_abi_version(void);
 const char* ssm_last_error(void);
 /* Number of CUs of the current device (0 and an error when no GPU).          */
@@ -279,11 +281,24 @@ int32_t ssm_kde_eval_noalloc(int32_t n_sets, int32_t n_choices, const int64_t* g
 
 /* ssm_lan_assemble replaces the row assembly and shuffle of
  * ResampleMixtureStrategy.generate (resample_mixture_strategy.py:129-214):
- * rows_dev[n_sets*m][n_theta + 3] = (theta_dev[set][:], rt, choice value, ll),
- * rows of each set in a Philox-keyed random order when shuffle != 0.          */
-int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
-                         const float* theta_dev, const float* pts_rt_dev, const int8_t* pts_choice_dev,
-                         const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream);
+ * rows_dev[n_sets*(m + n_neg)][n_theta + 3] = (theta_dev[set][:], rt, choice value, ll).
+ * Per set the m evaluated points (pts_rt / pts_choice / ll [n_sets*m]) come first, then
+ * n_neg negative-RT rows (:202-209): rt ~ U(-1, 1e-4), choice uniform over the n_choices
+ * values of choice_values_dev, ll = neg_ll (the reference's -66.77497). With shuffle != 0
+ * the m + n_neg <= 1024 rows of each set are put in one Philox-keyed random order
+ * (:211-212).                                                                 */
+int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_neg, int32_t n_choices,
+                         float neg_ll, int32_t shuffle, uint64_t seed, const float* theta_dev,
+                         const float* pts_rt_dev, const int8_t* pts_choice_dev, const float* ll_dev,
+                         const int32_t* choice_values_dev, float* rows_dev, void* stream);
+
+/* ssm_copy_rows replaces the array assignments of the simulation filter's re-draw loop
+ * (SimulationPipeline.generate_for_parameter_set, simulation_pipeline.py:103-115: a rejected
+ * set's simulations are replaced by those of an accepted re-draw) and the dropping of sets
+ * that never pass: row dst_idx[r] of dst_dev (r when dst_idx_dev is NULL) becomes row
+ * src_idx[r] of src_dev (r when NULL), for r < n_rows, rows of row_bytes bytes.        */
+int32_t ssm_copy_rows(int64_t n_rows, int64_t row_bytes, const void* src_dev, const int64_t* src_idx_dev,
+                      void* dst_dev, const int64_t* dst_idx_dev, void* stream);
 
 /* ssm_aux_labels replaces SimulationPipeline._compute_auxiliary_labels
  * (ssms/dataset_generators/pipelines/simulation_pipeline.py:258-351) as

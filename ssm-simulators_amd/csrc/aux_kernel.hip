@@ -130,4 +130,38 @@ int32_t theta_uniform_enqueue(int32_t n_sets, int32_t n_params, const double* lo
   return SSM_OK;
 }
 
+// ---- row copies between set-major buffers (the simulation filter's replacements) -----
+// dst row dst_idx[r] (or r) <- src row src_idx[r] (or r), rows of row_bytes bytes; one block
+// per row, 4-byte words when every row starts 4-byte aligned.
+__global__ void copy_rows_kernel(int64_t n_rows, int64_t row_bytes, const uint8_t* __restrict__ src,
+                                 const int64_t* __restrict__ src_idx, uint8_t* __restrict__ dst,
+                                 const int64_t* __restrict__ dst_idx, int words) {
+  for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+    const int64_t s = src_idx ? src_idx[r] : r;
+    const int64_t d = dst_idx ? dst_idx[r] : r;
+    const uint8_t* a = src + s * row_bytes;
+    uint8_t* b = dst + d * row_bytes;
+    if (words) {
+      const int64_t nw = row_bytes >> 2;
+      for (int64_t i = threadIdx.x; i < nw; i += blockDim.x)
+        reinterpret_cast<uint32_t*>(b)[i] = reinterpret_cast<const uint32_t*>(a)[i];
+    } else {
+      for (int64_t i = threadIdx.x; i < row_bytes; i += blockDim.x) b[i] = a[i];
+    }
+  }
+}
+
+int32_t copy_rows_enqueue(int64_t n_rows, int64_t row_bytes, const void* src, const int64_t* src_idx, void* dst,
+                          const int64_t* dst_idx, hipStream_t stream) {
+  if (n_rows < 0 || row_bytes < 0) return fail(SSM_EINVAL, "negative sizes");
+  if (n_rows == 0 || row_bytes == 0) return SSM_OK;
+  if (!src || !dst) return fail(SSM_EINVAL, "NULL buffer");
+  const int words = (row_bytes % 4 == 0) && ((uintptr_t)src % 4 == 0) && ((uintptr_t)dst % 4 == 0);
+  const int64_t grid = n_rows < 65536 ? n_rows : 65536;
+  hipLaunchKernelGGL(copy_rows_kernel, dim3((unsigned)grid), dim3(256), 0, stream, n_rows, row_bytes,
+                     (const uint8_t*)src, src_idx, (uint8_t*)dst, dst_idx, words);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
 }  // namespace ssmhip

@@ -83,9 +83,12 @@ int32_t kde_eval_noalloc_enqueue(int32_t n_sets, int32_t n_choices, const int64_
                                  const int32_t* kde_count, const int32_t* choice_total, const float* bandwidth,
                                  const int64_t* kde_start, const int64_t* offsets, float max_t, float lb, float eps,
                                  int64_t max_samples, int64_t n_points, float* out_ll, hipStream_t stream);
-int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
-                             const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
-                             const int32_t* choice_values, float* rows, hipStream_t stream);
+int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_neg, int32_t n_choices,
+                             float neg_ll, int32_t shuffle, uint64_t seed, const float* theta, const float* pts_rt,
+                             const int8_t* pts_choice, const float* ll, const int32_t* choice_values, float* rows,
+                             hipStream_t stream);
+int32_t copy_rows_enqueue(int64_t n_rows, int64_t row_bytes, const void* src, const int64_t* src_idx, void* dst,
+                          const int64_t* dst_idx, hipStream_t stream);
 
 int32_t mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
 int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStream_t s);
@@ -115,7 +118,7 @@ using namespace ssmhip;
 
 extern "C" {
 
-int32_t ssm_abi_version(void) { return 3; }
+int32_t ssm_abi_version(void) { return 4; }
 
 const char* ssm_last_error(void) { return g_last_error.c_str(); }
 
@@ -250,11 +253,17 @@ int32_t ssm_kde_eval_noalloc(int32_t n_sets, int32_t n_choices, const int64_t* g
                                   lb, eps, max_samples, n_points, out_ll_dev, (hipStream_t)stream);
 }
 
-int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
-                         const float* theta_dev, const float* pts_rt_dev, const int8_t* pts_choice_dev,
-                         const float* ll_dev, const int32_t* choice_values_dev, float* rows_dev, void* stream) {
-  return lan_assemble_enqueue(n_sets, n_theta, m, shuffle, seed, theta_dev, pts_rt_dev, pts_choice_dev, ll_dev,
-                              choice_values_dev, rows_dev, (hipStream_t)stream);
+int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_neg, int32_t n_choices,
+                         float neg_ll, int32_t shuffle, uint64_t seed, const float* theta_dev,
+                         const float* pts_rt_dev, const int8_t* pts_choice_dev, const float* ll_dev,
+                         const int32_t* choice_values_dev, float* rows_dev, void* stream) {
+  return lan_assemble_enqueue(n_sets, n_theta, m, n_neg, n_choices, neg_ll, shuffle, seed, theta_dev, pts_rt_dev,
+                              pts_choice_dev, ll_dev, choice_values_dev, rows_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_copy_rows(int64_t n_rows, int64_t row_bytes, const void* src_dev, const int64_t* src_idx_dev,
+                      void* dst_dev, const int64_t* dst_idx_dev, void* stream) {
+  return copy_rows_enqueue(n_rows, row_bytes, src_dev, src_idx_dev, dst_dev, dst_idx_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_aux_labels(int32_t n_sets, int32_t n_choices, const int32_t* choice_values_dev, int32_t max_choice,

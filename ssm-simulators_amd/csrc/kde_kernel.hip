@@ -483,16 +483,22 @@ __global__ void __launch_bounds__(kEvalThreads) kde_eval_finish_kernel(EvalArgs 
 }
 
 // ---- LAN rows: [theta..., rt, choice value, ll], shuffled per set -------------------
+// Per set, m evaluated points plus n_neg negative-RT rows (resample_mixture_strategy.py:
+// 202-209: rt ~ U(-1, 1e-4), a uniform random choice, label neg_ll), all m + n_neg rows of a
+// set shuffled together (:211-212) by sorting random keys in LDS.
 struct AssembleArgs {
   const float* theta;        // [n_sets][n_theta]
   const float* pts_rt;       // [n_sets*m]
   const int8_t* pts_choice;  // [n_sets*m] choice index
   const float* ll;           // [n_sets*m]
   const int32_t* choice_values;
-  float* rows;               // [n_sets*m][n_theta + 3]
+  float* rows;               // [n_sets*(m + n_neg)][n_theta + 3]
   int32_t n_theta;
   int32_t m;
+  int32_t n_neg;
+  int32_t n_choices;
   int32_t shuffle;
+  float neg_ll;
   uint32_t k0, k1;
 };
 
@@ -504,13 +510,14 @@ __global__ void __launch_bounds__(kSortN) lan_assemble_kernel(AssembleArgs A) {
   const int set = blockIdx.x;
   const int t = threadIdx.x;
   const int m = A.m;
+  const int M = A.m + A.n_neg;  // rows of this set
   const int W = A.n_theta + 3;
   // random keys (Philox) for a uniform random permutation; padding sorts last
-  if (t < m && A.shuffle) {
+  if (t < M && A.shuffle) {
     const u32x4 r = philox4x32_10(u32x4{(uint32_t)t, (uint32_t)set, 0u, 13u}, A.k0, A.k1);
     key[t] = r.x & 0xFFFFFC00u | (uint32_t)t;  // distinct keys
   } else {
-    key[t] = t < m ? (uint32_t)t : 0xFFFFFFFFu;
+    key[t] = t < M ? (uint32_t)t : 0xFFFFFFFFu;
   }
   idx[t] = t;
   __syncthreads();
@@ -528,14 +535,24 @@ __global__ void __launch_bounds__(kSortN) lan_assemble_kernel(AssembleArgs A) {
       __syncthreads();
     }
   }
-  if (t < m) {
+  if (t < M) {
     const int src = idx[t];
-    const int64_t si = (int64_t)set * m + src;
-    float* row = A.rows + ((int64_t)set * m + t) * W;
+    float* row = A.rows + ((int64_t)set * M + t) * W;
     for (int q = 0; q < A.n_theta; ++q) row[q] = A.theta[(int64_t)set * A.n_theta + q];
-    row[A.n_theta] = A.pts_rt[si];
-    row[A.n_theta + 1] = (float)A.choice_values[A.pts_choice[si]];
-    row[A.n_theta + 2] = A.ll[si];
+    if (src < m) {
+      const int64_t si = (int64_t)set * m + src;
+      row[A.n_theta] = A.pts_rt[si];
+      row[A.n_theta + 1] = (float)A.choice_values[A.pts_choice[si]];
+      row[A.n_theta + 2] = A.ll[si];
+    } else {
+      // negative-RT row src - m of this set: rt = -1 + (1e-4 + 1) u, choice index floor(u' C)
+      const u32x4 r = philox4x32_10(u32x4{(uint32_t)(src - m), (uint32_t)set, 0u, 14u}, A.k0, A.k1);
+      int c = (int)(u32_to_unit(r.y) * (float)A.n_choices);
+      if (c >= A.n_choices) c = A.n_choices - 1;
+      row[A.n_theta] = __builtin_fmaf(1.0001f, u32_to_unit(r.x), -1.0f);
+      row[A.n_theta + 1] = (float)A.choice_values[c];
+      row[A.n_theta + 2] = A.neg_ll;
+    }
   }
 }
 
@@ -649,15 +666,18 @@ int32_t kde_eval_noalloc_enqueue(int32_t n_sets, int32_t n_choices, const int64_
                           bandwidth, kde_start, offsets, max_t, lb, eps, max_samples, ws, out_ll, stream);
 }
 
-int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t shuffle, uint64_t seed,
-                             const float* theta, const float* pts_rt, const int8_t* pts_choice, const float* ll,
-                             const int32_t* choice_values, float* rows, hipStream_t stream) {
-  if (n_sets < 0 || n_theta < 0 || m < 0) return fail(SSM_EINVAL, "negative sizes");
-  if (m > kSortN) return fail(SSM_EINVAL, "at most 1024 rows per set");
-  if (n_sets == 0 || m == 0) return SSM_OK;
-  if (!theta || !pts_rt || !pts_choice || !ll || !choice_values || !rows) return fail(SSM_EINVAL, "NULL buffer");
-  AssembleArgs A{theta, pts_rt, pts_choice, ll, choice_values, rows, n_theta, m, shuffle, (uint32_t)seed,
-                 (uint32_t)(seed >> 32)};
+int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_neg, int32_t n_choices,
+                             float neg_ll, int32_t shuffle, uint64_t seed, const float* theta, const float* pts_rt,
+                             const int8_t* pts_choice, const float* ll, const int32_t* choice_values, float* rows,
+                             hipStream_t stream) {
+  if (n_sets < 0 || n_theta < 0 || m < 0 || n_neg < 0) return fail(SSM_EINVAL, "negative sizes");
+  if (m + n_neg > kSortN) return fail(SSM_EINVAL, "at most 1024 rows per set");
+  if (n_choices < 1 || n_choices > SSM_MAX_PARTICLES) return fail(SSM_EINVAL, "n_choices must be in [1, 16]");
+  if (n_sets == 0 || m + n_neg == 0) return SSM_OK;
+  if (!theta || !choice_values || !rows || (m > 0 && (!pts_rt || !pts_choice || !ll)))
+    return fail(SSM_EINVAL, "NULL buffer");
+  AssembleArgs A{theta, pts_rt, pts_choice, ll, choice_values, rows, n_theta, m, n_neg, n_choices, shuffle, neg_ll,
+                 (uint32_t)seed, (uint32_t)(seed >> 32)};
   hipLaunchKernelGGL(lan_assemble_kernel, dim3(n_sets), dim3(kSortN), 0, stream, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;

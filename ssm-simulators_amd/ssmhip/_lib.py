@@ -106,8 +106,10 @@ _SIGS = {
          C.c_int64, C.c_int64, _VP, _VP],
     ),
     "ssm_lan_assemble": (
-        C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+        C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_uint64, _VP,
+                    _VP, _VP, _VP, _VP, _VP, _VP]
     ),
+    "ssm_copy_rows": (C.c_int32, [C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP]),
     "ssm_mlp_sizes": (C.c_int32, [C.POINTER(C.c_int32)] * 5),
     "ssm_mlp_pack": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_mlp_forward": (C.c_int32, [_VP, C.c_int64, _VP, _VP, _VP, _VP, _VP]),

@@ -111,19 +111,24 @@ class LanKDELabeller:
         mark(4)
         m = pts.m
         W = self.theta.shape[1] + 3
-        rows = D.empty((self.n_sets * m, W), t.float32)
-        shuffle_here = self.shuffle and not self.n_neg  # with negative rows the shuffle follows their append
-        rc = L.load().ssm_lan_assemble(self.n_sets, self.theta.shape[1], m, int(shuffle_here), self.seed + 2,
-                                       D.ptr(self.d_theta), D.ptr(pts.rt), D.ptr(pts.choice), D.ptr(ll),
-                                       D.ptr(self.d_choice_values), D.ptr(rows), D.stream_ptr())
+        M = m + self.n_neg  # rows per set: the evaluated points, then the negative-RT rows
+        rows = D.empty((self.n_sets * M, W), t.float32)
+        # one kernel: the evaluated points, the negative-RT rows (resample_mixture_strategy.py:
+        # 202-209) and the shuffle of all of a set's rows (:211-212)
+        rc = L.load().ssm_lan_assemble(self.n_sets, self.theta.shape[1], m, self.n_neg, len(self.choice_values),
+                                       NEG_RT_LL, int(self.shuffle), self.seed + 2, D.ptr(self.d_theta),
+                                       D.ptr(pts.rt), D.ptr(pts.choice), D.ptr(ll), D.ptr(self.d_choice_values),
+                                       D.ptr(rows), D.stream_ptr())
         L.check(rc, "ssm_lan_assemble")
         mark(5)
         self.bank, self.points, self.ll = bank, pts, ll
-        if self.n_neg:
-            rows = self._append_negative(rows)
         if not self.kept.all():  # sets the filter rejected on every attempt carry no rows
-            keep = D.to_dev(np.flatnonzero(self.kept).astype(np.int64))
-            rows = rows.view(self.n_sets, -1, rows.shape[1])[keep].reshape(-1, rows.shape[1]).contiguous()
+            keep = np.flatnonzero(self.kept).astype(np.int64)
+            out = D.empty((keep.size * M, W), t.float32)
+            rc = L.load().ssm_copy_rows(int(keep.size), M * W * 4, D.ptr(rows), D.ptr(D.to_dev(keep)), D.ptr(out),
+                                        None, D.stream_ptr())
+            L.check(rc, "ssm_copy_rows")
+            rows = out
         if ev is not None:
             ev[5].synchronize()
             self.times = StageTimes(ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]),
@@ -176,8 +181,10 @@ class LanKDELabeller:
             if ok.any():
                 src = D.to_dev(np.flatnonzero(ok).astype(np.int64))
                 dst = D.to_dev(rejected[ok].astype(np.int64))
-                rts[dst] = sub.rts.view(-1, n0)[src]
-                chs[dst] = sub.choices.view(-1, n0)[src]
+                lib = L.load()
+                for a, b, nbytes in ((sub.rts, rts, 4), (sub.choices, chs, sub.choices.element_size())):
+                    L.check(lib.ssm_copy_rows(int(ok.sum()), n0 * nbytes, D.ptr(a), D.ptr(src), D.ptr(b),
+                                              D.ptr(dst), D.stream_ptr()), "ssm_copy_rows")
                 self.theta[rejected[ok]] = th[ok]
                 self.n_resampled += int(ok.sum())
             sub.close()
@@ -186,27 +193,6 @@ class LanKDELabeller:
         self.kept[rejected] = False
         self.d_theta = D.to_dev(self.theta)
         t.cuda.current_stream().synchronize()
-
-    def _append_negative(self, rows):
-        """Add n_neg negative-RT rows per set: rt ~ U(-1, 1e-4), random choice, label -66.77497
-        (resample_mixture_strategy.py:202-209), then shuffle each set's rows
-        (resample_mixture_strategy.py:211-212)."""
-        t = D.torch()
-        gen = t.Generator(device=rows.device)
-        gen.manual_seed(self.seed + 3)
-        S, P, W = self.n_sets, self.theta.shape[1], rows.shape[1]
-        m = rows.shape[0] // S
-        neg = t.empty((S, self.n_neg, W), dtype=t.float32, device=rows.device)
-        neg[:, :, :P] = self.d_theta[:, None, :]
-        neg[:, :, P] = -1.0 + 1.0001 * t.rand((S, self.n_neg), generator=gen, device=rows.device)
-        pick = t.randint(0, len(self.choice_values), (S, self.n_neg), generator=gen, device=rows.device)
-        neg[:, :, P + 1] = self.d_choice_values[pick].float()
-        neg[:, :, P + 2] = NEG_RT_LL
-        out = t.cat([rows.view(S, m, W), neg], dim=1)
-        if self.shuffle:
-            order = t.rand((S, m + self.n_neg), generator=gen, device=rows.device).argsort(dim=1)
-            out = t.gather(out, 1, order[:, :, None].expand(-1, -1, W))
-        return out.reshape(S * (m + self.n_neg), W).contiguous()
 
     def aux_counts(self) -> dict:
         """Integer counts behind the auxiliary labels of every set, from the last simulation

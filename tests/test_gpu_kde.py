@@ -174,6 +174,7 @@ def test_lan_rows_small_batch_against_oracle(torch_gpu):
     sim_rts = lab.sim.rts.cpu().numpy()
     sim_ch = lab.sim.choices.cpu().numpy()
     offs = lab.sim.offsets
+    neg_rt, neg_ch = [], []
     for k in range(6):
         blk = rows[k * 1000:(k + 1) * 1000]
         assert np.allclose(blk[:, :5], theta[k][None, :])
@@ -182,6 +183,10 @@ def test_lan_rows_small_batch_against_oracle(torch_gpu):
         assert neg.sum() == 100
         assert np.all(blk[neg, 7] == np.float32(-66.77497)) and np.all(blk[neg, 5] >= -1.0)
         assert set(np.unique(blk[neg, 6])) <= {-1.0, 1.0}
+        # shuffled together with the other rows: their positions spread over the block
+        assert 350 < np.flatnonzero(neg).mean() < 650
+        neg_rt.append(blk[neg, 5])
+        neg_ch.append(blk[neg, 6])
         pos = blk[~neg]
         fit = kde_ref.KDEFit(sim_rts[offs[k]:offs[k + 1]], sim_ch[offs[k]:offs[k + 1]], [-1, 1])
         ref = fit.eval(pos[:, 5].astype(np.float64), pos[:, 6])
@@ -190,7 +195,10 @@ def test_lan_rows_small_batch_against_oracle(torch_gpu):
         assert np.all(pos[:, 7] >= LB)  # the KDE floor; the negative-RT label lies just below it
         assert set(np.unique(blk[:, 6])) <= {-1.0, 1.0}
         # 100 uniform rows in [1e-4, 20] plus 800 KDE rows
-        assert np.sum(blk[:, 5] > 0) == 900 and np.all(pos[:, 5] <= 20.0)
+        assert np.sum(~neg) == 900 and np.all(pos[:, 5] >= 1e-4) and np.all(pos[:, 5] <= 20.0)
+    # negative rows: rt ~ U(-1, 1e-4), both choices
+    assert stats.kstest(np.concatenate(neg_rt), stats.uniform(loc=-1.0, scale=1.0001).cdf).pvalue > 1e-3
+    assert 0.4 < np.mean(np.concatenate(neg_ch) == 1.0) < 0.6
 
 
 def test_lan_rows_shuffle_is_a_permutation(torch_gpu):
@@ -206,3 +214,4 @@ def test_lan_rows_shuffle_is_a_permutation(torch_gpu):
         y = rb[k * 1000:(k + 1) * 1000]
         assert not np.array_equal(x, y)
         assert np.array_equal(x[np.lexsort(x.T)], y[np.lexsort(y.T)])
+        assert np.all(x[900:, 5] < 1e-4) and np.all(x[:900, 5] >= 1e-4)  # unshuffled: negative rows last
