@@ -387,10 +387,11 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
     x = np.concatenate([rng.uniform(lo, hi, (N, 5)), rng.uniform(0, 20, (N, 1)), rng.choice([-1.0, 1.0], (N, 1))],
                        axis=1).astype(np.float32)
     dx = torch.from_numpy(x).cuda()
-    net(dx)
+    for _ in range(20):  # steady state: the engine clock settles within the first few dozen launches
+        net(dx)
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 5
+    reps = 20
     s.record()
     for _ in range(reps):
         net(dx)

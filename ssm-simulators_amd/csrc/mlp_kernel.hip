@@ -149,7 +149,7 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 }
 
 // ---- forward (inference) --------------------------------------------------------------
-// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. Two row tiles (32 rows) per wave.
+// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. kFwdRowTiles row tiles (48 rows) per wave.
 // xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
 // The forward fragments (78 KB of bf16) and biases are staged once per block in LDS, so
 // the per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
@@ -163,13 +163,18 @@ constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
 // bf16, so the only change against carrying tanh is where bf16 rounds: u instead of tanh z.
 // The block prologue builds -2 P and bias' in LDS from the packed weights.
 
-// u of output tile a of both row tiles; once a pair (2b, 2b+1) is done it is packed as
+// Row tiles (16 rows each) per wave in the forward kernel. Three tiles give each wave more
+// independent MFMA and activation work between dependencies than two (165 VGPRs, 3 waves per
+// SIMD): 0.392 against 0.412 ms per 4M rows with the input transform on, identical results.
+constexpr int kFwdRowTiles = 3;
+
+// u of output tile a of every row tile; once a pair (2b, 2b+1) is done it is packed as
 // K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0
 // (its weights in the next layer are zero, so any value would do).
-template <int NA>
-__device__ __forceinline__ void activate_tile(f32x4 (&acc)[2][8], bf16x8 (&Bn)[2][4], int a) {
+template <int NA, int RT>
+__device__ __forceinline__ void activate_tile(f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4], int a) {
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < RT; ++t) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[t][a][i] = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(acc[t][a][i]) + 1.0f);
     if (a & 1) Bn[t][a >> 1] = pack_pair(acc[t][a - 1], acc[t][a]);
@@ -177,27 +182,45 @@ __device__ __forceinline__ void activate_tile(f32x4 (&acc)[2][8], bf16x8 (&Bn)[2
   }
 }
 
-// One layer for both row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
+// One layer for all row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
 // activation of output tile a-1 is placed between the MFMAs of tile a, so within one wave
 // the transcendental work of one tile overlaps the matrix-core work of the next.
-template <int NA, int KB>
+template <int NA, int KB, int RT>
 __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
-                                             const bf16x8 (&B)[2][4], f32x4 (&acc)[2][8], bf16x8 (&Bn)[2][4]) {
+                                             const bf16x8 (&B)[RT][4], f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4]) {
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     const f32x4 bt = bias_tile(bias, bo, a, g);
-    f32x4 c0 = bt, c1 = bt;
+    f32x4 c[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) c[t] = bt;
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
       const bf16x8 w = load_frag(frags, fo, a * KB + b, lane);
-      c0 = mfma(w, B[0][b], c0);
-      c1 = mfma(w, B[1][b], c1);
+#pragma unroll
+      for (int t = 0; t < RT; ++t) c[t] = mfma(w, B[t][b], c[t]);
     }
-    acc[0][a] = c0;
-    acc[1][a] = c1;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t][a] = c[t];
     if (a > 0) activate_tile<NA>(acc, Bn, a - 1);
   }
   activate_tile<NA>(acc, Bn, NA - 1);
+}
+
+// The input transform with shift, scale, rt mode and t index held in registers (read once per
+// block): the same operations as input_value.
+struct InputNorm {
+  float shift[kIn], scale[kIn], mode;
+  int tix;
+};
+__device__ __forceinline__ float input_value_r(const InputNorm& q, bool on, int j, const float* xr) {
+  float v = xr[j];
+  if (!on) return v;
+  if (j == 5) {
+    if (q.mode == 2.f) v -= xr[q.tix];
+    if (q.mode != 0.f) v = logf(fmaxf(v, kRtFloor));
+  }
+  return (v - q.shift[j]) * q.scale[j];
 }
 
 __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
@@ -241,26 +264,36 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
     }
   }
   __syncthreads();
+  constexpr int RT = kFwdRowTiles;
+  InputNorm qn;
+  const bool norm_on = xnorm != nullptr;
+#pragma unroll
+  for (int j = 0; j < kIn; ++j) {
+    qn.shift[j] = norm_on ? xnorm[j] : 0.f;
+    qn.scale[j] = norm_on ? xnorm[kIn + j] : 1.f;
+  }
+  qn.mode = norm_on ? xnorm[2 * kIn] : 0.f;
+  qn.tix = norm_on ? t_index(xnorm) : 0;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r0 = wave * 32; r0 < n; r0 += nwaves * 32) {
+  for (int64_t r0 = wave * (16 * RT); r0 < n; r0 += nwaves * (16 * RT)) {
     // an offset the compiler cannot see through keeps the LDS fragment and bias reads
     // inside the row loop (hoisting them would need hundreds of registers)
     int lo = 0;
     asm volatile("" : "+v"(lo));
     const __bf16* frags = lfrag + lo;
     const float* lb = lbias + lo;
-    bf16x8 B[2][4], Bn[2][4];
-    f32x4 acc[2][8];
+    bf16x8 B[RT][4], Bn[RT][4];
+    f32x4 acc[RT][8];
     // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < RT; ++t) {
       const int64_t row = r0 + 16 * t + r;
       bf16x8 v = {};
       if (g == 0 && row < n) {
         const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value(xnorm, j, xr);
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value_r(qn, norm_on, j, xr);
       }
       B[t][0] = v;
     }
@@ -272,7 +305,7 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
     hidden_layer<8, 4>(frags, fF3, lb, pB3, lane, g, B, acc, Bn);
     // output layer
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < RT; ++t) {
       f32x4 c = bias_tile(lb, pB4, 0, g);
 #pragma unroll
       for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), Bn[t][b], c);
@@ -856,8 +889,8 @@ int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, cons
   if (!x || !frags || !bias || !y) return fail(SSM_EINVAL, "NULL buffer");
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
-  // 512-thread blocks, two resident per CU (2 x 78 KB of LDS weights)
-  const int64_t waves_needed = (n + 31) / 32;
+  // 512-thread blocks, two resident per CU (2 x 78 KB of LDS weights), kFwdRowTiles x 16 rows per wave
+  const int64_t waves_needed = (n + 16 * kFwdRowTiles - 1) / (16 * kFwdRowTiles);
   int64_t blocks = (waves_needed + 7) / 8;
   const int64_t cap = (int64_t)cus * 2;
   if (blocks > cap) blocks = cap;

@@ -106,8 +106,9 @@ def run_kde_eval(n_sets: int = 1024, n_samples: int = 100_000, passes: int = 2, 
     return {"pairs": pairs, "ms": ms, "pairs_per_s": pairs / (min(ms) / 1e3)}
 
 
-def run_mlp_forward(n_rows: int = 4_000_000, reps: int = 3, seed: int = 0) -> dict:
-    """The surrogate's forward (mlp_forward_kernel) on random rows with random weights."""
+def run_mlp_forward(n_rows: int = 4_000_000, reps: int = 3, seed: int = 0, warmup: int = 0) -> dict:
+    """The surrogate's forward (mlp_forward_kernel) on random rows with random weights; `warmup`
+    untimed launches first, so that the engine clock has settled when the timed ones run."""
     from .surrogate import MLPSurrogate
 
     t = D.require_gpu()
@@ -117,6 +118,8 @@ def run_mlp_forward(n_rows: int = 4_000_000, reps: int = 3, seed: int = 0) -> di
     hi = np.array([3.0, 3.0, 0.9, 2.0, 1.3, 20.0, 1.0])
     x = D.to_dev(rng.uniform(lo, hi, (n_rows, 7)).astype(np.float32))
     net.set_input_normalisation(x)
+    for _ in range(warmup):
+        net(x)
     ms = []
     for _ in range(reps):
         s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
@@ -139,13 +142,14 @@ def main(argv=None) -> int:
     ap.add_argument("--samples", type=int, default=100_000)
     ap.add_argument("--rows", type=int, default=4_000_000)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=0, help="untimed launches before the timed ones (mlp-forward)")
     a = ap.parse_args(argv)
     if a.workload == "probes":
         res = run_probes(a.iters)
     elif a.workload == "kde-eval":
         res = run_kde_eval(a.sets, a.samples, a.reps)
     else:
-        res = run_mlp_forward(a.rows, a.reps)
+        res = run_mlp_forward(a.rows, a.reps, warmup=a.warmup)
     print(json.dumps(res), flush=True)
     return 0
 
