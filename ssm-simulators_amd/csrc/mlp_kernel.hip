@@ -149,7 +149,7 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 }
 
 // ---- forward (inference) --------------------------------------------------------------
-// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. kFwdRowTiles row tiles (48 rows) per wave.
+// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. kFwdRowTiles row tiles (32 rows) per wave.
 // xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
 // The forward fragments (78 KB of bf16) and biases are staged once per block in LDS, so
 // the per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
@@ -163,16 +163,19 @@ constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
 // bf16, so the only change against carrying tanh is where bf16 rounds: u instead of tanh z.
 // The block prologue builds -2 P and bias' in LDS from the packed weights.
 
-// Row tiles (16 rows each) per wave in the forward kernel. Three tiles give each wave more
-// independent MFMA and activation work between dependencies than two (165 VGPRs, 3 waves per
-// SIMD): 0.392 against 0.412 ms per 4M rows with the input transform on, identical results.
-constexpr int kFwdRowTiles = 3;
+// Row tiles (16 rows each) per wave in the forward kernel. Two tiles keep the kernel at
+// 120 VGPRs, so the LDS limit (two 512-thread blocks per CU, four waves per SIMD) binds
+// rather than registers; three tiles (165 VGPRs) leave two waves per SIMD and were slower.
+constexpr int kFwdRowTiles = 2;
 
 // u of output tile a of every row tile; once a pair (2b, 2b+1) is done it is packed as
 // K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0
-// (its weights in the next layer are zero, so any value would do).
+// (its weights in the next layer are zero, so any value would do). The wave raises its
+// issue priority for the activation: a SIMD's VALU is the busier pipe, and a wave with
+// activations ready goes first while the other waves' MFMAs run on the matrix core.
 template <int NA, int RT>
 __device__ __forceinline__ void activate_tile(f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4], int a) {
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int t = 0; t < RT; ++t) {
 #pragma unroll
@@ -180,14 +183,18 @@ __device__ __forceinline__ void activate_tile(f32x4 (&acc)[RT][8], bf16x8 (&Bn)[
     if (a & 1) Bn[t][a >> 1] = pack_pair(acc[t][a - 1], acc[t][a]);
     else if (a == NA - 1) Bn[t][a >> 1] = pack_pair(acc[t][a], f32x4{0.f, 0.f, 0.f, 0.f});
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // One layer for all row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
-// activation of output tile a-1 is placed between the MFMAs of tile a, so within one wave
-// the transcendental work of one tile overlaps the matrix-core work of the next.
-template <int NA, int KB, int RT>
+// activation of output tile a-1 runs beside the MFMAs of tile a. A layer leaves its last
+// tile's activation to the next layer (NAP = the previous layer's tile count, 0 for the
+// first), which runs it beside the MFMAs of its first tile's K blocks 0..KB-2: only K block
+// KB-1 reads that tile's units, so no layer boundary leaves the matrix core or the VALU
+// without independent work.
+template <int NA, int KB, int NAP, int RT>
 __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
-                                             const bf16x8 (&B)[RT][4], f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4]) {
+                                             bf16x8 (&B)[RT][4], f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4]) {
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     const f32x4 bt = bias_tile(bias, bo, a, g);
@@ -196,6 +203,9 @@ __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const 
     for (int t = 0; t < RT; ++t) c[t] = bt;
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
+      if constexpr (NAP > 0) {
+        if (a == 0 && b == KB - 1) activate_tile<NAP>(acc, B, NAP - 1);
+      }
       const bf16x8 w = load_frag(frags, fo, a * KB + b, lane);
 #pragma unroll
       for (int t = 0; t < RT; ++t) c[t] = mfma(w, B[t][b], c[t]);
@@ -204,18 +214,16 @@ __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const 
     for (int t = 0; t < RT; ++t) acc[t][a] = c[t];
     if (a > 0) activate_tile<NA>(acc, Bn, a - 1);
   }
-  activate_tile<NA>(acc, Bn, NA - 1);
 }
 
-// The input transform with shift, scale, rt mode and t index held in registers (read once per
-// block): the same operations as input_value.
+// The input transform: shift, scale, rt mode and t index are read once per block (the
+// identity when no transform is given, which leaves every input unchanged: (v - 0) * 1 = v).
 struct InputNorm {
   float shift[kIn], scale[kIn], mode;
   int tix;
 };
-__device__ __forceinline__ float input_value_r(const InputNorm& q, bool on, int j, const float* xr) {
+__device__ __forceinline__ float input_value_r(const InputNorm& q, int j, const float* xr) {
   float v = xr[j];
-  if (!on) return v;
   if (j == 5) {
     if (q.mode == 2.f) v -= xr[q.tix];
     if (q.mode != 0.f) v = logf(fmaxf(v, kRtFloor));
@@ -266,14 +274,22 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
   __syncthreads();
   constexpr int RT = kFwdRowTiles;
   InputNorm qn;
-  const bool norm_on = xnorm != nullptr;
 #pragma unroll
   for (int j = 0; j < kIn; ++j) {
-    qn.shift[j] = norm_on ? xnorm[j] : 0.f;
-    qn.scale[j] = norm_on ? xnorm[kIn + j] : 1.f;
+    qn.shift[j] = 0.f;
+    qn.scale[j] = 1.f;
   }
-  qn.mode = norm_on ? xnorm[2 * kIn] : 0.f;
-  qn.tix = norm_on ? t_index(xnorm) : 0;
+  qn.mode = 0.f;
+  qn.tix = 0;
+  if (xnorm) {
+#pragma unroll
+    for (int j = 0; j < kIn; ++j) {
+      qn.shift[j] = xnorm[j];
+      qn.scale[j] = xnorm[kIn + j];
+    }
+    qn.mode = xnorm[2 * kIn];
+    qn.tix = t_index(xnorm);
+  }
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t r0 = wave * (16 * RT); r0 < n; r0 += nwaves * (16 * RT)) {
@@ -293,24 +309,32 @@ __global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restric
       if (g == 0 && row < n) {
         const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value_r(qn, norm_on, j, xr);
+        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value_r(qn, j, xr);
       }
       B[t][0] = v;
     }
     // Output tile 7 of the 100-unit layers 1 and 2 (units 112..127) is all padding: its
     // weights and biases are zero and the next layer's weights for it are zero, so it
     // need not be computed.
-    hidden_layer<7, 1>(frags, fF1, lb, pB1, lane, g, B, acc, Bn);
-    hidden_layer<7, 4>(frags, fF2, lb, pB2, lane, g, Bn, acc, B);
-    hidden_layer<8, 4>(frags, fF3, lb, pB3, lane, g, B, acc, Bn);
-    // output layer
+    hidden_layer<7, 1, 0>(frags, fF1, lb, pB1, lane, g, B, acc, Bn);
+    hidden_layer<7, 4, 7>(frags, fF2, lb, pB2, lane, g, Bn, acc, B);
+    hidden_layer<8, 4, 7>(frags, fF3, lb, pB3, lane, g, B, acc, Bn);
+    // output layer, with layer 3's last activation beside its K blocks 0..2
+    f32x4 c[RT];
+    const f32x4 bt = bias_tile(lb, pB4, 0, g);
+#pragma unroll
+    for (int t = 0; t < RT; ++t) c[t] = bt;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b == 3) activate_tile<8>(acc, Bn, 7);
+      const bf16x8 w = load_frag(frags, fF4, b, lane);
+#pragma unroll
+      for (int t = 0; t < RT; ++t) c[t] = mfma(w, Bn[t][b], c[t]);
+    }
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
-      f32x4 c = bias_tile(lb, pB4, 0, g);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) c = mfma(load_frag(frags, fF4, b, lane), Bn[t][b], c);
       const int64_t row = r0 + 16 * t + r;
-      if (g == 0 && row < n) y[row] = c[0];
+      if (g == 0 && row < n) y[row] = c[t][0];
     }
   }
 }
