@@ -187,28 +187,38 @@ __device__ __forceinline__ void activate_tile(f32x4 (&acc)[RT][8], bf16x8 (&Bn)[
 }
 
 // One layer for all row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
-// activation of output tile a-1 runs beside the MFMAs of tile a. A layer leaves its last
-// tile's activation to the next layer (NAP = the previous layer's tile count, 0 for the
-// first), which runs it beside the MFMAs of its first tile's K blocks 0..KB-2: only K block
-// KB-1 reads that tile's units, so no layer boundary leaves the matrix core or the VALU
-// without independent work.
+// activation of output tile a-1 runs beside the MFMAs of tile a, and the weight fragments
+// and bias of tile a+1 are read from LDS while tile a runs (two register sets, 16 VGPRs
+// more; the kernel stays within 128). A layer leaves its last tile's activation to the next
+// layer (NAP = the previous layer's tile count, 0 for the first), which runs it beside the
+// MFMAs of its first tile's K blocks 0..KB-2: only K block KB-1 reads that tile's units, so
+// no layer boundary leaves the matrix core or the VALU without independent work.
 template <int NA, int KB, int NAP, int RT>
 __device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
                                              bf16x8 (&B)[RT][4], f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4]) {
+  bf16x8 w[2][KB];
+  f32x4 bt[2];
+#pragma unroll
+  for (int b = 0; b < KB; ++b) w[0][b] = load_frag(frags, fo, b, lane);
+  bt[0] = bias_tile(bias, bo, 0, g);
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
-    const f32x4 bt = bias_tile(bias, bo, a, g);
+    const int cur = a & 1;
+    if (a + 1 < NA) {
+#pragma unroll
+      for (int b = 0; b < KB; ++b) w[cur ^ 1][b] = load_frag(frags, fo, (a + 1) * KB + b, lane);
+      bt[cur ^ 1] = bias_tile(bias, bo, a + 1, g);
+    }
     f32x4 c[RT];
 #pragma unroll
-    for (int t = 0; t < RT; ++t) c[t] = bt;
+    for (int t = 0; t < RT; ++t) c[t] = bt[cur];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
       if constexpr (NAP > 0) {
         if (a == 0 && b == KB - 1) activate_tile<NAP>(acc, B, NAP - 1);
       }
-      const bf16x8 w = load_frag(frags, fo, a * KB + b, lane);
 #pragma unroll
-      for (int t = 0; t < RT; ++t) c[t] = mfma(w, B[t][b], c[t]);
+      for (int t = 0; t < RT; ++t) c[t] = mfma(w[cur][b], B[t][b], c[t]);
     }
 #pragma unroll
     for (int t = 0; t < RT; ++t) acc[t][a] = c[t];
@@ -231,7 +241,7 @@ __device__ __forceinline__ float input_value_r(const InputNorm& q, int j, const 
   return (v - q.shift[j]) * q.scale[j];
 }
 
-__global__ void __launch_bounds__(512) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
+__global__ void __launch_bounds__(512, 2) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
                                                           const __bf16* __restrict__ gfrags,
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ xnorm, float* __restrict__ y) {
