@@ -387,7 +387,7 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
     x = np.concatenate([rng.uniform(lo, hi, (N, 5)), rng.uniform(0, 20, (N, 1)), rng.choice([-1.0, 1.0], (N, 1))],
                        axis=1).astype(np.float32)
     dx = torch.from_numpy(x).cuda()
-    for _ in range(20):  # steady state: the engine clock settles within the first few dozen launches
+    for _ in range(150):  # steady state: after idle host-side work the engine clock takes ~100 launches (~35 ms) to settle
         net(dx)
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -468,7 +468,7 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
            "mae_vs_analytic_theta0_trained_on_1024x1e5_rows": mae_c4}
     rec["inference_published_peak"] = {
         "mfma_bf16": {"unit": "TFLOP/s", "achieved": flops / 1e12, "peak": 2500.0, "frac": flops / 2.5e15,
-                      "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 32.7% of SIMD cycles "
+                      "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 36.3% of SIMD cycles "
                                "(profiles/mlp_forward_4M_counters.md)"},
         "transcendental": {"unit": "T lane-instructions/s", "achieved": 708 * evals / 1e12,
                            "peak": PEAK_TRANS_PER_S / 1e12, "frac": 708 * evals / PEAK_TRANS_PER_S,

@@ -60,9 +60,10 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c.x = hi1 ^ c.y ^ k0;
+    // three-input XORs as one gfx950 v_bitop3_b32 each (truth table 0x96 = a ^ b ^ c)
+    c.x = __builtin_amdgcn_bitop3_b32(hi1, c.y, k0, 0x96);
     c.y = lo1;
-    c.z = hi0 ^ c.w ^ k1;
+    c.z = __builtin_amdgcn_bitop3_b32(hi0, c.w, k1, 0x96);
     c.w = lo0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
