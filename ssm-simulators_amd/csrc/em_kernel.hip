@@ -21,6 +21,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
 #include "ssmhip_internal.h"
 
 namespace ssmhip {
@@ -50,6 +55,9 @@ struct EmArgs {
   int32_t queue_block;  // largest index range a wave claims from the global queue at once
   int32_t traj_rows;
   float dt;
+  // t_fast[r] (r = 1 in replay mode): an upper bound on how far t can rise over the steps
+  // of one walk-loop iteration; NaN when n_steps > 2^24 (then no iteration runs fast)
+  float t_fast[2];
   uint32_t k0, k1;
 };
 
@@ -57,24 +65,30 @@ enum : int { kRunning = 0, kDone = 1, kEmpty = 2 };
 
 // Boundary height at step index ix (t_s[ix] = float32(ix) * dt exactly, as
 // np.arange(0, max_t + dt, dt).astype(float32) in _utils.pyx:636-640).
+// fx is float32(ix), exact for ix < 2^24; the table bound reads row[ix].
 template <int BOUND>
-__device__ __forceinline__ float boundary_at(int ix, float dt, float a, float b1, float b2,
-                                             const float* __restrict__ row) {
+__device__ __forceinline__ float boundary_at_f(float fx, int ix, float dt, float a, float b1, float b2,
+                                               const float* __restrict__ row) {
   if constexpr (BOUND == SSM_BOUNDARY_CONSTANT) {
     return a;
   } else if constexpr (BOUND == SSM_BOUNDARY_ANGLE) {
     // boundary_functions.py:57-72: a + t * (-sin(theta)/cos(theta)), slope precomputed in float32
-    const float ts = (float)ix * dt;
+    const float ts = fx * dt;
     return a + ts * b1;
   } else if constexpr (BOUND == SSM_BOUNDARY_WEIBULL) {
     // boundary_functions.py:103-122: a * exp(-(t/beta)^alpha); b1 = alpha, b2 = 1/beta
-    const float ts = (float)ix * dt;
+    const float ts = fx * dt;
     const float u = ts * b2;
     const float p = __builtin_amdgcn_exp2f(b1 * __builtin_amdgcn_logf(u));  // u == 0 -> 0
     return a * __builtin_amdgcn_exp2f(-1.4426950408889634f * p);
   } else {
     return row[ix];
   }
+}
+template <int BOUND>
+__device__ __forceinline__ float boundary_at(int ix, float dt, float a, float b1, float b2,
+                                             const float* __restrict__ row) {
+  return boundary_at_f<BOUND>((float)ix, ix, dt, a, b1, b2, row);
 }
 
 __device__ __forceinline__ int find_set(const int64_t* __restrict__ off, int n_sets, int64_t g) {
@@ -498,6 +512,63 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
       // up to kSubBlocks blocks of four steps per loop iteration, one Philox block each;
       // every walk starts block-aligned, so the draws do not depend on the grouping
       bool alive = state == kRunning;
+      // An iteration in which no walking lane can reach the end of t_s (ix + all its steps
+      // <= n_steps) or its deadline (t + t_fast <= dl_tmp, t_fast bounding the rise of t over
+      // those steps) needs neither the index guard nor the t <= dl_tmp test: both hold at
+      // every step it can take. It also counts the steps in float32, exact below 2^24, so
+      // that the boundary time needs no conversion per step.
+      bool fast = false;
+      if constexpr (!RECORD)
+        fast = __ballot(alive && !(ix + 4 * kSubBlocks<REPLAY> <= A.n_steps &&
+                                   t + A.t_fast[REPLAY ? 1 : 0] <= dl_tmp)) == 0;
+      if (fast) {
+        // fx counts the steps as float32(ix): it rises by one per step taken and stays at the
+        // index of the step whose test ended the walk
+        float fx = (float)ix;
+#pragma unroll
+        for (int sb = 0; sb < kSubBlocks<REPLAY>; ++sb) {
+          if (sb > 0) {
+            if (__ballot(alive) == 0) break;
+            ++iters;
+          }
+          if (alive) {
+            const int ixb = (int)fx;  // this block's first step index
+            float nz[4];
+            if constexpr (REPLAY) {
+              constexpr int off = MODEL == SSM_MODEL_FULL_DDM ? 1 : 0;
+              const float* rb = A.replay_normals + g * A.replay_stride;
+              const int64_t d0 = kTwoStage<MODEL> ? (int64_t)nh + (ixb - ix0) : (int64_t)(off + ixb);
+#pragma unroll
+              for (int i = 0; i < 4; ++i) nz[i] = rb[(d0 + i) % A.replay_stride];
+            } else if constexpr (MODEL == SSM_MODEL_LEVY) {
+              const u32x4 r0 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ixb >> 2), 0u}, A.k0, A.k1);
+              const u32x4 r1 = philox4x32_10(u32x4{(uint32_t)n, (uint32_t)k, (uint32_t)(ixb >> 2), 2u}, A.k0, A.k1);
+              nz[0] = stable_cms(r0.x, r0.y, alpha, alpha_inv, alpha_scale);
+              nz[1] = stable_cms(r0.z, r0.w, alpha, alpha_inv, alpha_scale);
+              nz[2] = stable_cms(r1.x, r1.y, alpha, alpha_inv, alpha_scale);
+              nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
+            } else if constexpr (kTwoStage<MODEL>) {
+              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ixb - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
+            } else {
+              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ixb >> 2), 0u, A.k0, A.k1, nz);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (alive) {
+                const float b = boundary_at_f<BOUND>(fx, ixb + i, dt, a, b1, b2, row);
+                if (inside<MODEL>(x[0], b)) {
+                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  t += dt;
+                  fx += 1.0f;
+                } else {
+                  alive = false;
+                }
+              }
+            }
+          }
+        }
+        if (state == kRunning) ix = (int)fx;
+      } else {
 #pragma unroll
       for (int sb = 0; sb < kSubBlocksR<REPLAY, RECORD>; ++sb) {
         if (sb > 0) {
@@ -571,6 +642,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             }
           }
         }
+      }
       }
       if (state == kRunning) {
         if (!alive) {
@@ -785,6 +857,44 @@ static KernelFn pick_record_kernel(int model, int bound, int p, bool replay) {
   }
 }
 
+// t after `steps` additions of dt from 0 in float32, as the walk accumulates it; the value of
+// the last (steps, dt) asked for is kept, since every launch of a job asks the same
+static float t_after_steps(int32_t steps, float dt) {
+  static thread_local int32_t c_steps = -1;
+  static thread_local uint32_t c_dt = 0;
+  static thread_local float c_t = 0.f;
+  uint32_t dt_bits;
+  std::memcpy(&dt_bits, &dt, sizeof dt_bits);
+  if (steps == c_steps && dt_bits == c_dt) return c_t;
+  float t = 0.f;
+  for (int32_t i = 0; i < steps; ++i) t = t + dt;
+  c_steps = steps;
+  c_dt = dt_bits;
+  c_t = t;
+  return t;
+}
+
+// Bounds on the rise of t over `steps` - 1 further steps of t += dt in float32, for walk-loop
+// iterations of `steps` steps. A walk's t never exceeds T = t after n_steps - 1 steps, so
+// every rounded step adds at most dt + u/2 with u = ulp(2 max(T, n_steps dt)); the bound
+// takes dt + u per step plus 2u for the rounding of t + bound in the kernel's test, and is
+// rounded up to float.
+static void fill_t_fast(const ssm_em_problem* P, EmArgs& A) {
+  const int steps[2] = {4 * kSubBlocks<false>, 4 * kSubBlocks<true>};
+  if (P->n_steps > (1 << 24)) {
+    A.t_fast[0] = A.t_fast[1] = std::numeric_limits<float>::quiet_NaN();
+    return;
+  }
+  const double top = 2.0 * std::max((double)t_after_steps(P->n_steps - 1, P->dt), (double)P->n_steps * P->dt);
+  int e = 0;
+  std::frexp(top, &e);
+  const double u = std::ldexp(1.0, e - 24);
+  for (int r = 0; r < 2; ++r) {
+    const double b = (steps[r] - 1) * ((double)P->dt + u) + 2.0 * u;
+    A.t_fast[r] = std::nextafter((float)b, std::numeric_limits<float>::infinity());
+  }
+}
+
 int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
                     const float* table, const float* rn, const float* ru, float* rts, void* choices,
                     uint64_t* ws) {
@@ -872,6 +982,7 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   A.refill_min = refill;
   A.queue_block = (L && L->queue_block) ? L->queue_block : 64;
   A.dt = P->dt;
+  fill_t_fast(P, A);
   A.k0 = (uint32_t)P->seed;
   A.k1 = (uint32_t)(P->seed >> 32);
 
@@ -932,6 +1043,7 @@ int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, 
   A.refill_min = 1;
   A.queue_block = 1;
   A.dt = P->dt;
+  fill_t_fast(P, A);
   A.k0 = (uint32_t)P->seed;
   A.k1 = (uint32_t)(P->seed >> 32);
   hipLaunchKernelGGL(fn, dim3(1), dim3(64), 0, stream, A);

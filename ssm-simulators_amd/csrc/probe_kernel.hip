@@ -19,24 +19,36 @@
 namespace ssmhip {
 
 __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out) {
+  // the simulator's step as its iterations far from the end of t_s and from the deadline run
+  // it, in the same form (float32 rounding of every operation, no contraction; exec-mask
+  // branches): the bound test at b = a + (fx dt) slope with the step counted in float32,
+  // then the position, time and counter updates. The bounds are wide enough that no walk
+  // ends, so every lane steps every time (the simulator's walks end and are refilled).
+#pragma clang fp contract(off)
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float y = 0.1f * (float)(gid & 7), t = 0.f;
-  const float dt = 0.001f, vdt = 0.0005f, sst = 0.0316f, a = 1.5f, slope = -0.2f;
-  int ix = 0;
+  float y = 0.1f * (float)(gid & 7), t = 0.f, fx = 0.f;
+  const float dt = 0.001f, vdt = 0.0005f, sst = 0.0316f, a = 3.0e4f, slope = -0.2f;
+  bool alive = true;
   for (int64_t it = 0; it < iters; ++it) {
-    float nz[4];
-    philox_normals4((uint32_t)gid, 7u, (uint32_t)it, 0u, 0x1234u, 0x5678u, nz);
+    if (alive) {
+      float nz[4];
+      philox_normals4((uint32_t)gid, 7u, (uint32_t)it, 0u, 0x1234u, 0x5678u, nz);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float b = a + ((float)ix * dt) * slope;
-      const bool go = (fabsf(y) <= b) && (t <= 20.f);  // the simulator's bound test
-      const float yn = y + (vdt + sst * nz[i]);
-      y = go ? yn : -y;
-      t = go ? t + dt : 0.f;
-      ix = go ? ix + 1 : 0;
+      for (int i = 0; i < 4; ++i) {
+        if (alive) {
+          const float b = a + (fx * dt) * slope;
+          if (fabsf(y) <= b) {
+            y += vdt + sst * nz[i];
+            t += dt;
+            fx += 1.0f;
+          } else {
+            alive = false;
+          }
+        }
+      }
     }
   }
-  out[gid] = y + t;
+  out[gid] = y + t + fx;
 }
 
 template <bool LCA>

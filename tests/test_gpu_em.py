@@ -108,6 +108,49 @@ def test_ddm_replay_matches_oracle_trajectory_by_trajectory(torch_gpu, bname, us
     assert steps == int(s0.sum())
 
 
+@pytest.mark.parametrize("bname", ["constant", "angle"])
+def test_ddm_replay_with_binding_deadlines_and_a_short_time_grid(torch_gpu, bname):
+    """Walks that end by their deadline (t > deadline - t0) or by reaching the end of t_s,
+    both a few steps into a block or an iteration: the kernel's iterations that skip the index
+    and deadline tests must hand over to the tested steps exactly where those tests bind."""
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    K, N, L_ = 24, 400, 1031
+    print(f"[case] ddm replay deadlines + short t_s {bname}", flush=True)
+    rng = np.random.default_rng(44)
+    v = rng.uniform(-0.3, 0.3, K).astype(F32)        # slow drift: many walks reach the cut-offs
+    a = rng.uniform(1.5, 3.0, K).astype(F32)         # wide bounds
+    z = rng.uniform(0.3, 0.7, K).astype(F32)
+    t = rng.uniform(0.0, 0.5, K).astype(F32)
+    dl = (t + rng.uniform(0.005, 0.4, K)).astype(F32)  # deadlines 5 to 400 steps after t
+    dl[::4] = F32(999.0)                              # some sets run into the end of t_s instead
+    bp = {"a": a, "theta": rng.uniform(-0.1, 0.2, K).astype(F32)}
+    bp = {k: bp[k] for k in E.BOUNDARIES[bname][1]}
+    max_t = 0.35
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, dl, np.ones(K, F32), n_samples=N, max_t=max_t, delta_t=DT,
+                                 boundary=bname, bparams=bp, smooth_unif=True, normals=normals, uniforms=unif)
+    fun = {"constant": BF.constant, "angle": BF.angle}[bname]
+    code = BF.kernel_boundary_code(fun)
+    rec = em.ddm_records(v, a, z, t, dl, np.ones(K, F32), max_t, DT, code, bp)
+    job = em.make_job(L.MODEL_DDM, code, 1, rec, em.uniform_offsets(K, N), delta_t=DT, max_t=max_t, seed=1,
+                      smooth_unif=True, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    r1, c1 = job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy()
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(r0, r1)
+    assert job.steps() == int(s0.sum())
+    # the cut-offs bind: walks omitted by their deadline, and walks that ran to the end of t_s
+    n_steps = int(round(max_t / DT)) + 1
+    assert np.sum(r1 == -999.0) > 0.05 * r1.size
+    assert np.any(s0 >= n_steps - 1)
+
+
 def test_ddm_start_point_rounding_matches_oracle(torch_gpu):
     # Sets whose walk length depends on the last bit of the start point: v = 0, s = 0 keeps y at
     # its start and the angle bound falls by half an ulp per step (tests/em_cases.py). The DDM
