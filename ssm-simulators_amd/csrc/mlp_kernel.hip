@@ -149,11 +149,10 @@ __global__ void mlp_pack_kernel(const float* __restrict__ P, __bf16* __restrict_
 }
 
 // ---- forward (inference) --------------------------------------------------------------
-// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32. kFwdRowTiles row tiles (32 rows) per wave.
+// x: [n][7] fp32 (theta..., rt, choice), y: [n] fp32, 32 rows per wave and loop iteration.
 // xnorm (optional): shift[7], scale[7]; the network sees (x - shift) * scale.
-// The forward fragments (78 KB of bf16) and biases are staged once per block in LDS, so
-// the per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
-constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
+// Each block builds its image of the forward weights (68 KB of bf16) in LDS from the packed
+// fragments once, so the per-row weight traffic is LDS bandwidth rather than L2 bandwidth.
 
 // Inference carries each hidden unit as u = 1 / (2^zs + 1) = (1 - tanh z) / 2, one
 // exp2, add and reciprocal, instead of tanh z = 1 - 2u (one FMA more per unit). The layer
@@ -163,108 +162,136 @@ constexpr int kFwdFrag = fB1;  // forward fragments occupy [0, fB1)
 // bf16, so the only change against carrying tanh is where bf16 rounds: u instead of tanh z.
 // The block prologue builds -2 P and bias' in LDS from the packed weights.
 
-// Row tiles (16 rows each) per wave in the forward kernel. Two tiles keep the kernel at
-// 120 VGPRs, so the LDS limit (two 512-thread blocks per CU, four waves per SIMD) binds
-// rather than registers; three tiles (165 VGPRs) leave two waves per SIMD and were slower.
-constexpr int kFwdRowTiles = 2;
+// The forward kernel works on 32 x 32 tiles with v_mfma_f32_32x32x16_bf16 (A 32 x 16,
+// B 16 x 32, D 32 x 32; lane l = 32 h + n):
+//   A: lane l holds A[m = l % 32][k = 8 h + i], i < 8;   B: B[k = 8 h + i][n = l % 32];
+//   D: 16 floats, register r = D[m = 8 (r / 4) + 4 h + r % 4][n = l % 32].
+// Every layer computes Z^T = W H^T (M = units, N = 32 rows, K = input units), so register r
+// of output tile t holds unit 32 t + 8 (r / 4) + 4 h + r % 4 of row n. Registers 0..7 and
+// 8..15 of tile t are K blocks 2t and 2t+1 of the next layer as they stand (8 bf16 per lane):
+// K block b lists, for lane half h and item i, unit fwd_unit(b, h, i), and the weights in the
+// block's LDS image are laid out in that K order.
+//
+// Padding is skipped per register: the last tile of the 100-unit layers keeps registers
+// 0..3 (units 96..103), that of the 120-unit layer registers 0..11 (units 96..119); the other
+// registers are never computed and enter the next layer as 0. Against 16 x 16 tiles, where
+// padding fills whole lanes, this computes 328 instead of 352 units per row, and one MFMA
+// covers 32 rows instead of 16 (68 instead of 142 per 32 rows).
+//
+// The biases ride in the products. Input slots 7 and 8 of layer 1 are constants 1, with the
+// bias split into bf16 high and low parts as their weights (hi + lo is the FP32 bias to
+// 2^-16 relative). Units 100 and 101 of layers 1 and 2, padding of the network, are
+// constant-one units: weight -200 from a constant input gives u = 1 / (2^-200 + 1) = 1
+// exactly, and the next layer carries its bias' (high, low) as their weights. The output
+// layer's bias' is added in FP32.
+constexpr int kF32L1 = 0, kF32L2 = 4, kF32L3 = kF32L2 + 28, kF32L4 = kF32L3 + 28, kF32N = kF32L4 + 8;
+constexpr int kImg32 = kF32N * kFrag;  // bf16 elements of a block's LDS image (69,632 B)
+constexpr int kConstU0 = 100, kConstU1 = 101;
+static_assert(kH1 <= kConstU0 && kH2 <= kConstU0, "the constant-one units sit in the padding of layers 1 and 2");
+static_assert(kH3 <= 120, "the output layer reads registers 0..11 of layer 3's last tile");
 
-// u of output tile a of every row tile; once a pair (2b, 2b+1) is done it is packed as
-// K block b of the next layer. With NA = 7 the all-padding tile 7 is zero and packs as 0
-// (its weights in the next layer are zero, so any value would do). The wave raises its
-// issue priority for the activation: a SIMD's VALU is the busier pipe, and a wave with
-// activations ready goes first while the other waves' MFMAs run on the matrix core.
-template <int NA, int RT>
-__device__ __forceinline__ void activate_tile(f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4], int a) {
-  __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-  for (int t = 0; t < RT; ++t) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[t][a][i] = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(acc[t][a][i]) + 1.0f);
-    if (a & 1) Bn[t][a >> 1] = pack_pair(acc[t][a - 1], acc[t][a]);
-    else if (a == NA - 1) Bn[t][a >> 1] = pack_pair(acc[t][a], f32x4{0.f, 0.f, 0.f, 0.f});
-  }
-  __builtin_amdgcn_s_setprio(0);
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int fwd_unit(int b, int h, int i) { return 16 * b + 8 * (i >> 2) + 4 * h + (i & 3); }
+__device__ __forceinline__ bf16x8 frag32(const __bf16* img, int f, int lane) {
+  return *reinterpret_cast<const bf16x8*>(img + (f * 64 + lane) * 8);
 }
 
-// One layer for all row tiles: output tiles 0..NA-1 from K blocks B (KB of them). The
-// activation of output tile a-1 runs beside the MFMAs of tile a, and the weight fragments
-// and bias of tile a+1 are read from LDS while tile a runs (two register sets, 16 VGPRs
-// more; the kernel stays within 128). A layer leaves its last tile's activation to the next
-// layer (NAP = the previous layer's tile count, 0 for the first), which runs it beside the
-// MFMAs of its first tile's K blocks 0..KB-2: only K block KB-1 reads that tile's units, so
-// no layer boundary leaves the matrix core or the VALU without independent work.
-template <int NA, int KB, int NAP, int RT>
-__device__ __forceinline__ void hidden_layer(const __bf16* frags, int fo, const float* bias, int bo, int lane, int g,
-                                             bf16x8 (&B)[RT][4], f32x4 (&acc)[RT][8], bf16x8 (&Bn)[RT][4]) {
-  bf16x8 w[2][KB];
-  f32x4 bt[2];
+// u = 1 / (2^zs + 1) of registers 0..R-1 of a tile
+template <int R>
+__device__ __forceinline__ void activate32(f32x16& c) {
 #pragma unroll
-  for (int b = 0; b < KB; ++b) w[0][b] = load_frag(frags, fo, b, lane);
-  bt[0] = bias_tile(bias, bo, 0, g);
+  for (int r = 0; r < R; ++r) c[r] = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(c[r]) + 1.0f);
+}
+// K blocks 2t and 2t+1 of the next layer from tile t; registers from R on are padding: 0
+template <int R>
+__device__ __forceinline__ void pack32(const f32x16& c, bf16x8& lo, bf16x8& hi) {
 #pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    const int cur = a & 1;
-    if (a + 1 < NA) {
+  for (int i = 0; i < 8; ++i) lo[i] = i < R ? (__bf16)c[i] : (__bf16)0.f;
 #pragma unroll
-      for (int b = 0; b < KB; ++b) w[cur ^ 1][b] = load_frag(frags, fo, (a + 1) * KB + b, lane);
-      bt[cur ^ 1] = bias_tile(bias, bo, a + 1, g);
-    }
-    f32x4 c[RT];
+  for (int i = 0; i < 8; ++i) hi[i] = 8 + i < R ? (__bf16)c[8 + i] : (__bf16)0.f;
+}
+
+// One hidden layer, 4 output tiles from KB input K blocks in Bin. Bin's blocks 6 and 7 come
+// from the previous layer's last tile `last` (RPREV real registers; 0: none pending), whose
+// activation runs beside the MFMAs of K blocks 0..5 of this layer's first tile: only blocks
+// 6 and 7 read it. The activation of tile t-1 runs after the MFMAs of tile t are issued. On
+// return `last` holds tile 3, whose activation is left to the next layer.
+template <int KB, int RPREV>
+__device__ __forceinline__ void hidden32(const __bf16* img, int f0, int lane, bf16x8 (&Bin)[8], f32x16& last,
+                                         bf16x8 (&Bout)[8]) {
+  f32x16 prev;
 #pragma unroll
-    for (int t = 0; t < RT; ++t) c[t] = bt[cur];
+  for (int t = 0; t < 4; ++t) {
+    f32x16 c = {};
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
-      if constexpr (NAP > 0) {
-        if (a == 0 && b == KB - 1) activate_tile<NAP>(acc, B, NAP - 1);
+      if constexpr (RPREV > 0) {
+        if (t == 0 && b == 6) {
+          activate32<RPREV>(last);
+          pack32<RPREV>(last, Bin[6], Bin[7]);
+        }
       }
-#pragma unroll
-      for (int t = 0; t < RT; ++t) c[t] = mfma(w[cur][b], B[t][b], c[t]);
+      c = mfma32(frag32(img, f0 + t * KB + b, lane), Bin[b], c);
     }
-#pragma unroll
-    for (int t = 0; t < RT; ++t) acc[t][a] = c[t];
-    if (a > 0) activate_tile<NA>(acc, Bn, a - 1);
+    if (t > 0) {
+      activate32<16>(prev);
+      pack32<16>(prev, Bout[2 * (t - 1)], Bout[2 * (t - 1) + 1]);
+    }
+    prev = c;
   }
+  last = prev;
 }
 
-// The input transform: shift, scale, rt mode and t index are read once per block (the
-// identity when no transform is given, which leaves every input unchanged: (v - 0) * 1 = v).
-struct InputNorm {
-  float shift[kIn], scale[kIn], mode;
-  int tix;
-};
-__device__ __forceinline__ float input_value_r(const InputNorm& q, int j, const float* xr) {
-  float v = xr[j];
-  if (j == 5) {
-    if (q.mode == 2.f) v -= xr[q.tix];
-    if (q.mode != 0.f) v = logf(fmaxf(v, kRtFloor));
-  }
-  return (v - q.shift[j]) * q.scale[j];
+// bf16 high and low parts of v: v - hi is exact in FP32
+__device__ __forceinline__ void split_bf16(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
 }
 
-__global__ void __launch_bounds__(512, 2) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
+__global__ void __launch_bounds__(1024, 1) mlp_forward_kernel(const float* __restrict__ x, int64_t n,
                                                           const __bf16* __restrict__ gfrags,
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ xnorm, float* __restrict__ y) {
-  __shared__ __attribute__((aligned(16))) __bf16 lfrag[kFwdFrag];
+  __shared__ __attribute__((aligned(16))) __bf16 img[kImg32];
   __shared__ __attribute__((aligned(16))) float lbias[kNBias];
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int lane = threadIdx.x & 63, h = lane >> 5, nr = lane & 31;
   {
-    // layer 1 fragments as packed; layers 2..4 (they consume u) as -2 P
-    const int4* src = reinterpret_cast<const int4*>(gfrags);
-    int4* dst = reinterpret_cast<int4*>(lfrag);
-    for (int i = threadIdx.x; i < kFwdFrag / 8; i += blockDim.x) {
-      if (i < fF2 / 8) {
-        dst[i] = src[i];
-      } else {
-        bf16x8 v = *reinterpret_cast<const bf16x8*>(src + i);
+    // 1. the image from the packed 16 x 16 fragments, four K slots (one 8-byte run of the
+    //    source) at a time; layers 2..4 consume u, so they take -2 P
+    for (int c = threadIdx.x; c < kF32N * 128; c += blockDim.x) {
+      const int f = c >> 7, l = (c >> 1) & 63, i0 = 4 * (c & 1);
+      const int m = l & 31, hl = l >> 5;
+      int layer, t, b;
+      if (f < kF32L2) { layer = 1; t = f; b = 0; }
+      else if (f < kF32L3) { layer = 2; t = (f - kF32L2) / 7; b = (f - kF32L2) % 7; }
+      else if (f < kF32L4) { layer = 3; t = (f - kF32L3) / 7; b = (f - kF32L3) % 7; }
+      else { layer = 4; t = 0; b = f - kF32L4; }
+      const int out = 32 * t + m;
+      uint2 v = make_uint2(0u, 0u);
+      if (layer == 1) {
+        const int k0 = 8 * hl + i0;  // inputs k0..k0+3 (slot k = input k)
+        v = *reinterpret_cast<const uint2*>(gfrags + fF1 + ((out >> 4) * 64 + (out & 15) + 16 * (k0 >> 3)) * 8 + (k0 & 7));
+      } else if (layer < 4 || out < 16) {
+        const int u0 = fwd_unit(b, hl, i0);  // units u0..u0+3
+        const int base = layer == 2 ? fF2 : (layer == 3 ? fF3 : fF4);
+        const int lsrc = (out & 15) + 16 * ((u0 >> 2) & 3), j0 = 4 * ((u0 >> 4) & 1);
+        uint2 w = *reinterpret_cast<const uint2*>(gfrags + base + (((out >> 4) * 4 + (u0 >> 5)) * 64 + lsrc) * 8 + j0);
+        __bf16 q[4];
+        __builtin_memcpy(q, &w, 8);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(-2.0f * (float)v[j]);
-        *reinterpret_cast<bf16x8*>(dst + i) = v;
+        for (int k = 0; k < 4; ++k) q[k] = (__bf16)(-2.0f * (float)q[k]);
+        __builtin_memcpy(&v, q, 8);
       }
+      *reinterpret_cast<uint2*>(img + (f * 64 + l) * 8 + i0) = v;
     }
+    // 2. bias' = bias + row sums of P for layers 2..4 (MFMAs against all-ones columns over
+    //    the 16 x 16 fragments: 17 output tiles, one per wave in turn); layer 1 takes bias
     for (int i = threadIdx.x; i < pB2; i += blockDim.x) lbias[i] = bias[i];
-    // bias' = bias + row sums of P, from MFMAs against all-ones columns: 17 output
-    // tiles (8 of layer 2, 8 of layer 3, the output tile), one per wave in turn
+    const int g = lane >> 4, r = lane & 15;
     bf16x8 ones;
 #pragma unroll
     for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
@@ -282,70 +309,95 @@ __global__ void __launch_bounds__(512, 2) mlp_forward_kernel(const float* __rest
     }
   }
   __syncthreads();
-  constexpr int RT = kFwdRowTiles;
-  InputNorm qn;
+  // 3. the bias slots: layer 1's constant inputs 7 and 8, and the constant-one units 100
+  //    and 101 as inputs of layers 2 and 3 (K block 6, lane half 1, items 0 and 1)
+  for (int e = threadIdx.x; e < 3 * 128; e += blockDim.x) {
+    const int layer = 1 + e / 128, out = e & 127, t = out >> 5, m = out & 31;
+    const bool konst = layer < 3 && (out == kConstU0 || out == kConstU1);
+    __bf16 hi, lo;
+    if (konst) {
+      hi = (__bf16)-200.0f;  // exact in bf16
+      lo = (__bf16)0.0f;
+    } else {
+      split_bf16(lbias[(layer == 1 ? pB1 : (layer == 2 ? pB2 : pB3)) + out], hi, lo);
+    }
+    if (layer == 1) {
+      img[(kF32L1 + t) * kFrag + m * 8 + 7] = hi;          // slot 7: lane m, item 7
+      img[(kF32L1 + t) * kFrag + (32 + m) * 8 + 0] = lo;   // slot 8: lane 32 + m, item 0
+    } else {
+      const int f = (layer == 2 ? kF32L2 : kF32L3) + t * 7 + 6;
+      img[f * kFrag + (32 + m) * 8 + 0] = hi;  // unit 100
+      img[f * kFrag + (32 + m) * 8 + 1] = lo;  // unit 101
+    }
+  }
+  __syncthreads();
+  const float bias4 = lbias[pB4];
+  // the input transform, read once per block (the identity when no transform is given:
+  // (v - 0) * 1 = v)
+  float shift[kIn], scale[kIn], mode = 0.f;
+  int tix = 0;
 #pragma unroll
   for (int j = 0; j < kIn; ++j) {
-    qn.shift[j] = 0.f;
-    qn.scale[j] = 1.f;
+    shift[j] = 0.f;
+    scale[j] = 1.f;
   }
-  qn.mode = 0.f;
-  qn.tix = 0;
   if (xnorm) {
 #pragma unroll
     for (int j = 0; j < kIn; ++j) {
-      qn.shift[j] = xnorm[j];
-      qn.scale[j] = xnorm[kIn + j];
+      shift[j] = xnorm[j];
+      scale[j] = xnorm[kIn + j];
     }
-    qn.mode = xnorm[2 * kIn];
-    qn.tix = t_index(xnorm);
+    mode = xnorm[2 * kIn];
+    tix = t_index(xnorm);
   }
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r0 = wave * (16 * RT); r0 < n; r0 += nwaves * (16 * RT)) {
-    // an offset the compiler cannot see through keeps the LDS fragment and bias reads
-    // inside the row loop (hoisting them would need hundreds of registers)
+  for (int64_t r0 = wave * 32; r0 < n; r0 += nwaves * 32) {
+    // an offset the compiler cannot see through keeps the LDS fragment reads inside the row
+    // loop (hoisting them would need hundreds of registers)
     int lo = 0;
     asm volatile("" : "+v"(lo));
-    const __bf16* frags = lfrag + lo;
-    const float* lb = lbias + lo;
-    bf16x8 B[RT][4], Bn[RT][4];
-    f32x4 acc[RT][8];
-    // layer 1 inputs: lane group 0 holds the 7 inputs of its row, other groups zero
-#pragma unroll
-    for (int t = 0; t < RT; ++t) {
-      const int64_t row = r0 + 16 * t + r;
+    const __bf16* fi = img + lo;
+    const int64_t row = r0 + nr;
+    bf16x8 B[8], Bn[8];
+    {
+      // layer 1 input slots: lane half 0 holds inputs 0..6 of its row and the constant 1 of
+      // slot 7, lane half 1 the constant 1 of slot 8 and zeros
       bf16x8 v = {};
-      if (g == 0 && row < n) {
-        const float* xr = x + row * kIn;
+      if (h == 0) {
+        if (row < n) {
+          const float* xr = x + row * kIn;
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) v[j] = (__bf16)input_value_r(qn, j, xr);
+          for (int j = 0; j < kIn; ++j) {
+            float q = xr[j];
+            if (j == 5) {
+              if (mode == 2.f) q -= xr[tix];
+              if (mode != 0.f) q = logf(fmaxf(q, kRtFloor));
+            }
+            v[j] = (__bf16)((q - shift[j]) * scale[j]);
+          }
+        }
+        v[7] = (__bf16)1.0f;
+      } else {
+        v[0] = (__bf16)1.0f;
       }
-      B[t][0] = v;
+      B[0] = v;
     }
-    // Output tile 7 of the 100-unit layers 1 and 2 (units 112..127) is all padding: its
-    // weights and biases are zero and the next layer's weights for it are zero, so it
-    // need not be computed.
-    hidden_layer<7, 1, 0>(frags, fF1, lb, pB1, lane, g, B, acc, Bn);
-    hidden_layer<7, 4, 7>(frags, fF2, lb, pB2, lane, g, Bn, acc, B);
-    hidden_layer<8, 4, 7>(frags, fF3, lb, pB3, lane, g, B, acc, Bn);
-    // output layer, with layer 3's last activation beside its K blocks 0..2
-    f32x4 c[RT];
-    const f32x4 bt = bias_tile(lb, pB4, 0, g);
+    f32x16 last;
+    hidden32<1, 0>(fi, kF32L1, lane, B, last, Bn);
+    hidden32<7, 4>(fi, kF32L2, lane, Bn, last, B);
+    hidden32<7, 4>(fi, kF32L3, lane, B, last, Bn);
+    // output layer: one tile (unit 0), K blocks 0..7; layer 3's last tile keeps registers 0..11
+    f32x16 c = {};
 #pragma unroll
-    for (int t = 0; t < RT; ++t) c[t] = bt;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b == 3) activate_tile<8>(acc, Bn, 7);
-      const bf16x8 w = load_frag(frags, fF4, b, lane);
-#pragma unroll
-      for (int t = 0; t < RT; ++t) c[t] = mfma(w, Bn[t][b], c[t]);
+    for (int b = 0; b < 8; ++b) {
+      if (b == 6) {
+        activate32<12>(last);
+        pack32<12>(last, Bn[6], Bn[7]);
+      }
+      c = mfma32(frag32(fi, kF32L4 + b, lane), Bn[b], c);
     }
-#pragma unroll
-    for (int t = 0; t < RT; ++t) {
-      const int64_t row = r0 + 16 * t + r;
-      if (g == 0 && row < n) y[row] = c[t][0];
-    }
+    if (h == 0 && row < n) y[row] = c[0] + bias4;
   }
 }
 
@@ -923,12 +975,12 @@ int32_t mlp_forward_enqueue(const float* x, int64_t n, const __bf16* frags, cons
   if (!x || !frags || !bias || !y) return fail(SSM_EINVAL, "NULL buffer");
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
-  // 512-thread blocks, two resident per CU (2 x 78 KB of LDS weights), kFwdRowTiles x 16 rows per wave
-  const int64_t waves_needed = (n + 16 * kFwdRowTiles - 1) / (16 * kFwdRowTiles);
-  int64_t blocks = (waves_needed + 7) / 8;
-  const int64_t cap = (int64_t)cus * 2;
+  // 1024-thread blocks, one per CU (71 KB of LDS image and biases), 32 rows per wave
+  const int64_t waves_needed = (n + 31) / 32;
+  int64_t blocks = (waves_needed + 15) / 16;
+  const int64_t cap = (int64_t)cus;
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(512), 0, s, x, n, frags, bias, xnorm, y);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(1024), 0, s, x, n, frags, bias, xnorm, y);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }
