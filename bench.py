@@ -452,12 +452,12 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
     kde_at[order] = kll
     kde_mae = float(np.mean(np.abs(kde_at - ref)))
     # roof: the surrogate's hidden unit u = 1 / (2^z + 1) (exp2 + rcp on the transcendental
-    # unit) at full occupancy; the forward computes 352 units per row (112 + 112 + 128 with
-    # padding), MFMA runs beside them
+    # unit) at full occupancy, over the network's 320 hidden units per row (100 + 100 + 120;
+    # the forward computes 328, padding included), MFMA running beside them
     unit_rate = None if args.no_probe else roofline.probe_rate("hidden_unit", iters=2048)
     rec_roof = None if unit_rate is None else {
-        "bound": "transcendental (hidden units)", "achieved": evals / 1e9, "peak": unit_rate / 352 / 1e9,
-        "unit": "G rows/s", "frac": evals / (unit_rate / 352), "hidden_units_per_s": unit_rate}
+        "bound": "transcendental (hidden units)", "achieved": evals / 1e9, "peak": unit_rate / 320 / 1e9,
+        "unit": "G rows/s", "frac": evals / (unit_rate / 320), "hidden_units_per_s": unit_rate}
     rec = {"inference_roofline": rec_roof, "label_sets": args.surrogate_sets, "label_samples_per_set": args.surrogate_samples,
            "label_s": label_s, "train_rows": int(train.shape[0]), "epochs": args.surrogate_epochs, "train_s": train_s,
            "train_rows_per_s": train.shape[0] * args.surrogate_epochs / train_s, "huber_history": hist[-3:],
@@ -468,11 +468,11 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
            "mae_vs_analytic_theta0_trained_on_1024x1e5_rows": mae_c4}
     rec["inference_published_peak"] = {
         "mfma_bf16": {"unit": "TFLOP/s", "achieved": flops / 1e12, "peak": 2500.0, "frac": flops / 2.5e15,
-                      "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 36.3% of SIMD cycles "
+                      "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 40.4% of SIMD cycles "
                                "(profiles/mlp_forward_4M_counters.md)"},
-        "transcendental": {"unit": "T lane-instructions/s", "achieved": 708 * evals / 1e12,
-                           "peak": PEAK_TRANS_PER_S / 1e12, "frac": 708 * evals / PEAK_TRANS_PER_S,
-                           "basis": "708 v_exp_f32 + v_rcp_f32 per row (counters)"},
+        "transcendental": {"unit": "T lane-instructions/s", "achieved": 658 * evals / 1e12,
+                           "peak": PEAK_TRANS_PER_S / 1e12, "frac": 658 * evals / PEAK_TRANS_PER_S,
+                           "basis": "658 v_exp_f32 + v_rcp_f32 per row (counters)"},
     }
     log(f"surrogate: {evals:.3e} evals/s ({rec['over_gpu_kde']:.0f}x GPU KDE), MAE vs analytic {mae:.3f} nats "
         f"(KDE labels: {kde_mae:.3f}; trained on the 1024 x 1e5 rows: {mae_c4})")

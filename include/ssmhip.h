@@ -124,7 +124,8 @@ typedef struct ssm_launch_config {
 /* 3: ssm_em_problem.set_stride (compact 16-float records for the single-particle
  *    diffusions; 0 keeps SSM_SET_STRIDE, the layout of version 2).
  * 4: ssm_lan_assemble takes the negative-RT rows (n_neg, n_choices, neg_ll);
- *    ssm_copy_rows, ssm_em_trajectory, ssm_kde_eval_noalloc.                  */
+ *    ssm_copy_rows, ssm_em_trajectory, ssm_kde_eval_noalloc.
+ * 5: ssm_mfma32_probe.                                                       */
 int32_t ssm_abi_version(void);
 const char* ssm_last_error(void);
 /* Number of CUs of the current device (0 and an error when no GPU).          */
@@ -389,6 +390,9 @@ int32_t ssm_mlp_train_steps(const float* x_dev, const float* target_dev, int64_t
 /* One v_mfma_f32_16x16x32_bf16 with per-lane operands a[64][8], b[64][8]
  * (bf16) and result d[64][4]: pins the operand / result lane layout.        */
 int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream);
+/* One v_mfma_f32_32x32x16_bf16 (the inference kernel's) with per-lane operands a[64][8],
+ * b[64][8] (bf16) and result d[64][16]: pins that operand / result lane layout.      */
+int32_t ssm_mfma32_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream);
 
 /* ---- VALU roofline probes (no counterpart in the reference) ----------------
  * Branch-free loops at the occupancy of `n_threads` lanes (multiple of 256),

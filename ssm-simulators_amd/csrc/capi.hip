@@ -105,6 +105,7 @@ int32_t mlp_train_steps_enqueue(const float* x, const float* target, int64_t bat
 int32_t mlp_adam_enqueue(float* P, const float* G, float* M, float* V, int32_t step, float lr, float b1, float b2,
                          float eps, float wd, hipStream_t s);
 int32_t mfma_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s);
+int32_t mfma32_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s);
 
 
 struct GraphHandle {
@@ -118,7 +119,7 @@ using namespace ssmhip;
 
 extern "C" {
 
-int32_t ssm_abi_version(void) { return 4; }
+int32_t ssm_abi_version(void) { return 5; }
 
 const char* ssm_last_error(void) { return g_last_error.c_str(); }
 
@@ -340,6 +341,10 @@ int32_t ssm_mlp_train_steps(const float* x_dev, const float* target_dev, int64_t
 
 int32_t ssm_mfma_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream) {
   return mfma_probe_enqueue(a_dev, b_dev, d_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_mfma32_probe(const void* a_dev, const void* b_dev, float* d_dev, void* stream) {
+  return mfma32_probe_enqueue(a_dev, b_dev, d_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_valu_probe(int32_t kind, int64_t iters, int64_t n_threads, float* out_dev, void* stream) {

@@ -944,10 +944,23 @@ __global__ void mfma_probe_kernel(const bf16x8* a, const bf16x8* b, f32x4* d) {
   d[l] = mfma(a[l], b[l], f32x4{0.f, 0.f, 0.f, 0.f});
 }
 
+// the same for the inference kernel's 32 x 32 x 16 MFMA (result d[64][16])
+__global__ void mfma32_probe_kernel(const bf16x8* a, const bf16x8* b, f32x16* d) {
+  const int l = threadIdx.x;
+  d[l] = mfma32(a[l], b[l], f32x16{});
+}
+
 // ---- host -------------------------------------------------------------------------------------
 int32_t mfma_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s) {
   if (!a || !b || !d) return fail(SSM_EINVAL, "NULL buffer");
   hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, (const bf16x8*)a, (const bf16x8*)b, (f32x4*)d);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+int32_t mfma32_probe_enqueue(const void* a, const void* b, float* d, hipStream_t s) {
+  if (!a || !b || !d) return fail(SSM_EINVAL, "NULL buffer");
+  hipLaunchKernelGGL(mfma32_probe_kernel, dim3(1), dim3(64), 0, s, (const bf16x8*)a, (const bf16x8*)b, (f32x16*)d);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

@@ -138,6 +138,7 @@ _SIGS = {
         C.c_int32, [C.c_int32, C.c_int32, _VP, _VP, _VP, C.c_int32, _VP, C.c_int64, _VP, C.c_size_t, _VP, _VP]
     ),
     "ssm_mfma_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
+    "ssm_mfma32_probe": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_valu_probe": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, _VP, _VP]),
     "ssm_philox_fill": (
         C.c_int32,

@@ -36,7 +36,7 @@ def test_abi_version_and_error_string_without_gpu():
     from ssmhip import _lib
 
     lib = _lib.load()
-    assert lib.ssm_abi_version() == 4
+    assert lib.ssm_abi_version() == 5
     assert isinstance(lib.ssm_last_error(), bytes)
 
 

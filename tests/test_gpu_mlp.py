@@ -56,6 +56,33 @@ def test_mfma_lane_layout(torch_gpu):
             assert abs(d[l, i] - ref[4 * (l // 16) + i, l % 16]) < 1e-3
 
 
+def test_mfma32_lane_layout(torch_gpu):
+    """The inference kernel's v_mfma_f32_32x32x16_bf16: A[m = l % 32][k = 8 (l // 32) + i],
+    B[k = 8 (l // 32) + i][n = l % 32], result register r = D[8 (r // 4) + 4 (l // 32) + r % 4][l % 32]."""
+    from ssmhip import _device as D
+    from ssmhip import _lib as L
+
+    torch = torch_gpu
+    rng = np.random.default_rng(1)
+    A = mlp_ref.bf16_round(rng.normal(size=(32, 16)).astype(np.float32))
+    B = mlp_ref.bf16_round(rng.normal(size=(16, 32)).astype(np.float32))
+    a = np.zeros((64, 8), np.float32)
+    b = np.zeros((64, 8), np.float32)
+    for l in range(64):
+        for j in range(8):
+            a[l, j] = A[l % 32, 8 * (l // 32) + j]
+            b[l, j] = B[8 * (l // 32) + j, l % 32]
+    da = D.to_dev(a).to(torch.bfloat16)
+    db = D.to_dev(b).to(torch.bfloat16)
+    dd = D.empty((64, 16), torch.float32)
+    L.check(L.load().ssm_mfma32_probe(D.ptr(da), D.ptr(db), D.ptr(dd), D.stream_ptr()))
+    d = dd.cpu().numpy()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    for l in range(64):
+        for r in range(16):
+            assert abs(d[l, r] - ref[8 * (r // 4) + 4 * (l // 32) + r % 4, l % 32]) < 1e-3
+
+
 def test_forward_matches_oracle_and_torch(torch_gpu):
     from ssmhip import _device as D
     from ssmhip.surrogate import MLPSurrogate
