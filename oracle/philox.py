@@ -50,10 +50,18 @@ def unit(u: np.ndarray) -> np.ndarray:
     return (u >> np.uint32(8)).astype(np.float64) * 2.0**-24
 
 
-def box_muller(a: np.ndarray, b: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """Box & Muller (1958): r = sqrt(-2 ln u1), angle 2*pi*u2 (float64)."""
-    r = np.sqrt(-2.0 * np.log(open01(a)))
-    th = 2.0 * np.pi * unit(b)
+def rev(u: np.ndarray) -> np.ndarray:
+    """Box–Muller angle in revolutions: top 23 bits * 2^-23, in [0, 1). The kernels place the
+    same bits under the exponent of 1.0f (a value in [1, 2)) and let the hardware sin/cos take
+    the fraction, which is exact."""
+    return (u >> np.uint32(9)).astype(np.float64) * 2.0**-23
+
+
+def box_muller(a: np.ndarray, b: np.ndarray, s: float = 1.0) -> tuple[np.ndarray, np.ndarray]:
+    """Box & Muller (1958): r = sqrt(-2 ln u1) s, angle 2*pi*u2 (float64); s scales both
+    normals as the kernels' radius sqrt(-2 ln u1 s^2) does."""
+    r = np.sqrt(-2.0 * np.log(open01(a))) * abs(s)
+    th = 2.0 * np.pi * rev(b)
     return r * np.cos(th), r * np.sin(th)
 
 

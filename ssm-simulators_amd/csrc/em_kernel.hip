@@ -111,6 +111,21 @@ __device__ __forceinline__ float step_increment(float y, float vdt, float v, flo
   if constexpr (MODEL == SSM_MODEL_ORNSTEIN) return ((v - (leak * y)) * dt) + (sst * nz);
   else return vdt + sst * nz;
 }
+// Native walks of the Gaussian diffusions draw their normals already scaled by sqrt_st
+// (Box–Muller radius sqrt(-2 ln u1 sqrt_st^2)): the same law as sqrt_st g, one multiply
+// less per step. Replay walks take the reference's product of the buffered draw.
+template <int MODEL, bool REPLAY>
+constexpr bool kScaledNormals = !REPLAY && MODEL != SSM_MODEL_LEVY;
+template <int MODEL, bool REPLAY>
+__device__ __forceinline__ float walk_increment(float y, float vdt, float v, float leak, float dt, float sst,
+                                                float nz) {
+  if constexpr (kScaledNormals<MODEL, REPLAY>) {
+    if constexpr (MODEL == SSM_MODEL_ORNSTEIN) return ((v - (leak * y)) * dt) + nz;
+    else return vdt + nz;
+  } else {
+    return step_increment<MODEL>(y, vdt, v, leak, dt, sst, nz);
+  }
+}
 
 // ---- float32 forms of the reference's double-precision expressions ----------------
 // A few reference lines mix float operands with C double literals (0.5, 2.0) or libc's
@@ -234,6 +249,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   int ix = 0;
   // per-trajectory constants
   float vdt[MAXP], sst[MAXP], vraw[MAXP];
+  float bmc = kBoxMullerLog2;  // native Gaussian diffusions: -2 ln 2 sqrt_st^2 (normals scaled by sqrt_st)
   float a = 0.f, b1 = 0.f, b2 = 0.f, t0 = 0.f, deadline = 0.f, dl_tmp = 0.f, leak = 0.f, inh = 0.f;
   float alpha = 2.f, alpha_inv = 0.5f, alpha_scale = -0.5f;  // Levy stability and its two exponents
   int win = -1;  // racing diffusion: first accumulator past the threshold in its last step
@@ -394,6 +410,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               const float z = sp[SSM_P_Z];
               vdt[0] = sp[SSM_P_V] * dt;
               sst[0] = sp[SSM_P_SQRT_ST];
+              if constexpr (kScaledNormals<MODEL, REPLAY>) bmc = kBoxMullerLog2 * (sst[0] * sst[0]);
               if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) {
                 x[0] = z * b0;  // ddm_models.pyx:457: start z a, in float
               } else if constexpr (MODEL == SSM_MODEL_FULL_DDM) {
@@ -548,16 +565,16 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               nz[2] = stable_cms(r1.x, r1.y, alpha, alpha_inv, alpha_scale);
               nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
             } else if constexpr (kTwoStage<MODEL>) {
-              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ixb - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
+              philox_normals4_c((uint32_t)n, (uint32_t)k, (uint32_t)((ixb - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, bmc, nz);
             } else {
-              philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ixb >> 2), 0u, A.k0, A.k1, nz);
+              philox_normals4_c((uint32_t)n, (uint32_t)k, (uint32_t)(ixb >> 2), 0u, A.k0, A.k1, bmc, nz);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               if (alive) {
                 const float b = boundary_at_f<BOUND>(fx, ixb + i, dt, a, b1, b2, row);
                 if (inside<MODEL>(x[0], b)) {
-                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  x[0] += walk_increment<MODEL, REPLAY>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   fx += 1.0f;
                 } else {
@@ -595,9 +612,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             nz[3] = stable_cms(r1.z, r1.w, alpha, alpha_inv, alpha_scale);
           } else if constexpr (kTwoStage<MODEL>) {
             // each stage's draws start block-aligned in a stream of their own (0 high, 4 low)
-            philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)((ix - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, nz);
+            philox_normals4_c((uint32_t)n, (uint32_t)k, (uint32_t)((ix - ix0) >> 2), stage ? 4u : 0u, A.k0, A.k1, bmc, nz);
           } else {
-            philox_normals4((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, nz);
+            philox_normals4_c((uint32_t)n, (uint32_t)k, (uint32_t)(ix >> 2), 0u, A.k0, A.k1, bmc, nz);
           }
           // Steps use exec-mask branches rather than selects: with every lane usually on the
           // same side, a branch costs scalar instructions while a select costs VALU issue slots.
@@ -608,7 +625,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               if (alive) {
                 const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
                 if (inside<MODEL>(x[0], b) && (t <= dl_tmp)) {
-                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  x[0] += walk_increment<MODEL, REPLAY>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   ++ix;
                   if constexpr (RECORD) {
@@ -629,7 +646,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   go = inside<MODEL>(x[0], b) && (t <= dl_tmp);
                 }
                 if (go) {
-                  x[0] += step_increment<MODEL>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
+                  x[0] += walk_increment<MODEL, REPLAY>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   ++ix;
                   if constexpr (RECORD) {

@@ -28,17 +28,19 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float y = 0.1f * (float)(gid & 7), t = 0.f, fx = 0.f;
   const float dt = 0.001f, vdt = 0.0005f, sst = 0.0316f, a = 3.0e4f, slope = -0.2f;
+  // native walks draw normals scaled by sqrt_st (Box–Muller radius sqrt(-2 ln u1 sqrt_st^2))
+  const float bmc = kBoxMullerLog2 * (sst * sst);
   bool alive = true;
   for (int64_t it = 0; it < iters; ++it) {
     if (alive) {
       float nz[4];
-      philox_normals4((uint32_t)gid, 7u, (uint32_t)it, 0u, 0x1234u, 0x5678u, nz);
+      philox_normals4_c((uint32_t)gid, 7u, (uint32_t)it, 0u, 0x1234u, 0x5678u, bmc, nz);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (alive) {
           const float b = a + (fx * dt) * slope;
           if (fabsf(y) <= b) {
-            y += vdt + sst * nz[i];
+            y += vdt + nz[i];
             t += dt;
             fx += 1.0f;
           } else {

@@ -80,22 +80,42 @@ __device__ __forceinline__ float u32_to_unit(uint32_t u) {
   return (float)(u >> 8) * 5.9604644775390625e-08f;
 }
 
-// Box–Muller (Box & Muller 1958): two uniforms -> two independent N(0,1).
-// r = sqrt(-2 ln u1) with ln from the hardware log2; the angle is kept in
-// revolutions because v_sin_f32 / v_cos_f32 take their argument as x * 2*pi.
-__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
+// uint32 -> angle in revolutions, 1 + (top 23 bits) * 2^-23 in [1, 2): the bits placed under
+// the exponent of 1.0f by one v_alignbit_b32 (0x7F:u >> 9). v_sin_f32 / v_cos_f32 take their
+// argument as x * 2 pi and reduce it to its fraction, which is exact here (x - 1), so the
+// angle is 2 pi (top 23 bits) 2^-23.
+__device__ __forceinline__ float u32_to_rev(uint32_t u) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7Fu, u, 9u));
+}
+
+// -2 ln 2: -2 ln(u) = kBoxMullerLog2 * log2(u)
+constexpr float kBoxMullerLog2 = -1.3862943611198906f;
+
+// Box–Muller (Box & Muller 1958): two uniforms -> two independent N(0, c / (-2 ln 2)).
+// r = sqrt(c log2 u1) with the hardware log2; c = -2 ln 2 gives N(0, 1), and
+// c = -2 ln 2 s^2 gives normals already scaled by s (|s|, the same law), so a walk's step
+// needs no multiply for its noise scale.
+__device__ __forceinline__ void box_muller_c(uint32_t a, uint32_t b, float c, float& z0, float& z1) {
   const float u1 = u32_to_open01(a);
-  const float rev = u32_to_unit(b);
-  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln2 log2(u1)
+  const float rev = u32_to_rev(b);
+  const float r = __builtin_amdgcn_sqrtf(c * __builtin_amdgcn_logf(u1));
   z0 = r * __builtin_amdgcn_cosf(rev);
   z1 = r * __builtin_amdgcn_sinf(rev);
 }
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
+  box_muller_c(a, b, kBoxMullerLog2, z0, z1);
+}
 
+// four normals from one Philox4x32-10 block, scaled by s where c = -2 ln 2 s^2
+__device__ __forceinline__ void philox_normals4_c(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                  uint32_t k0, uint32_t k1, float c, float out[4]) {
+  u32x4 r = philox4x32_10(u32x4{c0, c1, c2, c3}, k0, k1);
+  box_muller_c(r.x, r.y, c, out[0], out[1]);
+  box_muller_c(r.z, r.w, c, out[2], out[3]);
+}
 __device__ __forceinline__ void philox_normals4(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                                 uint32_t k0, uint32_t k1, float out[4]) {
-  u32x4 r = philox4x32_10(u32x4{c0, c1, c2, c3}, k0, k1);
-  box_muller(r.x, r.y, out[0], out[1]);
-  box_muller(r.z, r.w, out[2], out[3]);
+  philox_normals4_c(c0, c1, c2, c3, k0, k1, kBoxMullerLog2, out);
 }
 
 // ---- wavefront helpers (wave64) ----------------------------------------------

@@ -379,6 +379,37 @@ def test_ddm_native_single_set_1e5_matches_cpu_restatement_and_closed_form(torch
         assert stats.chi2.sf(chi2, df=19) > 1e-3, (c, chi2)
 
 
+@pytest.mark.parametrize("s", [0.5, 2.0])
+def test_ddm_native_noise_scale_matches_cpu_restatement_and_closed_form(torch_gpu, s):
+    """Native walks draw their normals already scaled by sqrt(dt) s (the Box–Muller radius
+    carries s^2). At s != 1 they must match the C++/OpenMP restatement, which multiplies
+    sqrt(dt) s by a unit normal, and the Wiener closed forms of the rescaled problem
+    (v / s, a / s, unit noise), with the Siegmund correction beta sqrt(dt) on a / s."""
+    import torch
+
+    L, em = _ssm()
+    N = 100_000
+    # z = 1/2: the start stays midway between the Siegmund-corrected bounds
+    v, a, z, t = F32(0.5), F32(1.0), F32(0.5), F32(0.2)
+    rec = em.ddm_records([v], [a], [z], [t], [999.0], [s], 20.0, DT, L.BOUNDARY_CONSTANT, {"a": [a]})
+    job = em.make_job(L.MODEL_DDM, L.BOUNDARY_CONSTANT, 1, rec, em.uniform_offsets(1, N), delta_t=DT, max_t=20.0,
+                      seed=3, smooth_unif=False)
+    job.launch()
+    torch.cuda.synchronize()
+    rg, cg = job.rts[:N].cpu().numpy(), job.choices[:N].cpu().numpy()
+    table = E.boundary_table("constant", {"a": [a]}, 1, 20.0, DT)
+    sqrt_st = F32(E.sqrt_dt32(DT) * F32(s))
+    rc_, cc_, _ = cpu_baseline.em_simulate(0, [v], [z], [sqrt_st], [t], [999.0], [20.0], table,
+                                           np.array([0, N], np.int64), dt=DT, seed=777)
+    _compare_samples(rg, cg, rc_, cc_, (-1, 1))
+    a_eff = float(a) / s + ddm_analytic.SIEGMUND_BETA * math.sqrt(DT)
+    p = ddm_analytic.p_upper(float(v) / s, a_eff, float(z))
+    assert abs(np.mean(cg == 1) - p) < 3 * math.sqrt(p * (1 - p) / N)
+    dec = rg.astype(np.float64) - float(t)
+    m = ddm_analytic.mean_decision_time(float(v) / s, a_eff, float(z))
+    assert abs(dec.mean() - m) < 4 * dec.std() / math.sqrt(N) + 2 * DT, (dec.mean(), m)
+
+
 @pytest.mark.parametrize("bname", ["angle", "weibull_cdf"])
 def test_ddm_native_in_register_bounds_match_cpu_restatement(torch_gpu, bname):
     """Native Philox walks with the angle or Weibull bound evaluated in registers, against the

@@ -66,3 +66,16 @@ def test_ziggurat_normals_statistics():
         expect = 2 * stats.norm.sf(k)
         got = np.mean(np.abs(z) > k)
         assert abs(got - expect) < 5 * np.sqrt(expect / z.size) + 1e-5
+
+
+def test_box_muller_angle_bits():
+    # the angle is the top 23 bits in revolutions: 0 for u < 2^9, one step 2^-23 at u = 2^9,
+    # 1 - 2^-23 for the largest u, and 1/2 at the top bit alone
+    u = np.array([0, 511, 512, 0xFFFFFFFF, 0x80000000], np.uint32)
+    assert philox.rev(u).tolist() == [0.0, 0.0, 2.0**-23, 1.0 - 2.0**-23, 0.5]
+    # s scales both normals of a pair; negative s gives the same law (|s|)
+    a = np.array([123456789, 987654321], np.uint32)
+    b = np.array([555555555, 3000000000], np.uint32)
+    z0, z1 = philox.box_muller(a, b)
+    w0, w1 = philox.box_muller(a, b, s=-2.5)
+    assert np.allclose(w0, 2.5 * z0, rtol=1e-15) and np.allclose(w1, 2.5 * z1, rtol=1e-15)
