@@ -206,6 +206,13 @@ template <bool REPLAY, bool RECORD>
 constexpr int kSubBlocksR = RECORD ? 1 : kSubBlocks<REPLAY>;
 template <bool REPLAY, int MAXP, bool RECORD>
 constexpr int kAccStepsR = RECORD ? 1 : kAccSteps<REPLAY, MAXP>;
+// Native single-stage Gaussian diffusions share out the Philox blocks of their last walks
+// across the wave's idle lanes once at most kHelperWalks walks are left in it.
+template <int MODEL, bool REPLAY, bool RECORD>
+constexpr bool kHelperTail = !REPLAY && !RECORD &&
+                             (MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM ||
+                              MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_DDM_LEGACY);
+constexpr int kHelperWalks = 32;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -542,6 +549,90 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         // fx counts the steps as float32(ix): it rises by one per step taken and stays at the
         // index of the step whose test ended the walk
         float fx = (float)ix;
+        bool helped = false;
+        if constexpr (kHelperTail<MODEL, REPLAY, RECORD>) {
+          const uint64_t run = __ballot(alive);
+          const int K = __popcll(run);  // >= 1: some lane is walking
+          if (K <= kHelperWalks) {
+            // Few walks left in the wave (the end of the queue): every lane computes one
+            // Philox block of one walk, lane = rank + kp * j for block j of the walk of that
+            // rank (kp = K rounded up to a power of two), and each walking lane reads its
+            // normals from those lanes. A walk's 32 steps then wait for one to four Philox
+            // passes instead of eight. Same counters (sample, set, step / 4), same draws.
+            helped = true;
+            const int lane = lane_id();
+            const int r = __popcll(run & ((1ull << lane) - 1ull));  // walking lanes below this one
+            const int lgk = K <= 1 ? 0 : 32 - __clz(K - 1);
+            const int kp = 1 << lgk;
+            const int nb = kSubBlocks<false> < (64 >> lgk) ? kSubBlocks<false> : (64 >> lgk);  // blocks per pass
+            // rank -> lane: walking lanes send their lane id to lane r, the others fill K..63
+            const int rank_lane = __builtin_amdgcn_ds_permute((alive ? r : K + lane - r) << 2, lane);
+            const int ol = __shfl(rank_lane, lane & (kp - 1));  // the walk this lane draws for
+            const uint32_t on = (uint32_t)__shfl(n, ol), ok = (uint32_t)__shfl(k, ol);
+            const uint32_t oblk = (uint32_t)__shfl(ix, ol) >> 2;  // walks are block-aligned
+            const float obmc = __shfl(bmc, ol);
+            bool more = true;
+            for (int p0 = 0; p0 < kSubBlocks<false> && more; p0 += nb) {
+              float hz[4];
+              philox_normals4_c(on, ok, oblk + (uint32_t)(p0 + (lane >> lgk)), 0u, A.k0, A.k1, obmc, hz);
+              // this walk's block p0 + j sits in lane r + kp j; the next block is read while
+              // the current one steps
+              float cur[4], nxt[4];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) cur[i] = __shfl(hz[i], r);
+#pragma unroll
+              for (int j = 0; j < kSubBlocks<false>; ++j) {
+                if (j >= nb) break;
+                if (p0 + j > 0) {
+                  if (__ballot(alive) == 0) {
+                    more = false;
+                    break;
+                  }
+                  ++iters;
+                }
+                if (j + 1 < nb) {
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) nxt[i] = __shfl(hz[i], r + ((j + 1) << lgk));
+                }
+                if (alive) {
+                  // A wave with few walks left waits on each walk's step latency rather than
+                  // on issue slots. The block's four positions and times are the same float
+                  // sums whether or not a test fails, so they are formed first; the four
+                  // tests then run independently and the walk keeps the state at its first
+                  // failing test (fx + i is exact). Same steps, same rounding, a chain of four
+                  // adds and four selects per block instead of four test-and-branch rounds.
+                  const int ixb = (int)fx;  // this block's first step index
+                  float yv[5], tv[5];
+                  yv[0] = x[0];
+                  tv[0] = t;
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) {
+                    yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cur[i]);
+                    tv[i + 1] = tv[i] + dt;
+                  }
+                  bool ok[4];
+#pragma unroll
+                  for (int i = 0; i < 4; ++i)
+                    ok[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
+                  float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
+#pragma unroll
+                  for (int i = 3; i >= 0; --i) {
+                    yf = ok[i] ? yf : yv[i];
+                    tf = ok[i] ? tf : tv[i];
+                    ff = ok[i] ? ff : (i ? fx + (float)i : fx);
+                  }
+                  alive = ok[0] && ok[1] && ok[2] && ok[3];
+                  x[0] = yf;
+                  t = tf;
+                  fx = ff;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
+              }
+            }
+          }
+        }
+        if (!helped) {
 #pragma unroll
         for (int sb = 0; sb < kSubBlocks<REPLAY>; ++sb) {
           if (sb > 0) {
@@ -583,6 +674,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               }
             }
           }
+        }
         }
         if (state == kRunning) ix = (int)fx;
       } else {

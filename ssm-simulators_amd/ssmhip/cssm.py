@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import _device as D
 from . import _lib as L
 from .boundary_functions import constant, kernel_boundary_code, time_grid
 from .em import accumulator_records, boundary_table, ddm_records, make_job, two_stage_records, uniform_offsets
@@ -70,8 +71,23 @@ def _boundary_trial0(boundary_fun, boundary_params, max_t, delta_t) -> np.ndarra
     return np.asarray(boundary_fun(t=t_s, **kw)).astype(np.float32)
 
 
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream():
+    """A second stream of the current device (default priority) for the trajectory replay."""
+    t = D.torch()
+    dev = t.cuda.current_device()
+    if dev not in _SIDE_STREAMS:
+        _SIDE_STREAMS[dev] = t.cuda.Stream(device=dev)
+    return _SIDE_STREAMS[dev]
+
+
 def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, seed, smooth_unif, boundary_fun,
-         boundary_params, replay_normals=None, replay_uniforms=None):
+         boundary_params, replay_normals=None, replay_uniforms=None, trajectory=False):
+    """One launch over n_trials sets of n_samples. With trajectory=True the replay of trial 0,
+    sample 0 (metadata['trajectory']) runs on a side stream beside the launch; the fourth
+    value returned is then that float32[len(t_s), n_particles] array, else None."""
     table = None
     if code == L.BOUNDARY_TABLE:
         table = boundary_table(boundary_fun, boundary_params, n_trials, max_t, delta_t)
@@ -81,20 +97,28 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
         out_layout=1, choice_bytes=4, table=table,
         replay_normals=replay_normals, replay_uniforms=replay_uniforms,
     )
-    job.launch()
+    traj_dev = side = None
+    if trajectory and n_samples >= 1 and n_trials >= 1:
+        t = D.torch()
+        inputs_ready = t.cuda.Event()
+        inputs_ready.record()  # the job's inputs are enqueued on the current stream
+        side = _side_stream()
+        side.wait_event(inputs_ready)
+        job.launch()
+        traj_dev = job.trajectory_async(time_grid(max_t, delta_t).shape[0], stream=side)
+    else:
+        job.launch()
     total = n_samples * n_trials
     rts = job.rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
     choices = job.choices[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
-    return rts, choices, job
-
-
-def _trajectory(job, n_particles, max_t, delta_t, n_samples, n_trials) -> np.ndarray:
-    """metadata['trajectory']: float32[len(t_s), n_particles], the positions of trial 0, sample 0
-    after every step (ddm_models.pyx:714-728, race_models.pyx:433-452), -999 after its end."""
-    rows = time_grid(max_t, delta_t).shape[0]
-    if n_samples < 1 or n_trials < 1:
-        return np.full((rows, n_particles), -999.0, dtype=np.float32)
-    return job.trajectory(rows)
+    traj = None
+    if trajectory:
+        if traj_dev is not None:
+            side.synchronize()
+            traj = traj_dev.cpu().numpy()
+        else:  # no walk: every row is -999 (ddm_models.pyx:714-728, race_models.pyx:433-452)
+            traj = np.full((time_grid(max_t, delta_t).shape[0], n_particles), -999.0, dtype=np.float32)
+    return rts, choices, job, traj
 
 
 def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, n_trials, boundary_fun,
@@ -116,13 +140,12 @@ def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, 
     delta_t = float(np.float32(delta_t))
     rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra,
                       stride=L.SET_STRIDE_SINGLE)
-    rts, choices, job = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t,
-                             _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
-                             replay_uniforms)
+    rts, choices, job, traj = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t,
+                                   _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
+                                   replay_uniforms, trajectory=return_option == "full")
     minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
-    traj = _trajectory(job, 1, max_t, delta_t, int(n_samples), int(n_trials))
     full = _full_meta(
         minimal,
         params,
@@ -219,13 +242,12 @@ def racing_diffusion_model(v, b, A, t, s, deadline, delta_t=0.001, max_t=20, n_s
     delta_t = float(np.float32(delta_t))
     rec = accumulator_records(v, col(A), col(t), col(s), deadline, max_t, delta_t, L.BOUNDARY_CONSTANT,
                               {"a": col(b)})
-    rts, choices, job = _run(L.MODEL_RDM, L.BOUNDARY_CONSTANT, n_particles, rec, int(n_samples), n_trials, max_t,
-                             delta_t, _seed(random_state), smooth_unif, constant, {"a": col(b)}, replay_normals,
-                             replay_uniforms)
+    rts, choices, job, traj = _run(L.MODEL_RDM, L.BOUNDARY_CONSTANT, n_particles, rec, int(n_samples), n_trials,
+                                   max_t, delta_t, _seed(random_state), smooth_unif, constant, {"a": col(b)},
+                                   replay_normals, replay_uniforms, trajectory=return_option == "full")
     minimal = _minimal_meta("rdm_simulator", list(np.arange(0, n_particles, 1)), int(n_samples), n_trials)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
-    traj = _trajectory(job, n_particles, max_t, delta_t, int(n_samples), n_trials)
     full = _full_meta(minimal, {"v": v, "b": b, "A": A, "t": t, "deadline": deadline, "s": s},
                       {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads}, traj=traj,
                       extra={f"v{i}": v[:, i] for i in range(n_particles)})
@@ -324,9 +346,9 @@ def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n
     max_t = float(np.float32(max_t))
     delta_t = float(np.float32(delta_t))
     rec = accumulator_records(v, z, t, s, deadline, max_t, delta_t, code, boundary_params, g=g, b=b)
-    rts, choices, job = _run(model, code, n_particles, rec, int(n_samples), int(n_trials), max_t, delta_t,
-                             _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
-                             replay_uniforms)
+    rts, choices, job, traj = _run(model, code, n_particles, rec, int(n_samples), int(n_trials), max_t, delta_t,
+                                   _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
+                                   replay_uniforms, trajectory=return_option == "full")
     minimal = _minimal_meta(name, [-1, 1], int(n_samples), int(n_trials), boundary_fun.__name__)
     if return_option == "minimal":
         return {"rts": rts, "choices": choices, "metadata": minimal}
@@ -337,7 +359,6 @@ def _accumulator(name, model, v, z, t, s, deadline, delta_t, max_t, n_samples, n
     extra = {f"v{i}": v[:, i] for i in range(n_particles)}
     zz = np.asarray(z, np.float32)
     extra.update({f"z{i}": zz[:, i] for i in range(zz.shape[1])})
-    traj = _trajectory(job, n_particles, max_t, delta_t, int(n_samples), int(n_trials))
     full = _full_meta(
         minimal, params, {"delta_t": delta_t, "max_t": max_t, "n_threads": n_threads},
         boundary_fun=boundary_fun,

@@ -10,6 +10,7 @@ so the kernel repeats the same float32 operations per step.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 from dataclasses import dataclass
 
@@ -223,19 +224,29 @@ class EmJob:
         rc = L.load().ssm_em_simulate(C.byref(self.prob), C.byref(lc), *self.pointers(), D.stream_ptr(stream))
         L.check(rc, "ssm_em_simulate")
 
-    def trajectory(self, n_rows: int, stream=None) -> np.ndarray:
-        """Positions of walk (set 0, sample 0) after every step, the start in row 0:
-        float32[n_rows, n_particles], rows after the walk's end -999 (the reference's
+    def trajectory_async(self, n_rows: int, stream=None):
+        """Enqueue the replay of walk (set 0, sample 0) on `stream` (default: the current stream)
+        and return its device buffer float32[n_rows, n_particles]: the positions after every
+        step, the start in row 0, rows after the walk's end -999 (the reference's
         metadata['trajectory']). The walk is replayed on its own draws in one wave; the job's
-        outputs and step counters are not touched."""
+        outputs and step counters are not touched, so it may run beside the job's launch."""
         t = D.torch()
         p = int(self.prob.n_particles)
-        traj = t.full((int(n_rows), p), -999.0, dtype=t.float32, device="cuda")
-        ws = D.zeros((L.EM_WORKSPACE_WORDS,), t.int64)
-        rc = L.load().ssm_em_trajectory(C.byref(self.prob), D.ptr(self.records), D.ptr(self.offsets),
-                                        D.ptr(self.table), D.ptr(self.replay_normals), D.ptr(self.replay_uniforms),
-                                        D.ptr(traj), int(n_rows), D.ptr(ws), D.stream_ptr(stream))
+        with t.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            traj = t.full((int(n_rows), p), -999.0, dtype=t.float32, device="cuda")
+            ws = D.zeros((L.EM_WORKSPACE_WORDS,), t.int64)
+            rc = L.load().ssm_em_trajectory(C.byref(self.prob), D.ptr(self.records), D.ptr(self.offsets),
+                                            D.ptr(self.table), D.ptr(self.replay_normals),
+                                            D.ptr(self.replay_uniforms), D.ptr(traj), int(n_rows), D.ptr(ws),
+                                            D.stream_ptr(stream))
         L.check(rc, "ssm_em_trajectory")
+        return traj
+
+    def trajectory(self, n_rows: int, stream=None) -> np.ndarray:
+        """trajectory_async, waited for and copied to the host."""
+        traj = self.trajectory_async(n_rows, stream)
+        if stream is not None:
+            stream.synchronize()
         return traj.cpu().numpy()
 
     def steps(self) -> int:

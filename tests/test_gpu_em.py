@@ -305,6 +305,54 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
     assert np.all(outs[0][0] > 0)
 
 
+@pytest.mark.parametrize("model", ["ddm", "ornstein", "full_ddm"])
+def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
+    """Once at most 32 walks are left in a wave, the wave's idle lanes draw the Philox blocks
+    of the walking lanes. A set of 64 walks launched alone ends that way: its wave holds all 64
+    and runs its last 32 in that mode, down to one walk. The same set launched ahead of 100 sets
+    of slow walks, with waves that refill as soon as a lane is free, runs every walk of it in a
+    full wave. Both launches must give identical rts, choices and step counts."""
+    import torch
+
+    L, em = _ssm()
+    K = 101
+    rng = np.random.default_rng(23)
+    v = np.concatenate([[0.7], np.zeros(K - 1)]).astype(F32)  # set 0: the one under test
+    a = np.concatenate([[1.4], np.full(K - 1, 2.5)]).astype(F32)  # slow sets: E[T] = 6.25 s
+    z = np.concatenate([[0.45], np.full(K - 1, 0.5)]).astype(F32)
+    t = np.full(K, 0.1, F32)
+    dl, s = np.full(K, 999, F32), np.ones(K, F32)
+    if model == "ddm":
+        bp = {"a": a, "theta": np.concatenate([[0.3], np.zeros(K - 1)]).astype(F32)}
+        kind, bound = L.MODEL_DDM, L.BOUNDARY_ANGLE
+        rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, bp)
+    elif model == "ornstein":
+        kind, bound = L.MODEL_ORNSTEIN, L.BOUNDARY_CONSTANT
+        rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, {"a": a}, g=np.full(K, 0.4, F32))
+    else:
+        kind, bound = L.MODEL_FULL_DDM, L.BOUNDARY_CONSTANT
+        rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, {"a": a}, sz=np.full(K, 0.1, F32),
+                             sv=np.full(K, 0.5, F32), st=np.full(K, 0.05, F32))
+    print(f"[case] tail walks {model}", flush=True)
+    n0 = 64
+    alone = em.make_job(kind, bound, 1, rec[:1], np.array([0, n0], np.int64), delta_t=DT, max_t=20.0, seed=9,
+                        smooth_unif=True)
+    alone.launch(em.LaunchParams(256, 8, 8, 64))
+    torch.cuda.synchronize()
+    sizes = np.concatenate([[n0], np.full(K - 1, 1000)]).astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    full = em.make_job(kind, bound, 1, rec, offs, delta_t=DT, max_t=20.0, seed=9, smooth_unif=True)
+    full.launch(em.LaunchParams(64, 1, 1, 1))
+    torch.cuda.synchronize()
+    r_alone, c_alone = alone.rts[:n0].cpu().numpy(), alone.choices[:n0].cpu().numpy()
+    assert np.array_equal(r_alone, full.rts[:n0].cpu().numpy())
+    assert np.array_equal(c_alone, full.choices[:n0].cpu().numpy())
+    # the set's walks span a range of lengths, so the alone launch's wave shrank through every
+    # tail regime (32, 16, 8 and fewer walks)
+    steps = np.round((r_alone - 0.1) / DT)
+    assert np.ptp(steps) > 1000, (steps.min(), steps.max())
+
+
 def test_replay_is_independent_of_queue_order_on_ragged_sets(torch_gpu):
     """Replay rows are addressed by the trajectory's set-major index, whatever queue position
     it is drawn from: the oracle's trajectories come back in any visiting order of the sets."""
