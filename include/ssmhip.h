@@ -207,8 +207,10 @@ int32_t ssm_lba_simulate(int32_t nact, int32_t n_sets, int64_t n_total, int32_t 
 
 /* ---- RNG diagnostics --------------------------------------------------------
  * Philox4x32-10 (Salmon et al., SC'11) blocks for counters (c0 + i, c1, c2, c3),
- * key = seed, i < n: raw_dev uint32[n][4] (nullable) and the Box–Muller
- * normals the simulators use, normals_dev float32[n][4] (nullable).          */
+ * key = seed, i < n: raw_dev uint32[n][4] (nullable) and their N(0, 1)
+ * Box–Muller normals, normals_dev float32[n][4] (nullable). The simulators
+ * draw the same normals; native Gaussian-diffusion walks draw them with the
+ * radius scaled by sqrt(dt) s.                                               */
 int32_t ssm_philox_fill(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                         int64_t n, uint32_t* raw_dev, float* normals_dev, void* stream);
 

@@ -96,19 +96,38 @@ struct FitArgs {
   int32_t choice_bytes;
 };
 
-// One block per set, three passes, every load and store coalesced:
-//  1. count per choice (draws with that choice, and those not omitted) by block strides;
-//  2. compact log(rt) per choice in sample order: each 1024-sample tile places its kept
-//     samples by wave ballots plus a prefix over the tile's waves, behind a running
-//     cursor per choice (double-buffered counts, two barriers per tile);
-//  3. mean and population variance of each choice's contiguous run, summed in a fixed
-//     order (deterministic), then the Silverman bandwidth.
+// One block per set, two passes over the set's draws, every load and store coalesced:
+//  1. per choice: the draws with that choice, those not omitted, and the sum of their
+//     log(rt), which gives the mean;
+//  2. compact log(rt) per choice in sample order, 4096 draws per tile: slot j of the tile
+//     holds draws j*1024 + thread, so the tile's kept draws are placed by wave ballots and a
+//     prefix over (slot, wave), which one wave scans, behind a running cursor per choice
+//     (two barriers per tile; the counts alternate between two buffers).
+//     The same pass sums (x - mean)^2 per choice: the population variance of the two-pass
+//     algorithm without reading the compacted samples back.
+// Then the Silverman bandwidth. Sums are per thread, then over the block in a fixed order
+// (deterministic). MC >= n_choices bounds the per-choice registers.
+// draws per thread per tile (and per pass-1 iteration); kFitSlots x (kFitThreads / 64) = 64
+// counts per choice and tile, one per lane of the wave that scans them
+constexpr int kFitSlots = 4;
+static_assert(kFitSlots * (kFitThreads / 64) == 64, "one count per lane of the scanning wave");
+
+template <int MC>
+__device__ __forceinline__ int fit_choice_index(int ch, const int (&cv)[MC]) {
+  int k = -1;
+#pragma unroll
+  for (int c = MC - 1; c >= 0; --c)
+    if (ch == cv[c]) k = c;  // first match (unused slots hold a value no choice takes)
+  return k;
+}
+
+template <int MC>
 __global__ void __launch_bounds__(kFitThreads) kde_fit_kernel(FitArgs A) {
   constexpr int NW = kFitThreads / 64;
-  constexpr int MC = SSM_MAX_PARTICLES;
   __shared__ float redf[NW + 1];
   __shared__ int redi[NW + 1];
-  __shared__ int wave_cnt[2][NW][MC];
+  constexpr int S = kFitSlots;
+  __shared__ int wave_cnt[2][S][NW][MC];
   __shared__ int run[MC];
   const int set = blockIdx.x;
   const int64_t lo = A.offsets[set], hi = A.offsets[set + 1];
@@ -117,34 +136,50 @@ __global__ void __launch_bounds__(kFitThreads) kde_fit_kernel(FitArgs A) {
   int cv[MC];
 #pragma unroll
   for (int c = 0; c < MC; ++c) cv[c] = c < C ? A.choice_values[c] : 0x7fffffff;
-  auto choice_index = [&](int ch) {
-    int k = -1;
-#pragma unroll
-    for (int c = MC - 1; c >= 0; --c)
-      if (c < C && ch == cv[c]) k = c;  // first match
-    return k;
-  };
   // pass 1
   int kept[MC], all[MC];
+  float sx[MC];
 #pragma unroll
-  for (int c = 0; c < MC; ++c) kept[c] = all[c] = 0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kFitThreads) {
-    const int k = choice_index(load_choice(A.choices, A.choice_bytes, i));
-    const int keep = A.rts[i] != kOmission ? 1 : 0;
+  for (int c = 0; c < MC; ++c) {
+    kept[c] = all[c] = 0;
+    sx[c] = 0.f;
+  }
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += S * kFitThreads) {
+    float r[S];
+    int k[S];
 #pragma unroll
-    for (int c = 0; c < MC; ++c) {
-      all[c] += (k == c) ? 1 : 0;
-      kept[c] += (k == c) ? keep : 0;
+    for (int j = 0; j < S; ++j) {
+      const int64_t i = i0 + (int64_t)j * kFitThreads;
+      r[j] = kOmission;
+      k[j] = -1;
+      if (i < hi) {
+        r[j] = A.rts[i];
+        k[j] = fit_choice_index<MC>(load_choice(A.choices, A.choice_bytes, i), cv);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const bool keep = r[j] != kOmission;
+      const float x = keep ? logf(r[j]) : 0.f;
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        all[c] += (k[j] == c) ? 1 : 0;
+        kept[c] += (k[j] == c && keep) ? 1 : 0;
+        sx[c] += (k[j] == c) ? x : 0.f;
+      }
     }
   }
   int tot[MC], tot_all[MC], base[MC];
+  float mean[MC];
   int acc = 0;
 #pragma unroll
   for (int c = 0; c < MC; ++c) {
     tot[c] = tot_all[c] = 0;
+    mean[c] = 0.f;
     if (c < C) {
       block_scan_excl<kFitThreads>(kept[c], redi, &tot[c]);
       block_scan_excl<kFitThreads>(all[c], redi, &tot_all[c]);
+      mean[c] = block_sum<kFitThreads>(sx[c], redf) / (float)(tot[c] > 0 ? tot[c] : 1);
     }
     base[c] = acc;
     acc += tot[c];
@@ -152,72 +187,114 @@ __global__ void __launch_bounds__(kFitThreads) kde_fit_kernel(FitArgs A) {
   // pass 2
   if (threadIdx.x < MC) run[threadIdx.x] = 0;
   __syncthreads();
+  float q2[MC];
+#pragma unroll
+  for (int c = 0; c < MC; ++c) q2[c] = 0.f;
   const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
-  const int64_t ntiles = (hi - lo + kFitThreads - 1) / kFitThreads;
+  constexpr int kTile = S * kFitThreads;
+  const int64_t ntiles = (hi - lo + kTile - 1) / kTile;
+  // up to 4 choices, the next tile's draws are loaded while the current one is placed (with
+  // 16 the per-choice registers leave no room for them)
+  constexpr bool kAhead = MC <= 4;
+  float rn[S];
+  int cn[S];
+  auto load_tile = [&](int64_t tt) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int64_t i = lo + tt * kTile + (int64_t)j * kFitThreads + threadIdx.x;
+      rn[j] = i < hi ? A.rts[i] : kOmission;
+      cn[j] = i < hi ? load_choice(A.choices, A.choice_bytes, i) : 0;
+    }
+  };
+  if (kAhead) load_tile(0);
   for (int64_t t = 0; t < ntiles; ++t) {
     const int par = (int)(t & 1);
-    const int64_t i = lo + t * kFitThreads + threadIdx.x;
-    int k = -1;
-    float r = 0.f;
-    if (i < hi) {
-      r = A.rts[i];
-      if (r != kOmission) k = choice_index(load_choice(A.choices, A.choice_bytes, i));
-    }
-    int inwave = 0;
-    for (int c = 0; c < C; ++c) {
-      const uint64_t m = __ballot(k == c);
-      if (k == c) inwave = __popcll(m & below);
-      if (l == 0) wave_cnt[par][w][c] = __popcll(m);
-    }
-    __syncthreads();
-    if (k >= 0) {
-      int pos = run[k] + inwave;
-      for (int v = 0; v < w; ++v) pos += wave_cnt[par][v][k];
-      int b = 0;
+    if (!kAhead) load_tile(t);
+    float rc[S];
+    int cc[S];
 #pragma unroll
-      for (int c = 0; c < MC; ++c) b = (k == c) ? base[c] : b;
-      A.samples[lo + b + pos] = logf(r);
+    for (int j = 0; j < S; ++j) {
+      rc[j] = rn[j];
+      cc[j] = cn[j];
+    }
+    if (kAhead && t + 1 < ntiles) load_tile(t + 1);
+    float x[S];
+    int k[S], inwave[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      k[j] = -1;
+      x[j] = 0.f;
+      if (rc[j] != kOmission) {  // also the padding past the set's end
+        k[j] = fit_choice_index<MC>(cc[j], cv);
+        x[j] = logf(rc[j]);
+      }
+      inwave[j] = 0;
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        if (c < C) {
+          const uint64_t m = __ballot(k[j] == c);
+          if (k[j] == c) inwave[j] = __popcll(m & below);
+          if (l == 0) wave_cnt[par][j][w][c] = __popcll(m);
+        }
+      }
     }
     __syncthreads();
-    if (threadIdx.x < C) {
-      int sum = 0;
-      for (int v = 0; v < NW; ++v) sum += wave_cnt[par][v][threadIdx.x];
-      run[threadIdx.x] += sum;
+    // wave 0 turns each choice's 64 (slot, wave) counts into the output offsets of those
+    // groups: the running cursor plus an exclusive prefix in (slot, wave) order
+    if (w == 0) {
+      for (int c = 0; c < C; ++c) {
+        int* cnt = &wave_cnt[par][l / NW][l % NW][c];
+        const int v = *cnt;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o);
+          if (l >= o) incl += y;
+        }
+        const int r0 = run[c];
+        *cnt = r0 + incl - v;
+        if (l == 63) run[c] = r0 + incl;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      if (k[j] >= 0) {
+        const int pos = wave_cnt[par][j][w][k[j]] + inwave[j];
+        int b = 0;
+        float mu = 0.f;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+          b = (k[j] == c) ? base[c] : b;
+          mu = (k[j] == c) ? mean[c] : mu;
+        }
+        A.samples[lo + b + pos] = x[j];
+        const float d = x[j] - mu;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) q2[c] += (k[j] == c) ? d * d : 0.f;
+      }
     }
   }
-  __syncthreads();
-  // pass 3
-#pragma unroll 1
-  for (int c = 0; c < C; ++c) {
-    int tc = 0, tac = 0, bc = 0;
+  // bandwidths
 #pragma unroll
-    for (int q = 0; q < MC; ++q) {
-      if (q == c) { tc = tot[q]; tac = tot_all[q]; bc = base[q]; }
-    }
-    const float* sp = A.samples + lo + bc;
-    float s = 0.f;
-    for (int j = threadIdx.x; j < tc; j += kFitThreads) s += sp[j];
-    const float mean = block_sum<kFitThreads>(s, redf) / (float)(tc > 0 ? tc : 1);
-    float q2 = 0.f;
-    for (int j = threadIdx.x; j < tc; j += kFitThreads) {
-      const float d = sp[j] - mean;
-      q2 += d * d;
-    }
-    const float ss = block_sum<kFitThreads>(q2, redf);
-    if (threadIdx.x == 0) {
-      const int g = set * C + c;
-      float h = 0.f;
-      if (tc > 1) {
-        float sd = sqrtf(ss / (float)tc);  // np.std: population
-        if (sd < 1e-3f) sd = 1e-3f;
-        h = powf(4.0f / (3.0f * (float)tc), 0.2f) * sd;
-      } else if (tc == 1) {
-        h = powf(4.0f / 3.0f, 0.2f) * 10.0f;
+  for (int c = 0; c < MC; ++c) {
+    if (c < C) {
+      const float ss = block_sum<kFitThreads>(q2[c], redf);
+      if (threadIdx.x == 0) {
+        const int tc = tot[c];
+        const int g = set * C + c;
+        float h = 0.f;
+        if (tc > 1) {
+          float sd = sqrtf(ss / (float)tc);  // np.std: population
+          if (sd < 1e-3f) sd = 1e-3f;
+          h = powf(4.0f / (3.0f * (float)tc), 0.2f) * sd;
+        } else if (tc == 1) {
+          h = powf(4.0f / 3.0f, 0.2f) * 10.0f;
+        }
+        A.kde_count[g] = tc;
+        A.choice_total[g] = tot_all[c];
+        A.bandwidth[g] = h;
+        A.kde_start[g] = lo + base[c];
       }
-      A.kde_count[g] = tc;
-      A.choice_total[g] = tac;
-      A.bandwidth[g] = h;
-      A.kde_start[g] = lo + bc;
     }
   }
 }
@@ -570,7 +647,12 @@ int32_t kde_fit_enqueue(int32_t n_sets, int32_t n_choices, const int32_t* choice
     return fail(SSM_EINVAL, "NULL buffer");
   FitArgs A{rts, choices, offsets, choice_values, samples, kde_count, choice_total, bandwidth, kde_start,
             n_choices, choice_bytes};
-  hipLaunchKernelGGL(kde_fit_kernel, dim3(n_sets), dim3(kFitThreads), 0, stream, A);
+  if (n_choices <= 2)
+    hipLaunchKernelGGL(kde_fit_kernel<2>, dim3(n_sets), dim3(kFitThreads), 0, stream, A);
+  else if (n_choices <= 4)
+    hipLaunchKernelGGL(kde_fit_kernel<4>, dim3(n_sets), dim3(kFitThreads), 0, stream, A);
+  else
+    hipLaunchKernelGGL(kde_fit_kernel<SSM_MAX_PARTICLES>, dim3(n_sets), dim3(kFitThreads), 0, stream, A);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

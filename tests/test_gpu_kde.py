@@ -61,6 +61,43 @@ def test_fit_matches_oracle(torch_gpu):
                 assert abs(h[k, c] - fit.h[c]) <= 1e-4 * fit.h[c]
 
 
+@pytest.mark.parametrize("vals", [(-1, 1), (0, 1, 2), (0, 1, 2, 3, 4, 5)])
+def test_fit_compacts_each_choice_in_draw_order(torch_gpu, vals):
+    """Each (set, choice) group holds log(rt) of that choice's kept draws in draw order, the
+    data LogKDE keeps (kde_class.py:438-504), with the oracle's counts and bandwidths. Sets
+    are empty, single, across the kernel's 4096-draw tiles and ragged; one set lacks a choice.
+    2, 3 and 6 possible choices take the kernel's three per-choice register layouts."""
+    print(f"[case] kde fit order, choices {vals}", flush=True)
+    rng = np.random.default_rng(31 + len(vals))
+    sizes = [0, 1, 4095, 4097, 8192, 12289, 30001]
+    rts, chs = [], []
+    for n in sizes:
+        ch = rng.choice(np.array(vals), n).astype(np.int8)
+        rt = (0.2 + rng.lognormal(-0.4, 0.7, n)).astype(F32)
+        rt[rng.uniform(size=n) < 0.02] = -999.0
+        rts.append(rt)
+        chs.append(ch)
+    chs[4][chs[4] == vals[-1]] = vals[0]
+    bank, _ = _bank(rts, chs, vals)
+    samples = bank.samples.cpu().numpy()
+    start = bank.kde_start.cpu().numpy().reshape(len(sizes), len(vals))
+    cnt, tot, h = bank.host_state()
+    assert np.all(cnt[0] == 0) and np.all(tot[0] == 0) and np.all(h[0] == 0.0)
+    for k in range(1, len(sizes)):
+        fit = kde_ref.KDEFit(rts[k], chs[k], list(vals))
+        for c in range(len(vals)):
+            got = samples[start[k, c]:start[k, c] + cnt[k, c]]
+            ref = fit.logs[c].astype(F32)
+            assert got.shape == ref.shape
+            # device logf against NumPy's float32 log: a few ulp; any reordering is far larger
+            np.testing.assert_allclose(got, ref, rtol=5e-7, atol=5e-7)
+            assert tot[k, c] == int(round(fit.p[c] * fit.N))
+            if fit.h[c] is None:
+                assert h[k, c] == 0.0
+            else:
+                assert abs(h[k, c] - fit.h[c]) <= 1e-4 * fit.h[c]
+
+
 def test_eval_matches_oracle(torch_gpu):
     from ssmhip.kde import PointBatch
 
