@@ -37,6 +37,17 @@ def bandwidth_silverman(sample: Iterable[float] = (0, 0, 0), std_cutoff: float =
     return np.power((4 / (3 * n)), 1 / 5) * std
 
 
+_CHOICE_VALUES: dict = {}
+
+
+def _choice_values_dev(values: tuple):
+    """The possible choice values as a device int32 array, uploaded once per device."""
+    key = (D.torch().cuda.current_device(), values)
+    if key not in _CHOICE_VALUES:
+        _CHOICE_VALUES[key] = D.to_dev(np.asarray(values, np.int32))
+    return _CHOICE_VALUES[key]
+
+
 class KDEBank:
     """Per-(set, choice) KDEs for simulations resident on the device.
 
@@ -44,7 +55,9 @@ class KDEBank:
     with int64 ``offsets`` (device tensor, n_sets + 1 entries).
     """
 
-    def __init__(self, rts, choices, offsets, choice_values, *, stream=None):
+    def __init__(self, rts, choices, offsets, choice_values, *, offsets_host=None, stream=None):
+        """offsets_host: the same offsets as a host array, when the caller holds them (saves a
+        device-to-host read of them)."""
         t = D.require_gpu()
         self.n_sets = int(offsets.shape[0]) - 1
         self.choice_values = [int(c) for c in choice_values]
@@ -52,9 +65,11 @@ class KDEBank:
         if not 1 <= self.C <= L.MAX_PARTICLES:
             raise ValueError("between 1 and 16 possible choices are supported")
         self.offsets = offsets
-        offs_h = offsets.cpu().numpy()
+        offs_h = offsets.cpu().numpy() if offsets_host is None else np.asarray(offsets_host, np.int64)
+        if offs_h.shape != (self.n_sets + 1,):
+            raise ValueError("offsets_host must have n_sets + 1 entries")
         self.max_samples = int(np.max(np.diff(offs_h))) if offs_h.size > 1 else 0
-        self.d_choice_values = D.to_dev(np.asarray(self.choice_values, np.int32))
+        self.d_choice_values = _choice_values_dev(tuple(self.choice_values))
         total = int(rts.shape[0])
         G = self.n_sets * self.C
         self.samples = D.empty((max(total, 1),), t.float32)

@@ -103,7 +103,8 @@ class LanKDELabeller:
             if self.filters:
                 self._apply_filter()
         mark(1)
-        bank = KDEBank(self.sim.rts, self.sim.choices, self.sim.job.offsets, self.choice_values)
+        bank = KDEBank(self.sim.rts, self.sim.choices, self.sim.job.offsets, self.choice_values,
+                       offsets_host=self.sim.offsets)
         mark(2)
         pts = bank.sample(self.n_kde, self.n_unif, max_t=self.max_t, unif_lo=1e-4, seed=self.seed + 1)
         mark(3)
