@@ -665,6 +665,9 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               if (alive) {
                 const float b = boundary_at_f<BOUND>(fx, ixb + i, dt, a, b1, b2, row);
                 if (inside<MODEL>(x[0], b)) {
+                  // keeps the step a branch: speculated as selects, the first step of each
+                  // block costs three VALU instructions more than the exec-mask branch
+                  asm volatile("");
                   x[0] += walk_increment<MODEL, REPLAY>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
                   t += dt;
                   fx += 1.0f;
