@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
         if (alive) {
           const float b = a + (fx * dt) * slope;
           if (fabsf(y) <= b) {
+            asm volatile("");  // a branch, as in the simulator (not speculated into selects)
             y += vdt + nz[i];
             t += dt;
             fx += 1.0f;
