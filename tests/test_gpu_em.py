@@ -305,7 +305,7 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
     assert np.all(outs[0][0] > 0)
 
 
-@pytest.mark.parametrize("model", ["ddm", "ornstein", "full_ddm"])
+@pytest.mark.parametrize("model", ["ddm", "ddm_table", "ddm_legacy", "ornstein", "full_ddm"])
 def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     """Once at most 32 walks are left in a wave, the wave's idle lanes draw the Philox blocks
     of the walking lanes. A set of 64 walks launched alone ends that way: its wave holds all 64
@@ -322,10 +322,21 @@ def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     z = np.concatenate([[0.45], np.full(K - 1, 0.5)]).astype(F32)
     t = np.full(K, 0.1, F32)
     dl, s = np.full(K, 999, F32), np.ones(K, F32)
+    table = None
     if model == "ddm":
         bp = {"a": a, "theta": np.concatenate([[0.3], np.zeros(K - 1)]).astype(F32)}
         kind, bound = L.MODEL_DDM, L.BOUNDARY_ANGLE
         rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, bp)
+    elif model == "ddm_table":  # a collapsing Weibull bound read from the per-set table
+        from ssmhip import boundary_functions as BF
+
+        bp = {"a": a, "alpha": np.full(K, 2.0, F32), "beta": np.full(K, 4.0, F32)}
+        kind, bound = L.MODEL_DDM, L.BOUNDARY_TABLE
+        rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, bp)
+        table = em.boundary_table(BF.weibull_cdf, bp, K, 20.0, DT)
+    elif model == "ddm_legacy":  # bounds [0, a], start z a
+        kind, bound = L.MODEL_DDM_LEGACY, L.BOUNDARY_CONSTANT
+        rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, {"a": a})
     elif model == "ornstein":
         kind, bound = L.MODEL_ORNSTEIN, L.BOUNDARY_CONSTANT
         rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, {"a": a}, g=np.full(K, 0.4, F32))
@@ -336,12 +347,12 @@ def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     print(f"[case] tail walks {model}", flush=True)
     n0 = 64
     alone = em.make_job(kind, bound, 1, rec[:1], np.array([0, n0], np.int64), delta_t=DT, max_t=20.0, seed=9,
-                        smooth_unif=True)
+                        smooth_unif=True, table=None if table is None else table[:1])
     alone.launch(em.LaunchParams(256, 8, 8, 64))
     torch.cuda.synchronize()
     sizes = np.concatenate([[n0], np.full(K - 1, 1000)]).astype(np.int64)
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    full = em.make_job(kind, bound, 1, rec, offs, delta_t=DT, max_t=20.0, seed=9, smooth_unif=True)
+    full = em.make_job(kind, bound, 1, rec, offs, delta_t=DT, max_t=20.0, seed=9, smooth_unif=True, table=table)
     full.launch(em.LaunchParams(64, 1, 1, 1))
     torch.cuda.synchronize()
     r_alone, c_alone = alone.rts[:n0].cpu().numpy(), alone.choices[:n0].cpu().numpy()
