@@ -125,7 +125,8 @@ typedef struct ssm_launch_config {
  *    diffusions; 0 keeps SSM_SET_STRIDE, the layout of version 2).
  * 4: ssm_lan_assemble takes the negative-RT rows (n_neg, n_choices, neg_ll);
  *    ssm_copy_rows, ssm_em_trajectory, ssm_kde_eval_noalloc.
- * 5: ssm_mfma32_probe.                                                       */
+ * 5: ssm_mfma32_probe.
+ * 6: ssm_em_records, ssm_uniform_offsets.                                     */
 int32_t ssm_abi_version(void);
 const char* ssm_last_error(void);
 /* Number of CUs of the current device (0 and an error when no GPU).          */
@@ -294,6 +295,21 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_n
                          float neg_ll, int32_t shuffle, uint64_t seed, const float* theta_dev,
                          const float* pts_rt_dev, const int8_t* pts_choice_dev, const float* ll_dev,
                          const int32_t* choice_values_dev, float* rows_dev, void* stream);
+
+/* ssm_em_records replaces the per-trial setup of the simulators' sequential loops (the
+ * float32 quantities a trial forms once before its walk: ddm_models.pyx:696-717,
+ * race_models.pyx:378-414) for many sets at once, building the set records on the device.
+ * The host forms each used slot as one float32 column of n_sets values and uploads the
+ * columns only: record k of records_dev (set_stride floats) gets slot slots[c] =
+ * columns_dev[c * n_sets + k] for c < n_slots and 0 in every other slot. slots is a host
+ * array of n_slots distinct indices below set_stride (set_stride <= SSM_SET_STRIDE).      */
+int32_t ssm_em_records(int32_t n_sets, int32_t set_stride, int32_t n_slots, const int32_t* slots,
+                       const float* columns_dev, float* records_dev, void* stream);
+
+/* ssm_uniform_offsets writes offsets_dev[k] = k * n_samples for k <= n_sets: the segments
+ * of Simulator's n_trials sets of n_samples draws each in the set-major outputs (the
+ * reference indexes its [n_samples, n_trials] arrays directly, _utils.pyx:625-632).       */
+int32_t ssm_uniform_offsets(int32_t n_sets, int64_t n_samples, int64_t* offsets_dev, void* stream);
 
 /* ssm_copy_rows replaces the array assignments of the simulation filter's re-draw loop
  * (SimulationPipeline.generate_for_parameter_set, simulation_pipeline.py:103-115: a rejected

@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "ssmhip_internal.h"
 
 namespace ssmhip {
@@ -160,6 +162,64 @@ int32_t copy_rows_enqueue(int64_t n_rows, int64_t row_bytes, const void* src, co
   const int64_t grid = n_rows < 65536 ? n_rows : 65536;
   hipLaunchKernelGGL(copy_rows_kernel, dim3((unsigned)grid), dim3(256), 0, stream, n_rows, row_bytes,
                      (const uint8_t*)src, src_idx, (uint8_t*)dst, dst_idx, words);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+// ---- per-set records laid out on the device from per-slot columns ----------------------
+// record k (stride floats): slot j = columns[col_of[j] * n_sets + k] when col_of[j] >= 0, else 0;
+// one thread per record float, so the record writes are coalesced.
+struct SlotColumns {
+  int8_t col_of[SSM_SET_STRIDE];
+};
+__global__ void em_records_kernel(int64_t n_sets, int stride, SlotColumns map, const float* __restrict__ columns,
+                                  float* __restrict__ records) {
+  const int64_t n = n_sets * stride;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / stride;
+    const int j = (int)(e - k * stride);
+    const int c = map.col_of[j];
+    records[e] = c >= 0 ? columns[(int64_t)c * n_sets + k] : 0.f;
+  }
+}
+
+int32_t em_records_enqueue(int32_t n_sets, int32_t set_stride, int32_t n_slots, const int32_t* slots,
+                           const float* columns, float* records, hipStream_t stream) {
+  if (n_sets < 0) return fail(SSM_EINVAL, "n_sets must be >= 0");
+  if (set_stride < 1 || set_stride > SSM_SET_STRIDE)
+    return fail(SSM_EINVAL, "set_stride must be in [1, " + std::to_string(SSM_SET_STRIDE) + "]");
+  if (n_slots < 0 || n_slots > set_stride) return fail(SSM_EINVAL, "n_slots must be in [0, set_stride]");
+  if (n_sets == 0) return SSM_OK;
+  if (!records || (n_slots > 0 && (!slots || !columns))) return fail(SSM_EINVAL, "NULL buffer");
+  SlotColumns map;
+  for (int j = 0; j < SSM_SET_STRIDE; ++j) map.col_of[j] = -1;
+  for (int c = 0; c < n_slots; ++c) {
+    const int j = slots[c];
+    if (j < 0 || j >= set_stride) return fail(SSM_EINVAL, "slot index outside the record");
+    if (map.col_of[j] >= 0) return fail(SSM_EINVAL, "slot index given twice");
+    map.col_of[j] = (int8_t)c;
+  }
+  const int64_t n = (int64_t)n_sets * set_stride;
+  const int64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+  hipLaunchKernelGGL(em_records_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (int64_t)n_sets, set_stride,
+                     map, columns, records);
+  SSM_HIP(hipGetLastError());
+  return SSM_OK;
+}
+
+// offsets[k] = k * n_samples, k <= n_sets
+__global__ void uniform_offsets_kernel(int64_t n_sets, int64_t n_samples, int64_t* __restrict__ offsets) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n_sets; k += (int64_t)gridDim.x * blockDim.x)
+    offsets[k] = k * n_samples;
+}
+
+int32_t uniform_offsets_enqueue(int32_t n_sets, int64_t n_samples, int64_t* offsets, hipStream_t stream) {
+  if (n_sets < 0 || n_samples < 0) return fail(SSM_EINVAL, "n_sets and n_samples must be >= 0");
+  if (!offsets) return fail(SSM_EINVAL, "NULL buffer");
+  const int64_t n = (int64_t)n_sets + 1;
+  const int64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+  hipLaunchKernelGGL(uniform_offsets_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (int64_t)n_sets, n_samples,
+                     offsets);
   SSM_HIP(hipGetLastError());
   return SSM_OK;
 }

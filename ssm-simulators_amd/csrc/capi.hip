@@ -89,6 +89,9 @@ int32_t lan_assemble_enqueue(int32_t n_sets, int32_t n_theta, int32_t m, int32_t
                              hipStream_t stream);
 int32_t copy_rows_enqueue(int64_t n_rows, int64_t row_bytes, const void* src, const int64_t* src_idx, void* dst,
                           const int64_t* dst_idx, hipStream_t stream);
+int32_t em_records_enqueue(int32_t n_sets, int32_t set_stride, int32_t n_slots, const int32_t* slots,
+                           const float* columns, float* records, hipStream_t stream);
+int32_t uniform_offsets_enqueue(int32_t n_sets, int64_t n_samples, int64_t* offsets, hipStream_t stream);
 
 int32_t mlp_sizes(int32_t* n_params, int32_t* n_frag, int32_t* n_bias, int32_t* act_rows, int32_t* delta_rows);
 int32_t mlp_pack_enqueue(const float* params, __bf16* frags, float* bias, hipStream_t s);
@@ -119,7 +122,7 @@ using namespace ssmhip;
 
 extern "C" {
 
-int32_t ssm_abi_version(void) { return 5; }
+int32_t ssm_abi_version(void) { return 6; }
 
 const char* ssm_last_error(void) { return g_last_error.c_str(); }
 
@@ -260,6 +263,15 @@ int32_t ssm_lan_assemble(int32_t n_sets, int32_t n_theta, int32_t m, int32_t n_n
                          const int32_t* choice_values_dev, float* rows_dev, void* stream) {
   return lan_assemble_enqueue(n_sets, n_theta, m, n_neg, n_choices, neg_ll, shuffle, seed, theta_dev, pts_rt_dev,
                               pts_choice_dev, ll_dev, choice_values_dev, rows_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_em_records(int32_t n_sets, int32_t set_stride, int32_t n_slots, const int32_t* slots,
+                       const float* columns_dev, float* records_dev, void* stream) {
+  return em_records_enqueue(n_sets, set_stride, n_slots, slots, columns_dev, records_dev, (hipStream_t)stream);
+}
+
+int32_t ssm_uniform_offsets(int32_t n_sets, int64_t n_samples, int64_t* offsets_dev, void* stream) {
+  return uniform_offsets_enqueue(n_sets, n_samples, offsets_dev, (hipStream_t)stream);
 }
 
 int32_t ssm_copy_rows(int64_t n_rows, int64_t row_bytes, const void* src_dev, const int64_t* src_idx_dev,

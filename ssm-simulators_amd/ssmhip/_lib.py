@@ -110,6 +110,8 @@ _SIGS = {
                     _VP, _VP, _VP, _VP, _VP, _VP]
     ),
     "ssm_copy_rows": (C.c_int32, [C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP]),
+    "ssm_em_records": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _VP, _VP, _VP, _VP]),
+    "ssm_uniform_offsets": (C.c_int32, [C.c_int32, C.c_int64, _VP, _VP]),
     "ssm_mlp_sizes": (C.c_int32, [C.POINTER(C.c_int32)] * 5),
     "ssm_mlp_pack": (C.c_int32, [_VP, _VP, _VP, _VP]),
     "ssm_mlp_forward": (C.c_int32, [_VP, C.c_int64, _VP, _VP, _VP, _VP, _VP]),

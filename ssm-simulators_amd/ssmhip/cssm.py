@@ -17,7 +17,7 @@ import numpy as np
 from . import _device as D
 from . import _lib as L
 from .boundary_functions import constant, kernel_boundary_code, time_grid
-from .em import accumulator_records, boundary_table, ddm_records, make_job, two_stage_records, uniform_offsets
+from .em import accumulator_records, boundary_table, ddm_records_device, make_job, two_stage_records, uniform_offsets
 
 MAX_THREADS = 256  # _constants.pxi:10
 OMISSION_SENTINEL = -999.0
@@ -92,7 +92,7 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
     if code == L.BOUNDARY_TABLE:
         table = boundary_table(boundary_fun, boundary_params, n_trials, max_t, delta_t)
     job = make_job(
-        model, code, n_particles, rec, uniform_offsets(n_trials, n_samples),
+        model, code, n_particles, rec, int(n_samples),
         delta_t=delta_t, max_t=max_t, seed=seed, smooth_unif=smooth_unif,
         out_layout=1, choice_bytes=4, table=table,
         replay_normals=replay_normals, replay_uniforms=replay_uniforms,
@@ -138,8 +138,8 @@ def _diffusion(name, model, v, a, z, t, deadline, s, max_t, delta_t, n_samples, 
     code = kernel_boundary_code(boundary_fun)
     max_t = float(np.float32(max_t))
     delta_t = float(np.float32(delta_t))
-    rec = ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra,
-                      stride=L.SET_STRIDE_SINGLE)
+    rec = ddm_records_device(v, a, z, t, deadline, s, max_t, delta_t, code, boundary_params, **extra,
+                             stride=L.SET_STRIDE_SINGLE)
     rts, choices, job, traj = _run(model, code, 1, rec, int(n_samples), int(n_trials), max_t, delta_t,
                                    _seed(random_state), smooth_unif, boundary_fun, boundary_params, replay_normals,
                                    replay_uniforms, trajectory=return_option == "full")

@@ -56,18 +56,30 @@ def new_records(n_sets: int) -> np.ndarray:
     return np.zeros((n_sets, L.SET_STRIDE), dtype=F32)
 
 
-def fill_boundary(rec: np.ndarray, code: int, params: dict) -> None:
-    """Write boundary parameters of the in-register boundary kinds into the records."""
-    n = rec.shape[0]
+def first_column(val, n: int) -> np.ndarray:
+    """float32[n]: a per-set value given as [n] or [n, 1] (the first column), or a scalar for
+    n == 1; empty when n == 0."""
+    arr = np.asarray(val, F32)
+    return arr.reshape(n, -1)[:, 0] if n else np.zeros(0, F32)
+
+
+def boundary_columns(code: int, params: dict, n: int) -> dict:
+    """{slot: float32 column} of the in-register boundary kinds' parameters."""
 
     def col(name):
-        return np.asarray(params[name], F32).reshape(n, -1)[:, 0]
+        return first_column(params[name], n)
 
     if code == L.BOUNDARY_ANGLE:
-        rec[:, L.P_B1] = angle_slope32(col("theta"))
-    elif code == L.BOUNDARY_WEIBULL:
-        rec[:, L.P_B1] = col("alpha")
-        rec[:, L.P_B2] = col("beta")
+        return {L.P_B1: angle_slope32(col("theta"))}
+    if code == L.BOUNDARY_WEIBULL:
+        return {L.P_B1: col("alpha"), L.P_B2: col("beta")}
+    return {}
+
+
+def fill_boundary(rec: np.ndarray, code: int, params: dict) -> None:
+    """Write boundary parameters of the in-register boundary kinds into the records."""
+    for slot, col in boundary_columns(code, params, rec.shape[0]).items():
+        rec[:, slot] = col
 
 
 def levy_scale32(s, alpha, delta_t) -> np.ndarray:
@@ -76,6 +88,33 @@ def levy_scale32(s, alpha, delta_t) -> np.ndarray:
     s64 = np.asarray(s, F32).astype(np.float64)
     a64 = np.asarray(alpha, F32).astype(np.float64)
     return (s64 * np.power(np.float64(F32(delta_t)), 1.0 / a64)).astype(F32)
+
+
+def ddm_columns(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None, sv=None, st=None, g=None,
+                alpha=None) -> tuple[int, dict]:
+    """(n, {slot: float32 column}) of the single-particle diffusions' records: every slot they
+    use, each formed once per set as the reference forms it per trial (see ddm_records)."""
+    v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
+    n = v.shape[0]
+    cols = {}
+    for slot, val in ((L.P_SZ, sz), (L.P_SV, sv), (L.P_ST, st), (L.P_G, g)):
+        if val is not None:
+            cols[slot] = first_column(val, n)
+    cols[L.P_V] = v
+    cols[L.P_Z] = z
+    cols[L.P_T0] = t
+    cols[L.P_DEADLINE] = deadline
+    cols[L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
+    cols[L.P_SQRT_ST] = (sqrt_dt32(delta_t) * s).astype(F32)
+    if alpha is not None:
+        al = first_column(alpha, n)
+        if np.any(~(al > 0)) or np.any(al > 2):
+            raise ValueError("alpha must be in (0, 2]")
+        cols[L.P_ALPHA] = al
+        cols[L.P_SQRT_ST] = levy_scale32(s, al, delta_t)
+    cols[L.P_A] = first_column(bparams.get("a", a), n)
+    cols.update(boundary_columns(code, bparams, n))
+    return n, cols
 
 
 def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None, sv=None, st=None,
@@ -87,35 +126,45 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     models accept (a quarter of the bytes to build and upload)."""
     if stride not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
         raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
-    v, a, z, t, deadline, s = (np.asarray(x, F32).reshape(-1) for x in (v, a, z, t, deadline, s))
-    n = v.shape[0]
+    n, cols = ddm_columns(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=sz, sv=sv, st=st, g=g,
+                          alpha=alpha)
     # the single-particle slots all lie in the first 16 floats: each slot is written as one
     # contiguous row of a slot-major (16, n) array (rec is its transposed view), then the
     # records are laid out by one transposing copy (and, for 64-float records, one more)
     rec = np.zeros((16, n), F32).T
-    for slot, val in ((L.P_SZ, sz), (L.P_SV, sv), (L.P_ST, st), (L.P_G, g)):
-        if val is not None:
-            rec[:, slot] = np.asarray(val, F32).reshape(n, -1)[:, 0]
-    rec[:, L.P_V] = v
-    rec[:, L.P_Z] = z
-    rec[:, L.P_T0] = t
-    rec[:, L.P_DEADLINE] = deadline
-    rec[:, L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
-    rec[:, L.P_SQRT_ST] = (sqrt_dt32(delta_t) * s).astype(F32)
-    if alpha is not None:
-        al = np.asarray(alpha, F32).reshape(n, -1)[:, 0]
-        if np.any(~(al > 0)) or np.any(al > 2):
-            raise ValueError("alpha must be in (0, 2]")
-        rec[:, L.P_ALPHA] = al
-        rec[:, L.P_SQRT_ST] = levy_scale32(s, al, delta_t)
-    a_b = np.asarray(bparams.get("a", a), F32).reshape(n, -1)[:, 0]
-    rec[:, L.P_A] = a_b
-    fill_boundary(rec, code, bparams)
+    for slot, col in cols.items():
+        rec[:, slot] = col
     if stride == L.SET_STRIDE_SINGLE:
         return np.ascontiguousarray(rec)
     full = new_records(n)
     full[:, :16] = rec
     return full
+
+
+def records_on_device(n: int, cols: dict, stride: int):
+    """The set records as a device tensor float32[n, stride], laid out on the device from the
+    columns (ssm_em_records): only the used slots cross to the device, a column at a time,
+    and no host transpose is made. Bit-identical to the host layout of the same columns."""
+    t = D.require_gpu()
+    slots = sorted(cols)
+    out = D.empty((max(n, 1), stride), t.float32)
+    if n == 0:
+        return out[:0]
+    dev_cols = D.empty((len(slots), n), t.float32)
+    for j, slot in enumerate(slots):
+        dev_cols[j].copy_(t.from_numpy(np.ascontiguousarray(cols[slot], F32)))
+    arr = (C.c_int32 * len(slots))(*slots)
+    rc = L.load().ssm_em_records(n, stride, len(slots), arr, D.ptr(dev_cols), D.ptr(out), D.stream_ptr())
+    L.check(rc, "ssm_em_records")
+    return out
+
+
+def ddm_records_device(*args, stride: int = L.SET_STRIDE_SINGLE, **kwargs):
+    """ddm_records laid out on the device (records_on_device); same arguments."""
+    if stride not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
+        raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
+    n, cols = ddm_columns(*args, **kwargs)
+    return records_on_device(n, cols, stride)
 
 
 def two_stage_records(vh, vl1, vl2, zh, zl1, zl2, t, deadline, s, max_t, delta_t, code, bparams) -> np.ndarray:
@@ -287,8 +336,8 @@ def make_job(
     model: int,
     boundary: int,
     n_particles: int,
-    records: np.ndarray,
-    offsets: np.ndarray,
+    records,
+    offsets,
     *,
     delta_t: float,
     max_t: float,
@@ -300,17 +349,30 @@ def make_job(
     replay_normals: np.ndarray | None = None,
     replay_uniforms: np.ndarray | None = None,
 ) -> EmJob:
+    """One persistent launch's inputs and outputs on the device. records: float32[n_sets, stride]
+    on the host, or already on the device (records_on_device). offsets: int64[n_sets + 1] on
+    the host, or an int n: every set has n samples, and the offsets are written on the device."""
     t = D.require_gpu()
-    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    on_device = not isinstance(records, np.ndarray) and bool(getattr(records, "is_cuda", False))
+    if on_device and (records.dtype != t.float32 or not records.is_contiguous()):
+        raise ValueError("device records must be a contiguous float32 tensor")
     if records.ndim != 2 or records.shape[1] not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
         raise ValueError(f"records must have {L.SET_STRIDE} (or, single-particle diffusions, "
                          f"{L.SET_STRIDE_SINGLE}) floats per set")
-    n_sets = records.shape[0]
-    if offsets.shape != (n_sets + 1,):
-        raise ValueError("offsets must have n_sets + 1 entries")
-    if np.any(np.diff(offsets) < 0) or offsets[0] != 0:
-        raise ValueError("offsets must start at 0 and be non-decreasing")
-    n_total = int(offsets[-1])
+    n_sets = int(records.shape[0])
+    uniform = isinstance(offsets, (int, np.integer))
+    if uniform:
+        per_set = int(offsets)
+        if per_set < 0:
+            raise ValueError("n_samples per set must be >= 0")
+        n_total = n_sets * per_set
+    else:
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        if offsets.shape != (n_sets + 1,):
+            raise ValueError("offsets must have n_sets + 1 entries")
+        if np.any(np.diff(offsets) < 0) or offsets[0] != 0:
+            raise ValueError("offsets must start at 0 and be non-decreasing")
+        n_total = int(offsets[-1])
     from .boundary_functions import n_steps as _n_steps
 
     nst = _n_steps(max_t, delta_t)
@@ -354,8 +416,8 @@ def make_job(
     cdt = t.int8 if choice_bytes == 1 else t.int32
     return EmJob(
         prob=prob,
-        records=D.to_dev(records, F32),
-        offsets=D.to_dev(offsets, np.int64),
+        records=records if on_device else D.to_dev(records, F32),
+        offsets=uniform_offsets_device(n_sets, per_set) if uniform else D.to_dev(offsets, np.int64),
         table=D.to_dev(table) if table is not None else None,
         replay_normals=D.to_dev(replay_normals) if replay_normals is not None else None,
         replay_uniforms=D.to_dev(np.ascontiguousarray(replay_uniforms, F32)) if replay_uniforms is not None else None,
@@ -364,6 +426,15 @@ def make_job(
         choices=D.empty((max(n_total, 1),), cdt),
         workspace=D.zeros((L.EM_WORKSPACE_WORDS,), t.int64),
     )
+
+
+def uniform_offsets_device(n_sets: int, n_samples: int):
+    """int64[n_sets + 1] device tensor k * n_samples, written on the device (ssm_uniform_offsets)."""
+    t = D.require_gpu()
+    out = D.empty((n_sets + 1,), t.int64)
+    L.check(L.load().ssm_uniform_offsets(int(n_sets), int(n_samples), D.ptr(out), D.stream_ptr()),
+            "ssm_uniform_offsets")
+    return out
 
 
 def uniform_offsets(n_sets: int, n_samples: int) -> np.ndarray:

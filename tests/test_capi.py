@@ -36,7 +36,7 @@ def test_abi_version_and_error_string_without_gpu():
     from ssmhip import _lib
 
     lib = _lib.load()
-    assert lib.ssm_abi_version() == 5
+    assert lib.ssm_abi_version() == 6
     assert isinstance(lib.ssm_last_error(), bytes)
 
 
@@ -66,3 +66,22 @@ def test_kde_eval_noalloc_rejects_sizes_beyond_its_scratch_before_touching_the_g
     assert "no-allocation scratch" in _lib.last_error()
     assert lib.ssm_kde_eval_noalloc(1, 2, *([None] * 9), C.c_float(20.0), C.c_float(-66.774), C.c_float(1e-4),
                                     100_000, -1, None, None) == _lib.SSM_EINVAL
+
+
+def test_records_and_offsets_reject_bad_arguments_before_touching_the_gpu():
+    from ssmhip import _lib
+
+    lib = _lib.load()
+    slots = (C.c_int32 * 2)(3, 3)
+    assert lib.ssm_em_records(4, 16, 2, slots, None, None, None) == _lib.SSM_EINVAL
+    slots = (C.c_int32 * 2)(3, 3)
+    x = C.c_float(0.0)
+    assert lib.ssm_em_records(4, 16, 2, slots, C.byref(x), C.byref(x), None) == _lib.SSM_EINVAL
+    assert "twice" in _lib.last_error()
+    slots = (C.c_int32 * 1)(16,)
+    assert lib.ssm_em_records(4, 16, 1, slots, C.byref(x), C.byref(x), None) == _lib.SSM_EINVAL
+    assert "outside the record" in _lib.last_error()
+    assert lib.ssm_em_records(4, 65, 0, None, None, C.byref(x), None) == _lib.SSM_EINVAL
+    assert lib.ssm_em_records(0, 16, 0, None, None, None, None) == _lib.SSM_OK
+    assert lib.ssm_uniform_offsets(-1, 5, None, None) == _lib.SSM_EINVAL
+    assert lib.ssm_uniform_offsets(3, 5, None, None) == _lib.SSM_EINVAL
