@@ -56,8 +56,9 @@ struct EmArgs {
   int32_t traj_rows;
   float dt;
   // t_fast[r] (r = 1 in replay mode): an upper bound on how far t can rise over the steps
-  // of one walk-loop iteration; NaN when n_steps > 2^24 (then no iteration runs fast)
-  float t_fast[2];
+  // of one walk-loop iteration; t_fast[2] the same over a long tail iteration (native mode);
+  // NaN when n_steps > 2^24 (then no iteration runs fast)
+  float t_fast[3];
   uint32_t k0, k1;
 };
 
@@ -213,6 +214,11 @@ constexpr bool kHelperTail = !REPLAY && !RECORD &&
                              (MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM ||
                               MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_DDM_LEGACY);
 constexpr int kHelperWalks = 32;
+// With at most kLoneWalks walks left, one Philox pass of the whole wave serves one tail
+// iteration of 64 / kp blocks (kp = the walks rounded up to a power of two): at most
+// kLoneSteps steps per walk per iteration.
+constexpr int kLoneWalks = 4;
+constexpr int kLoneSteps = 256;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -558,12 +564,17 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             // Philox block of one walk, lane = rank + kp * j for block j of the walk of that
             // rank (kp = K rounded up to a power of two), and each walking lane reads its
             // normals from those lanes. A walk's 32 steps then wait for one to four Philox
-            // passes instead of eight. Same counters (sample, set, step / 4), same draws.
+            // passes instead of eight, and up to four walks take 256, 128 or 64 steps from one
+            // pass. Same counters (sample, set, step / 4), same draws.
             helped = true;
             const int lane = lane_id();
             const int r = __popcll(run & ((1ull << lane) - 1ull));  // walking lanes below this one
             const int lgk = K <= 1 ? 0 : 32 - __clz(K - 1);
             const int kp = 1 << lgk;
+            // at most kLoneWalks walks left, far enough from the end of t_s and every deadline:
+            // one pass of 64 / kp blocks; otherwise 8 blocks in passes of nb
+            const bool lone = K <= kLoneWalks &&
+                              __ballot(alive && !(ix + kLoneSteps <= A.n_steps && t + A.t_fast[2] <= dl_tmp)) == 0;
             const int nb = kSubBlocks<false> < (64 >> lgk) ? kSubBlocks<false> : (64 >> lgk);  // blocks per pass
             // rank -> lane: walking lanes send their lane id to lane r, the others fill K..63
             const int rank_lane = __builtin_amdgcn_ds_permute((alive ? r : K + lane - r) << 2, lane);
@@ -571,63 +582,84 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             const uint32_t on = (uint32_t)__shfl(n, ol), ok = (uint32_t)__shfl(k, ol);
             const uint32_t oblk = (uint32_t)__shfl(ix, ol) >> 2;  // walks are block-aligned
             const float obmc = __shfl(bmc, ol);
-            bool more = true;
-            for (int p0 = 0; p0 < kSubBlocks<false> && more; p0 += nb) {
-              float hz[4];
-              philox_normals4_c(on, ok, oblk + (uint32_t)(p0 + (lane >> lgk)), 0u, A.k0, A.k1, obmc, hz);
-              // this walk's block p0 + j sits in lane r + kp j; the next block is read while
-              // the current one steps
-              float cur[4], nxt[4];
+            float hz[4];
+            // A wave with few walks left waits on each walk's step latency rather than on issue
+            // slots. A block's four positions and times are the same float sums whether or not
+            // a test fails, so they are formed first; the four tests then run independently and
+            // the walk keeps the state at its first failing test (fx + i is exact). Same steps,
+            // same rounding, a chain of four adds and four selects per block instead of four
+            // test-and-branch rounds.
+            auto step_block = [&](const float (&cz)[4]) {
+              const int ixb = (int)fx;  // this block's first step index
+              float yv[5], tv[5];
+              yv[0] = x[0];
+              tv[0] = t;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cz[i]);
+                tv[i + 1] = tv[i] + dt;
+              }
+              bool okb[4];
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                okb[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
+              float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
+#pragma unroll
+              for (int i = 3; i >= 0; --i) {
+                yf = okb[i] ? yf : yv[i];
+                tf = okb[i] ? tf : tv[i];
+                ff = okb[i] ? ff : (i ? fx + (float)i : fx);
+              }
+              alive = okb[0] && okb[1] && okb[2] && okb[3];
+              x[0] = yf;
+              t = tf;
+              fx = ff;
+            };
+            // this walk's block p0 + j sits in lane r + kp j; the next block is read while the
+            // current one steps
+            float cur[4], nxt[4];
+            if (lone) {
+              philox_normals4_c(on, ok, oblk + (uint32_t)(lane >> lgk), 0u, A.k0, A.k1, obmc, hz);
 #pragma unroll
               for (int i = 0; i < 4; ++i) cur[i] = __shfl(hz[i], r);
-#pragma unroll
-              for (int j = 0; j < kSubBlocks<false>; ++j) {
-                if (j >= nb) break;
-                if (p0 + j > 0) {
-                  if (__ballot(alive) == 0) {
-                    more = false;
-                    break;
-                  }
+              const int total = 64 >> lgk;
+              for (int j = 0; j < total; ++j) {
+                if (j > 0) {
+                  if (__ballot(alive) == 0) break;
                   ++iters;
                 }
-                if (j + 1 < nb) {
+                if (j + 1 < total) {
 #pragma unroll
                   for (int i = 0; i < 4; ++i) nxt[i] = __shfl(hz[i], r + ((j + 1) << lgk));
                 }
-                if (alive) {
-                  // A wave with few walks left waits on each walk's step latency rather than
-                  // on issue slots. The block's four positions and times are the same float
-                  // sums whether or not a test fails, so they are formed first; the four
-                  // tests then run independently and the walk keeps the state at its first
-                  // failing test (fx + i is exact). Same steps, same rounding, a chain of four
-                  // adds and four selects per block instead of four test-and-branch rounds.
-                  const int ixb = (int)fx;  // this block's first step index
-                  float yv[5], tv[5];
-                  yv[0] = x[0];
-                  tv[0] = t;
-#pragma unroll
-                  for (int i = 0; i < 4; ++i) {
-                    yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cur[i]);
-                    tv[i + 1] = tv[i] + dt;
-                  }
-                  bool ok[4];
-#pragma unroll
-                  for (int i = 0; i < 4; ++i)
-                    ok[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
-                  float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
-#pragma unroll
-                  for (int i = 3; i >= 0; --i) {
-                    yf = ok[i] ? yf : yv[i];
-                    tf = ok[i] ? tf : tv[i];
-                    ff = ok[i] ? ff : (i ? fx + (float)i : fx);
-                  }
-                  alive = ok[0] && ok[1] && ok[2] && ok[3];
-                  x[0] = yf;
-                  t = tf;
-                  fx = ff;
-                }
+                if (alive) step_block(cur);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
+              }
+            } else {
+              bool more = true;
+              for (int p0 = 0; p0 < kSubBlocks<false> && more; p0 += nb) {
+                philox_normals4_c(on, ok, oblk + (uint32_t)(p0 + (lane >> lgk)), 0u, A.k0, A.k1, obmc, hz);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cur[i] = __shfl(hz[i], r);
+#pragma unroll
+                for (int j = 0; j < kSubBlocks<false>; ++j) {
+                  if (j >= nb) break;
+                  if (p0 + j > 0) {
+                    if (__ballot(alive) == 0) {
+                      more = false;
+                      break;
+                    }
+                    ++iters;
+                  }
+                  if (j + 1 < nb) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) nxt[i] = __shfl(hz[i], r + ((j + 1) << lgk));
+                  }
+                  if (alive) step_block(cur);
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
+                }
               }
             }
           }
@@ -992,16 +1024,16 @@ static float t_after_steps(int32_t steps, float dt) {
 // takes dt + u per step plus 2u for the rounding of t + bound in the kernel's test, and is
 // rounded up to float.
 static void fill_t_fast(const ssm_em_problem* P, EmArgs& A) {
-  const int steps[2] = {4 * kSubBlocks<false>, 4 * kSubBlocks<true>};
+  const int steps[3] = {4 * kSubBlocks<false>, 4 * kSubBlocks<true>, kLoneSteps};
   if (P->n_steps > (1 << 24)) {
-    A.t_fast[0] = A.t_fast[1] = std::numeric_limits<float>::quiet_NaN();
+    A.t_fast[0] = A.t_fast[1] = A.t_fast[2] = std::numeric_limits<float>::quiet_NaN();
     return;
   }
   const double top = 2.0 * std::max((double)t_after_steps(P->n_steps - 1, P->dt), (double)P->n_steps * P->dt);
   int e = 0;
   std::frexp(top, &e);
   const double u = std::ldexp(1.0, e - 24);
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < 3; ++r) {
     const double b = (steps[r] - 1) * ((double)P->dt + u) + 2.0 * u;
     A.t_fast[r] = std::nextafter((float)b, std::numeric_limits<float>::infinity());
   }

@@ -305,7 +305,7 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
     assert np.all(outs[0][0] > 0)
 
 
-@pytest.mark.parametrize("model", ["ddm", "ddm_table", "ddm_legacy", "ornstein", "full_ddm"])
+@pytest.mark.parametrize("model", ["ddm", "ddm_deadline", "ddm_table", "ddm_legacy", "ornstein", "full_ddm"])
 def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     """Once at most 32 walks are left in a wave, the wave's idle lanes draw the Philox blocks
     of the walking lanes. A set of 64 walks launched alone ends that way: its wave holds all 64
@@ -323,7 +323,9 @@ def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     t = np.full(K, 0.1, F32)
     dl, s = np.full(K, 999, F32), np.ones(K, F32)
     table = None
-    if model == "ddm":
+    if model in ("ddm", "ddm_deadline"):
+        if model == "ddm_deadline":  # the longest walks end at the deadline, inside a tail
+            dl[0] = 1.9
         bp = {"a": a, "theta": np.concatenate([[0.3], np.zeros(K - 1)]).astype(F32)}
         kind, bound = L.MODEL_DDM, L.BOUNDARY_ANGLE
         rec = em.ddm_records(v, a, z, t, dl, s, 20.0, DT, bound, bp)
@@ -360,6 +362,9 @@ def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     assert np.array_equal(c_alone, full.choices[:n0].cpu().numpy())
     # the set's walks span a range of lengths, so the alone launch's wave shrank through every
     # tail regime (32, 16, 8 and fewer walks)
+    if model == "ddm_deadline":
+        assert np.sum(r_alone == -999) >= 2  # walks cut at the deadline
+        r_alone = r_alone[r_alone > 0]
     steps = np.round((r_alone - 0.1) / DT)
     assert np.ptp(steps) > 1000, (steps.min(), steps.max())
 
