@@ -189,6 +189,22 @@ __device__ __forceinline__ bool inside(float y, float b) {
   if constexpr (MODEL == SSM_MODEL_DDM_LEGACY) return (y <= b) && (y >= 0.f);
   else return fabsf(y) <= b;
 }
+// Lane shifts read the source lane whether or not this lane's result is used, so every call
+// sits where all 64 lanes are active: DPP treats an inactive source lane as out of range.
+// Lane i of each row of 16 lanes reads lane i - 1 of its row; lane 0 of a row reads 0
+// (DPP row_shr:1 with bound_ctrl; the compiler folds it into the add that consumes it).
+__device__ __forceinline__ float row_shr1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
+}
+// Lane i of each row of 16 lanes reads lane i - 1 of its row; lane 0 of a row keeps `old`.
+__device__ __forceinline__ float row_shr1_or(float old, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v),
+                                                               0x111, 0xf, 0xf, false));
+}
+// Every lane of a row of 16 lanes reads lane 15 of its row (DPP row_newbcast:15).
+__device__ __forceinline__ float row_bcast15(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x15F, 0xf, 0xf, false));
+}
 // hierarchical two-choice DDMs: a high walk, then a low walk chosen by its outcome
 template <int MODEL>
 constexpr bool kTwoStage = MODEL == SSM_MODEL_PAR2 || MODEL == SSM_MODEL_SEQ2;
@@ -214,11 +230,10 @@ constexpr bool kHelperTail = !REPLAY && !RECORD &&
                              (MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM ||
                               MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_DDM_LEGACY);
 constexpr int kHelperWalks = 32;
-// With at most kLoneWalks walks left, one Philox pass of the whole wave serves one tail
-// iteration of 64 / kp blocks (kp = the walks rounded up to a power of two): at most
-// kLoneSteps steps per walk per iteration.
+// With at most kLoneWalks walks left, each walk takes a row of 16 lanes, which form its
+// positions and times 16 steps at a time; one iteration takes at most kLoneSteps steps.
 constexpr int kLoneWalks = 4;
-constexpr int kLoneSteps = 256;
+constexpr int kLoneSteps = 64;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -564,103 +579,181 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             // Philox block of one walk, lane = rank + kp * j for block j of the walk of that
             // rank (kp = K rounded up to a power of two), and each walking lane reads its
             // normals from those lanes. A walk's 32 steps then wait for one to four Philox
-            // passes instead of eight, and up to four walks take 256, 128 or 64 steps from one
-            // pass. Same counters (sample, set, step / 4), same draws.
+            // passes instead of eight; up to four walks take their steps from chains of lane
+            // shifts. Same counters (sample, set, step / 4), same draws.
             helped = true;
             const int lane = lane_id();
             const int r = __popcll(run & ((1ull << lane) - 1ull));  // walking lanes below this one
-            const int lgk = K <= 1 ? 0 : 32 - __clz(K - 1);
-            const int kp = 1 << lgk;
-            // at most kLoneWalks walks left, far enough from the end of t_s and every deadline:
-            // one pass of 64 / kp blocks; otherwise 8 blocks in passes of nb
+            // at most kLoneWalks walks left, far enough from the end of t_s and every deadline
             const bool lone = K <= kLoneWalks &&
                               __ballot(alive && !(ix + kLoneSteps <= A.n_steps && t + A.t_fast[2] <= dl_tmp)) == 0;
-            const int nb = kSubBlocks<false> < (64 >> lgk) ? kSubBlocks<false> : (64 >> lgk);  // blocks per pass
             // rank -> lane: walking lanes send their lane id to lane r, the others fill K..63
             const int rank_lane = __builtin_amdgcn_ds_permute((alive ? r : K + lane - r) << 2, lane);
+            if (lone) {
+              // Up to four walks: the 16 lanes of row g (lanes 16g .. 16g + 15) step the walk of
+              // rank g, 16 steps per chunk, lane i holding the position and time after i steps
+              // of the chunk. Both come from chains of row shifts, value(i) = value(i - 1) + e(i)
+              // with e(i) the increment of step i - 1 and lane 0 starting from 0 + the chunk's
+              // start: each lane's value is formed by the same float additions, in the same
+              // order, as the walk's own steps (0 + y == y but for y = -0, whose sign never
+              // reaches a result). The 16 tests then run at once and the walk keeps the state
+              // at its first failing step. One Philox pass (block j of rank w in lane 4j + w)
+              // serves four chunks; the iteration ends after the chunk in which a walk ends.
+              const int gi = lane & 15, grp = lane >> 4;
+              const int ol = __shfl(rank_lane, (lane & 3) < K ? (lane & 3) : 0);
+              const uint32_t on = (uint32_t)__shfl(n, ol), ok = (uint32_t)__shfl(k, ol);
+              const uint32_t oblk = (uint32_t)__shfl(ix, ol) >> 2;  // walks are block-aligned
+              const float obmc = __shfl(bmc, ol);
+              float hz[4];
+              philox_normals4_c(on, ok, oblk + (uint32_t)(lane >> 2), 0u, A.k0, A.k1, obmc, hz);
+              // the walk of this row (rows past the last walk repeat walk 0 and never end)
+              const bool gwalk = grp < K;
+              const int wl = __shfl(rank_lane, gwalk ? grp : 0);
+              float Y = __shfl(x[0], wl), T = __shfl(t, wl);
+              const float F = __shfl(fx, wl);
+              const float gvdt = __shfl(vdt[0], wl);
+              float gv = 0.f, gleak = 0.f;
+              if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
+                gv = __shfl(vraw[0], wl);
+                gleak = __shfl(leak, wl);
+              }
+              const float ga = __shfl(a, wl), gb1 = __shfl(b1, wl), gb2 = __shfl(b2, wl);
+              const float* grow = row;
+              if constexpr (BOUND == SSM_BOUNDARY_TABLE)
+                grow = (const float*)(uintptr_t)(((uint64_t)(uint32_t)__shfl((int)(uint32_t)(uintptr_t)row, wl)) |
+                                                 ((uint64_t)(uint32_t)__shfl((int)(uint32_t)((uintptr_t)row >> 32), wl) << 32));
+              const int gix = (int)F;
+              float yf = Y, tf = T, ff = F;
+              bool gexit = false;
+              for (int c = 0; c < 4; ++c) {
+                iters += c ? 4 : 3;  // four blocks of four steps; the loop counted the first
+                const int q = (c << 4) + gi;  // this lane's step within the pass
+                const int src = ((q >> 2) << 2) + (gwalk ? grp : 0);
+                const float z0 = __shfl(hz[0], src), z1 = __shfl(hz[1], src), z2 = __shfl(hz[2], src),
+                            z3 = __shfl(hz[3], src);
+                const float za = (gi & 1) ? z1 : z0, zb = (gi & 1) ? z3 : z2;
+                const float z = (gi & 2) ? zb : za;  // the normal of step q
+                const float et = gi == 0 ? T : dt;
+                float V, W = et;
+                if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
+                  // the increment depends on the position: one shift and one step per link
+                  const float zs = row_shr1(z);
+                  V = Y;
+#pragma unroll
+                  for (int s = 1; s < 16; ++s) {
+                    float vs = row_shr1(V);
+                    asm volatile("" : "+v"(vs));  // the shift stays outside the select below
+                    const float vn = vs + walk_increment<MODEL, REPLAY>(vs, gvdt, gv, gleak, dt, 0.f, zs);
+                    V = gi == 0 ? Y : vn;
+                    W = row_shr1(W) + et;
+                  }
+                } else {
+                  const float d = walk_increment<MODEL, REPLAY>(0.f, gvdt, gv, gleak, dt, 0.f, z);
+                  const float e = row_shr1_or(Y, d);  // lane 0: the chunk's start
+                  V = e;
+#pragma unroll
+                  for (int s = 1; s < 16; ++s) {
+                    V = row_shr1(V) + e;
+                    W = row_shr1(W) + et;
+                  }
+                }
+                const bool okq = !gwalk || inside<MODEL>(V, boundary_at_f<BOUND>(F + (float)q, gix + q, dt, ga, gb1, gb2, grow));
+                // the state after this lane's step; lane 15's starts the next chunk
+                const float Vn = V + walk_increment<MODEL, REPLAY>(V, gvdt, gv, gleak, dt, 0.f, z);
+                const float Wn = W + dt;
+                const uint64_t fail = __ballot(!okq);
+                if (fail != 0) {
+                  const uint32_t fg = (uint32_t)(fail >> (grp << 4)) & 0xFFFFu;
+                  gexit = fg != 0;
+                  const int j = gexit ? __builtin_ctz(fg) : 15;
+                  const int sl = (grp << 4) + j;
+                  yf = __shfl(gexit ? V : Vn, sl);
+                  tf = __shfl(gexit ? W : Wn, sl);
+                  ff = F + (float)((c << 4) + (gexit ? j : 16));
+                  break;
+                }
+                Y = row_bcast15(Vn);
+                T = row_bcast15(Wn);
+                yf = Y;
+                tf = T;
+                ff = F + (float)((c + 1) << 4);
+              }
+              const int sg = alive ? (r << 4) : 0;
+              const float ny = __shfl(yf, sg), nt = __shfl(tf, sg), nf = __shfl(ff, sg);
+              const bool nex = __shfl((int)gexit, sg) != 0;
+              if (alive) {
+                x[0] = ny;
+                t = nt;
+                fx = nf;
+                alive = !nex;
+              }
+            } else {
+            const int lgk = K <= 1 ? 0 : 32 - __clz(K - 1);
+            const int kp = 1 << lgk;
+            const int nb = kSubBlocks<false> < (64 >> lgk) ? kSubBlocks<false> : (64 >> lgk);  // blocks per pass
             const int ol = __shfl(rank_lane, lane & (kp - 1));  // the walk this lane draws for
             const uint32_t on = (uint32_t)__shfl(n, ol), ok = (uint32_t)__shfl(k, ol);
             const uint32_t oblk = (uint32_t)__shfl(ix, ol) >> 2;  // walks are block-aligned
             const float obmc = __shfl(bmc, ol);
             float hz[4];
-            // A wave with few walks left waits on each walk's step latency rather than on issue
-            // slots. A block's four positions and times are the same float sums whether or not
-            // a test fails, so they are formed first; the four tests then run independently and
-            // the walk keeps the state at its first failing test (fx + i is exact). Same steps,
-            // same rounding, a chain of four adds and four selects per block instead of four
-            // test-and-branch rounds.
-            auto step_block = [&](const float (&cz)[4]) {
-              const int ixb = (int)fx;  // this block's first step index
-              float yv[5], tv[5];
-              yv[0] = x[0];
-              tv[0] = t;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cz[i]);
-                tv[i + 1] = tv[i] + dt;
-              }
-              bool okb[4];
-#pragma unroll
-              for (int i = 0; i < 4; ++i)
-                okb[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
-              float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
-#pragma unroll
-              for (int i = 3; i >= 0; --i) {
-                yf = okb[i] ? yf : yv[i];
-                tf = okb[i] ? tf : tv[i];
-                ff = okb[i] ? ff : (i ? fx + (float)i : fx);
-              }
-              alive = okb[0] && okb[1] && okb[2] && okb[3];
-              x[0] = yf;
-              t = tf;
-              fx = ff;
-            };
-            // this walk's block p0 + j sits in lane r + kp j; the next block is read while the
-            // current one steps
-            float cur[4], nxt[4];
-            if (lone) {
-              philox_normals4_c(on, ok, oblk + (uint32_t)(lane >> lgk), 0u, A.k0, A.k1, obmc, hz);
+            bool more = true;
+            for (int p0 = 0; p0 < kSubBlocks<false> && more; p0 += nb) {
+              philox_normals4_c(on, ok, oblk + (uint32_t)(p0 + (lane >> lgk)), 0u, A.k0, A.k1, obmc, hz);
+              // this walk's block p0 + j sits in lane r + kp j; the next block is read while
+              // the current one steps
+              float cur[4], nxt[4];
 #pragma unroll
               for (int i = 0; i < 4; ++i) cur[i] = __shfl(hz[i], r);
-              const int total = 64 >> lgk;
-              for (int j = 0; j < total; ++j) {
-                if (j > 0) {
-                  if (__ballot(alive) == 0) break;
+#pragma unroll
+              for (int j = 0; j < kSubBlocks<false>; ++j) {
+                if (j >= nb) break;
+                if (p0 + j > 0) {
+                  if (__ballot(alive) == 0) {
+                    more = false;
+                    break;
+                  }
                   ++iters;
                 }
-                if (j + 1 < total) {
+                if (j + 1 < nb) {
 #pragma unroll
                   for (int i = 0; i < 4; ++i) nxt[i] = __shfl(hz[i], r + ((j + 1) << lgk));
                 }
-                if (alive) step_block(cur);
+                if (alive) {
+                  // A wave with few walks left waits on each walk's step latency rather than on
+                  // issue slots. The block's four positions and times are the same float sums
+                  // whether or not a test fails, so they are formed first; the four tests then
+                  // run independently and the walk keeps the state at its first failing test
+                  // (fx + i is exact). Same steps, same rounding, a chain of four adds and four
+                  // selects per block instead of four test-and-branch rounds.
+                  const int ixb = (int)fx;  // this block's first step index
+                  float yv[5], tv[5];
+                  yv[0] = x[0];
+                  tv[0] = t;
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) {
+                    yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cur[i]);
+                    tv[i + 1] = tv[i] + dt;
+                  }
+                  bool okb[4];
+#pragma unroll
+                  for (int i = 0; i < 4; ++i)
+                    okb[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
+                  float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
+#pragma unroll
+                  for (int i = 3; i >= 0; --i) {
+                    yf = okb[i] ? yf : yv[i];
+                    tf = okb[i] ? tf : tv[i];
+                    ff = okb[i] ? ff : (i ? fx + (float)i : fx);
+                  }
+                  alive = okb[0] && okb[1] && okb[2] && okb[3];
+                  x[0] = yf;
+                  t = tf;
+                  fx = ff;
+                }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
               }
-            } else {
-              bool more = true;
-              for (int p0 = 0; p0 < kSubBlocks<false> && more; p0 += nb) {
-                philox_normals4_c(on, ok, oblk + (uint32_t)(p0 + (lane >> lgk)), 0u, A.k0, A.k1, obmc, hz);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cur[i] = __shfl(hz[i], r);
-#pragma unroll
-                for (int j = 0; j < kSubBlocks<false>; ++j) {
-                  if (j >= nb) break;
-                  if (p0 + j > 0) {
-                    if (__ballot(alive) == 0) {
-                      more = false;
-                      break;
-                    }
-                    ++iters;
-                  }
-                  if (j + 1 < nb) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) nxt[i] = __shfl(hz[i], r + ((j + 1) << lgk));
-                  }
-                  if (alive) step_block(cur);
-#pragma unroll
-                  for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
-                }
-              }
+            }
             }
           }
         }
