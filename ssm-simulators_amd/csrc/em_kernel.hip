@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 
 #include "ssmhip_internal.h"
 
@@ -201,6 +202,13 @@ __device__ __forceinline__ float row_shr1_or(float old, float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v),
                                                                0x111, 0xf, 0xf, false));
 }
+// Every lane of a row of 16 lanes reads lane 7 of its row (DPP row_newbcast:7).
+__device__ __forceinline__ float row_bcast7(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x157, 0xf, 0xf, false));
+}
+// Keeps a lane-shift result where it is formed: without it the compiler may move the shift
+// into the branch of a select that uses it, where only some lanes are active.
+__device__ __forceinline__ void dpp_fence(float& v) { asm volatile("" : "+v"(v)); }
 // Every lane of a row of 16 lanes reads lane 15 of its row (DPP row_newbcast:15).
 __device__ __forceinline__ float row_bcast15(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x15F, 0xf, 0xf, false));
@@ -230,9 +238,10 @@ constexpr bool kHelperTail = !REPLAY && !RECORD &&
                              (MODEL == SSM_MODEL_DDM || MODEL == SSM_MODEL_FULL_DDM ||
                               MODEL == SSM_MODEL_ORNSTEIN || MODEL == SSM_MODEL_DDM_LEGACY);
 constexpr int kHelperWalks = 32;
-// With at most kLoneWalks walks left, each walk takes a row of 16 lanes, which form its
-// positions and times 16 steps at a time; one iteration takes at most kLoneSteps steps.
-constexpr int kLoneWalks = 4;
+// With at most kLoneWalks walks left, each walk takes a group of 16 or 8 lanes, which form
+// its positions and times that many steps at a time; one iteration takes at most kLoneSteps
+// steps.
+constexpr int kLoneWalks = 8;
 constexpr int kLoneSteps = 64;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
@@ -579,7 +588,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             // Philox block of one walk, lane = rank + kp * j for block j of the walk of that
             // rank (kp = K rounded up to a power of two), and each walking lane reads its
             // normals from those lanes. A walk's 32 steps then wait for one to four Philox
-            // passes instead of eight; up to four walks take their steps from chains of lane
+            // passes instead of eight; up to 8 walks take their steps from chains of lane
             // shifts. Same counters (sample, set, step / 4), same draws.
             helped = true;
             const int lane = lane_id();
@@ -589,24 +598,29 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                               __ballot(alive && !(ix + kLoneSteps <= A.n_steps && t + A.t_fast[2] <= dl_tmp)) == 0;
             // rank -> lane: walking lanes send their lane id to lane r, the others fill K..63
             const int rank_lane = __builtin_amdgcn_ds_permute((alive ? r : K + lane - r) << 2, lane);
-            if (lone) {
-              // Up to four walks: the 16 lanes of row g (lanes 16g .. 16g + 15) step the walk of
-              // rank g, 16 steps per chunk, lane i holding the position and time after i steps
-              // of the chunk. Both come from chains of row shifts, value(i) = value(i - 1) + e(i)
-              // with e(i) the increment of step i - 1 and lane 0 starting from 0 + the chunk's
-              // start: each lane's value is formed by the same float additions, in the same
-              // order, as the walk's own steps (0 + y == y but for y = -0, whose sign never
-              // reaches a result). The 16 tests then run at once and the walk keeps the state
-              // at its first failing step. One Philox pass (block j of rank w in lane 4j + w)
-              // serves four chunks; the iteration ends after the chunk in which a walk ends.
-              const int gi = lane & 15, grp = lane >> 4;
-              const int ol = __shfl(rank_lane, (lane & 3) < K ? (lane & 3) : 0);
+            // Chains of lane shifts: each walk takes a group of G lanes (G = 16 or 8 for up to 4
+            // or 8 walks) and steps G steps per chunk, lane i holding the position and time
+            // after i steps of the chunk. Both come from chains value(i) = value(i - 1) + e(i),
+            // e(i) the increment of step i - 1, value(0) the chunk's start: after G - 1 links
+            // every lane's value is formed by the same float additions, in the same order, as
+            // the walk's own steps. The G tests then run at once and the walk keeps the state at
+            // its first failing step. One Philox pass (block j of rank w in lane j (64 / G) + w)
+            // serves four chunks; the iteration ends after the chunk in which a walk ends.
+            // A group's first lane: for G = 16 the row shift reads 0 there, and 0 + y == y but
+            // for y = -0, whose sign never reaches a result; for G = 8 a select restores it
+            // after every link (the row shift reads across the group boundary at lane 8).
+            auto chain_tail = [&](auto gtag) {
+              constexpr int G = decltype(gtag)::value;
+              constexpr int NW = 64 / G;  // walks one pass serves
+              const int gi = lane & (G - 1), grp = lane / G;
+              const int pw = lane % NW;
+              const int ol = __shfl(rank_lane, pw < K ? pw : 0);
               const uint32_t on = (uint32_t)__shfl(n, ol), ok = (uint32_t)__shfl(k, ol);
               const uint32_t oblk = (uint32_t)__shfl(ix, ol) >> 2;  // walks are block-aligned
               const float obmc = __shfl(bmc, ol);
               float hz[4];
-              philox_normals4_c(on, ok, oblk + (uint32_t)(lane >> 2), 0u, A.k0, A.k1, obmc, hz);
-              // the walk of this row (rows past the last walk repeat walk 0 and never end)
+              philox_normals4_c(on, ok, oblk + (uint32_t)(lane / NW), 0u, A.k0, A.k1, obmc, hz);
+              // the walk of this group (groups past the last walk repeat walk 0 and never end)
               const bool gwalk = grp < K;
               const int wl = __shfl(rank_lane, gwalk ? grp : 0);
               float Y = __shfl(x[0], wl), T = __shfl(t, wl);
@@ -623,62 +637,91 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                 grow = (const float*)(uintptr_t)(((uint64_t)(uint32_t)__shfl((int)(uint32_t)(uintptr_t)row, wl)) |
                                                  ((uint64_t)(uint32_t)__shfl((int)(uint32_t)((uintptr_t)row >> 32), wl) << 32));
               const int gix = (int)F;
-              float yf = Y, tf = T, ff = F;
-              bool gexit = false;
+              const bool first = gi == 0;
+              // the normals of this lane's step in each chunk, all read before the first chunk
+              float zc[4];
+#pragma unroll
               for (int c = 0; c < 4; ++c) {
-                iters += c ? 4 : 3;  // four blocks of four steps; the loop counted the first
-                const int q = (c << 4) + gi;  // this lane's step within the pass
-                const int src = ((q >> 2) << 2) + (gwalk ? grp : 0);
+                const int src = ((c * G + gi) >> 2) * NW + (gwalk ? grp : 0);
                 const float z0 = __shfl(hz[0], src), z1 = __shfl(hz[1], src), z2 = __shfl(hz[2], src),
                             z3 = __shfl(hz[3], src);
                 const float za = (gi & 1) ? z1 : z0, zb = (gi & 1) ? z3 : z2;
-                const float z = (gi & 2) ? zb : za;  // the normal of step q
-                const float et = gi == 0 ? T : dt;
+                zc[c] = (gi & 2) ? zb : za;
+              }
+              float yf = Y, tf = T, ff = F;
+              bool gexit = false;
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                iters += c ? G / 4 : G / 4 - 1;  // blocks of four steps; the loop counted one
+                const int q = c * G + gi;  // this lane's step within the pass
+                const float z = zc[c];     // the normal of step q
+                const float et = first ? T : dt;
                 float V, W = et;
                 if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
                   // the increment depends on the position: one shift and one step per link
-                  const float zs = row_shr1(z);
+                  float zs = row_shr1(z);
+                  dpp_fence(zs);
                   V = Y;
 #pragma unroll
-                  for (int s = 1; s < 16; ++s) {
+                  for (int s = 1; s < G; ++s) {
                     float vs = row_shr1(V);
-                    asm volatile("" : "+v"(vs));  // the shift stays outside the select below
+                    dpp_fence(vs);
                     const float vn = vs + walk_increment<MODEL, REPLAY>(vs, gvdt, gv, gleak, dt, 0.f, zs);
-                    V = gi == 0 ? Y : vn;
+                    V = first ? Y : vn;
                     W = row_shr1(W) + et;
+                    if constexpr (G == 8) {
+                      dpp_fence(W);
+                      W = first ? T : W;
+                    }
                   }
                 } else {
-                  const float d = walk_increment<MODEL, REPLAY>(0.f, gvdt, gv, gleak, dt, 0.f, z);
-                  const float e = row_shr1_or(Y, d);  // lane 0: the chunk's start
+                  const float e = row_shr1_or(Y, walk_increment<MODEL, REPLAY>(0.f, gvdt, gv, gleak, dt, 0.f, z));
                   V = e;
+                  if constexpr (G == 8) V = first ? Y : V;
 #pragma unroll
-                  for (int s = 1; s < 16; ++s) {
+                  for (int s = 1; s < G; ++s) {
                     V = row_shr1(V) + e;
                     W = row_shr1(W) + et;
+                    if constexpr (G == 8) {
+                      dpp_fence(V);
+                      dpp_fence(W);
+                      V = first ? Y : V;
+                      W = first ? T : W;
+                    }
                   }
                 }
                 const bool okq = !gwalk || inside<MODEL>(V, boundary_at_f<BOUND>(F + (float)q, gix + q, dt, ga, gb1, gb2, grow));
-                // the state after this lane's step; lane 15's starts the next chunk
+                // the state after this lane's step; lane G - 1's starts the next chunk
                 const float Vn = V + walk_increment<MODEL, REPLAY>(V, gvdt, gv, gleak, dt, 0.f, z);
                 const float Wn = W + dt;
                 const uint64_t fail = __ballot(!okq);
                 if (fail != 0) {
-                  const uint32_t fg = (uint32_t)(fail >> (grp << 4)) & 0xFFFFu;
+                  const uint32_t fg = (uint32_t)(fail >> (grp * G)) & ((1u << G) - 1u);
                   gexit = fg != 0;
-                  const int j = gexit ? __builtin_ctz(fg) : 15;
-                  const int sl = (grp << 4) + j;
+                  const int j = gexit ? __builtin_ctz(fg) : G - 1;
+                  const int sl = grp * G + j;
                   yf = __shfl(gexit ? V : Vn, sl);
                   tf = __shfl(gexit ? W : Wn, sl);
-                  ff = F + (float)((c << 4) + (gexit ? j : 16));
+                  ff = F + (float)(c * G + (gexit ? j : G));
                   break;
                 }
-                Y = row_bcast15(Vn);
-                T = row_bcast15(Wn);
+                if constexpr (G == 16) {
+                  Y = row_bcast15(Vn);
+                  T = row_bcast15(Wn);
+                } else {
+                  float y7 = row_bcast7(Vn), y15 = row_bcast15(Vn), t7 = row_bcast7(Wn), t15 = row_bcast15(Wn);
+                  dpp_fence(y7);
+                  dpp_fence(y15);
+                  dpp_fence(t7);
+                  dpp_fence(t15);
+                  Y = (lane & 8) ? y15 : y7;
+                  T = (lane & 8) ? t15 : t7;
+                }
                 yf = Y;
                 tf = T;
-                ff = F + (float)((c + 1) << 4);
+                ff = F + (float)((c + 1) * G);
               }
-              const int sg = alive ? (r << 4) : 0;
+              const int sg = alive ? r * G : 0;
               const float ny = __shfl(yf, sg), nt = __shfl(tf, sg), nf = __shfl(ff, sg);
               const bool nex = __shfl((int)gexit, sg) != 0;
               if (alive) {
@@ -687,6 +730,10 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                 fx = nf;
                 alive = !nex;
               }
+            };
+            if (lone) {
+              if (K <= 4) chain_tail(std::integral_constant<int, 16>{});
+              else chain_tail(std::integral_constant<int, 8>{});
             } else {
             const int lgk = K <= 1 ? 0 : 32 - __clz(K - 1);
             const int kp = 1 << lgk;
