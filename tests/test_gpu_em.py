@@ -308,10 +308,11 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
 @pytest.mark.parametrize("model", ["ddm", "ddm_deadline", "ddm_table", "ddm_legacy", "ornstein", "full_ddm"])
 def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     """Once at most 32 walks are left in a wave, the wave's idle lanes draw the Philox blocks
-    of the walking lanes. A set of 64 walks launched alone ends that way: its wave holds all 64
-    and runs its last 32 in that mode, down to one walk. The same set launched ahead of 100 sets
-    of slow walks, with waves that refill as soon as a lane is free, runs every walk of it in a
-    full wave. Both launches must give identical rts, choices and step counts."""
+    of the walking lanes, and from 8 walks on each walk forms its positions in a group of lanes.
+    A set of 64 walks launched alone ends that way: its wave holds all 64 and runs its last 32
+    in those modes, down to one walk. The same set launched ahead of 100 sets of slow walks,
+    with waves that refill as soon as a lane is free, runs every walk of it in a full wave.
+    Both launches must give identical rts and choices."""
     import torch
 
     L, em = _ssm()
@@ -361,7 +362,7 @@ def test_native_tail_walks_equal_full_wave_walks(torch_gpu, model):
     assert np.array_equal(r_alone, full.rts[:n0].cpu().numpy())
     assert np.array_equal(c_alone, full.choices[:n0].cpu().numpy())
     # the set's walks span a range of lengths, so the alone launch's wave shrank through every
-    # tail regime (32, 16, 8 and fewer walks)
+    # tail regime (32, 16, 8, 4 and fewer walks)
     if model == "ddm_deadline":
         assert np.sum(r_alone == -999) >= 2  # walks cut at the deadline
         r_alone = r_alone[r_alone > 0]
