@@ -22,12 +22,18 @@ def torch():
     return _torch
 
 
+_gpu_ok = False
+
+
 def require_gpu():
-    """Return the torch module after checking that a HIP device is usable."""
+    """Return the torch module after checking (once per process) that a HIP device is usable."""
+    global _gpu_ok
     t = torch()
-    if not t.cuda.is_available():
-        raise RuntimeError("ssmhip needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path")
-    _lib.load()
+    if not _gpu_ok:
+        if not t.cuda.is_available():
+            raise RuntimeError("ssmhip needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path")
+        _lib.load()
+        _gpu_ok = True
     return t
 
 

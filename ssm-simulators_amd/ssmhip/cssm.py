@@ -110,7 +110,7 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
         job.launch()
     total = n_samples * n_trials
     rts = job.rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
-    choices = job.choices[:total].cpu().numpy().astype(np.intc).reshape(n_samples, n_trials, 1)
+    choices = job.choices[:total].cpu().numpy().astype(np.intc, copy=False).reshape(n_samples, n_trials, 1)
     traj = None
     if trajectory:
         if traj_dev is not None:

@@ -30,16 +30,31 @@ def sqrt_dt32(delta_t: float) -> np.float32:
 
 def deadline_tmp32(max_t: float, deadline: np.ndarray, t: np.ndarray) -> np.ndarray:
     """compute_deadline_tmp (_utils.pyx:712-718): min(max_t, deadline - t) in float32."""
-    d = (np.asarray(deadline, F32) - np.asarray(t, F32)).astype(F32)
-    return np.minimum(F32(max_t), d).astype(F32)
+    d = np.subtract(np.asarray(deadline, F32), np.asarray(t, F32), dtype=F32)
+    return np.minimum(d, F32(max_t), out=d)
+
+
+def _slope(th: np.ndarray) -> np.ndarray:
+    s = np.sin(th)
+    np.negative(s, out=s)
+    np.divide(s, np.cos(th), out=s)
+    return s
 
 
 def angle_slope32(theta: np.ndarray) -> np.ndarray:
     """-sin(theta)/cos(theta) in float32, the slope boundary_functions.angle forms from each
     trial's scalar theta. NumPy's float32 sin/cos run one SIMD loop with masked tails for
-    any length, so the array call gives the per-scalar values (tests/test_host.py checks it)."""
+    any length, so the array call gives the per-scalar values (tests/test_host.py checks it).
+    Long inputs made of runs of one value (parameters broadcast over trials) are evaluated
+    once per run."""
     th = np.asarray(theta, F32).reshape(-1)
-    return (-np.sin(th) / np.cos(th)).astype(F32)
+    if th.size >= 65536:
+        starts = np.flatnonzero(th[1:] != th[:-1]) + 1
+        if starts.size < th.size // 8:
+            starts = np.concatenate([[0], starts])
+            runs = np.diff(np.append(starts, th.size))
+            return np.repeat(_slope(th[starts]), runs)
+    return _slope(th)
 
 
 def boundary_table(fun, params: dict, n_sets: int, max_t: float, delta_t: float) -> np.ndarray:
@@ -105,7 +120,7 @@ def ddm_columns(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
     cols[L.P_T0] = t
     cols[L.P_DEADLINE] = deadline
     cols[L.P_DEADLINE_TMP] = deadline_tmp32(max_t, deadline, t)
-    cols[L.P_SQRT_ST] = (sqrt_dt32(delta_t) * s).astype(F32)
+    cols[L.P_SQRT_ST] = np.multiply(s, sqrt_dt32(delta_t), dtype=F32)
     if alpha is not None:
         al = first_column(alpha, n)
         if np.any(~(al > 0)) or np.any(al > 2):
@@ -128,6 +143,11 @@ def ddm_records(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=None,
         raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
     n, cols = ddm_columns(v, a, z, t, deadline, s, max_t, delta_t, code, bparams, sz=sz, sv=sv, st=st, g=g,
                           alpha=alpha)
+    return records_from_columns(n, cols, stride)
+
+
+def records_from_columns(n: int, cols: dict, stride: int) -> np.ndarray:
+    """The host layout float32[n, stride] of single-particle {slot: column} records."""
     # the single-particle slots all lie in the first 16 floats: each slot is written as one
     # contiguous row of a slot-major (16, n) array (rec is its transposed view), then the
     # records are laid out by one transposing copy (and, for 64-float records, one more)
@@ -159,11 +179,19 @@ def records_on_device(n: int, cols: dict, stride: int):
     return out
 
 
+# below this many sets the records are laid out on the host and uploaded in one copy (a
+# copy per slot costs more than the transpose); from it on, on the device (records_on_device)
+DEVICE_LAYOUT_MIN_SETS = 4096
+
+
 def ddm_records_device(*args, stride: int = L.SET_STRIDE_SINGLE, **kwargs):
-    """ddm_records laid out on the device (records_on_device); same arguments."""
+    """ddm_records as a device tensor float32[n, stride]; same arguments. Bit-identical
+    whichever side lays them out."""
     if stride not in (L.SET_STRIDE, L.SET_STRIDE_SINGLE):
         raise ValueError(f"stride must be {L.SET_STRIDE} or {L.SET_STRIDE_SINGLE}")
     n, cols = ddm_columns(*args, **kwargs)
+    if n < DEVICE_LAYOUT_MIN_SETS:
+        return D.to_dev(records_from_columns(n, cols, stride), F32)
     return records_on_device(n, cols, stride)
 
 

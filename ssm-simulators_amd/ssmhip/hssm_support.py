@@ -67,9 +67,12 @@ def _prepare_theta_and_shape(arg_arrays, size):
         else:
             raise ValueError(f"Size must be a tuple of length 1 or an integer, but got {type(size)}")
         return True, np.tile(theta, (n_rows, 1)), None, None
-    sizes = [np.asarray(arg).size for arg in arg_arrays]
-    max_shape = np.asarray(arg_arrays[int(np.argmax(sizes))]).shape
-    theta = np.column_stack([np.broadcast_to(arg, max_shape).reshape(-1) for arg in arg_arrays])
+    arrays = [np.asarray(arg) for arg in arg_arrays]
+    max_shape = arrays[int(np.argmax([a.size for a in arrays]))].shape
+    # column-major, so that each parameter's column is contiguous for the records built from it
+    theta = np.empty((int(np.prod(max_shape)), len(arrays)), dtype=np.result_type(*arrays), order="F")
+    for j, arr in enumerate(arrays):
+        theta[:, j].reshape(max_shape)[...] = arr
     return False, theta, max_shape, max_shape[-1]
 
 

@@ -1,5 +1,6 @@
 """Set records laid out on the device from per-slot columns (ssm_em_records) equal the host
-layout of the same columns bit for bit, for every single-particle diffusion record kind; the
+layout of the same columns bit for bit, for every single-particle diffusion record kind, at set
+counts on either side of the one from which the simulators lay records out on the device; the
 uniform offsets written on the device equal k * n_samples."""
 
 import numpy as np
@@ -28,7 +29,7 @@ def _cases(n, rng):
     yield "levy", base + (L.BOUNDARY_CONSTANT, {"a": a}), {"alpha": rng.uniform(1.0, 2.0, n)}
 
 
-@pytest.mark.parametrize("n", [1, 257, 100_003])
+@pytest.mark.parametrize("n", [1, 257, 4095, 4096, 100_003])
 def test_device_records_equal_host_records(torch_gpu, n):
     from ssmhip import _lib as L
     from ssmhip import em
@@ -38,9 +39,12 @@ def test_device_records_equal_host_records(torch_gpu, n):
         print(f"[case] records on the device: {name}, {n} sets", flush=True)
         for stride in (L.SET_STRIDE_SINGLE, L.SET_STRIDE):
             host = em.ddm_records(*args, **kw, stride=stride)
-            dev = em.ddm_records_device(*args, **kw, stride=stride).cpu().numpy()
-            assert dev.shape == host.shape
-            assert np.array_equal(dev.view(np.uint32), host.view(np.uint32)), name
+            # the device layout itself, and the helper that picks a side by the set count
+            n_cols, cols = em.ddm_columns(*args, **kw)
+            for dev in (em.records_on_device(n_cols, cols, stride).cpu().numpy(),
+                        em.ddm_records_device(*args, **kw, stride=stride).cpu().numpy()):
+                assert dev.shape == host.shape
+                assert np.array_equal(dev.view(np.uint32), host.view(np.uint32)), name
 
 
 def test_device_records_for_zero_sets_and_uniform_offsets(torch_gpu):

@@ -189,6 +189,29 @@ def test_angle_slope_array_call_equals_the_per_trial_scalar_calls():
     th[:3] = np.float32([0.0, 1.3, -0.1])
     scalar = np.array([(-np.sin(x) / np.cos(x)) for x in th], dtype=np.float32)
     assert np.array_equal(em.angle_slope32(th), scalar)
+    # long inputs made of runs (a parameter broadcast over trials) are evaluated once per run:
+    # runs of unequal lengths, a run of one, and the same value in two separate runs
+    lens = np.random.default_rng(1).integers(1, 3000, 200)
+    lens[7] = 1
+    vals = th[:200].copy()
+    vals[150] = vals[10]
+    runs = np.repeat(vals, lens)
+    assert runs.size >= 65536
+    want = scalar[:200].copy()
+    want[150] = scalar[10]
+    assert np.array_equal(em.angle_slope32(runs), np.repeat(want, lens))
+
+
+def test_deadline_tmp_is_the_float32_minimum_of_max_t_and_deadline_minus_t():
+    """compute_deadline_tmp (_utils.pyx:712-718) per trial, in float32."""
+    from ssmhip import em
+
+    rng = np.random.default_rng(3)
+    t = rng.uniform(0, 2, 1000).astype(np.float32)
+    dl = np.where(rng.uniform(size=1000) < 0.5, 999.0, rng.uniform(0, 30, 1000)).astype(np.float32)
+    want = np.array([min(np.float32(20.0), np.float32(d - x)) for d, x in zip(dl, t)], dtype=np.float32)
+    got = em.deadline_tmp32(20.0, dl, t)
+    assert got.dtype == np.float32 and np.array_equal(got, want)
 
 
 def test_bench_refuses_more_than_one_gpu():
