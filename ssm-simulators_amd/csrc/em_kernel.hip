@@ -288,6 +288,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
   float vdt[MAXP], sst[MAXP], vraw[MAXP];
   float bmc = kBoxMullerLog2;  // native Gaussian diffusions: -2 ln 2 sqrt_st^2 (normals scaled by sqrt_st)
   float a = 0.f, b1 = 0.f, b2 = 0.f, t0 = 0.f, deadline = 0.f, dl_tmp = 0.f, leak = 0.f, inh = 0.f;
+  float leak_dt = 0.f, inh_dt = 0.f;  // native LCA walks: g dt and b dt
   float alpha = 2.f, alpha_inv = 0.5f, alpha_scale = -0.5f;  // Levy stability and its two exponents
   int win = -1;  // racing diffusion: first accumulator past the threshold in its last step
   // two-stage models: stage (0 high, 1 low), choice so far, low choice already taken, the
@@ -507,6 +508,8 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             } else {
               leak = sp[SSM_P_G];
               inh = sp[SSM_P_BINH];
+              leak_dt = leak * dt;
+              inh_dt = inh * dt;
 #pragma unroll
               for (int j = 0; j < MAXP; ++j) {
                 if (j < P) {
@@ -973,15 +976,23 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         }
       }
     } else {
-      // up to kAccSteps steps per loop iteration (the loop's own work is paid once for them)
+      // up to kAccSteps steps per loop iteration (the loop's own work is paid once for them).
+      // Native LCA: an iteration in which no walking lane can reach the end of t_s or its
+      // deadline skips the per-step index and deadline tests, as the diffusions' do (measured:
+      // LCA-4 faster, race-4 slower with it)
+      constexpr int kIter = kAccStepsR<REPLAY, MAXP, RECORD>;
+      bool untested = false;
+      if constexpr (MODEL == SSM_MODEL_LCA && !REPLAY && !RECORD)
+        untested = __ballot(state == kRunning && !(ix + kIter <= A.n_steps && t + A.t_fast[0] <= dl_tmp)) == 0;
 #pragma unroll
-      for (int sb = 0; sb < kAccStepsR<REPLAY, MAXP, RECORD>; ++sb) {
+      for (int sb = 0; sb < kIter; ++sb) {
         if (sb > 0) {
           if (__ballot(state == kRunning) == 0) break;  // wave-uniform: no lane left to step
           ++iters;
         }
         if (state == kRunning) {
-          bool go = ix < A.n_steps && t <= dl_tmp;
+          bool go = true;
+          if (!untested) go = ix < A.n_steps && t <= dl_tmp;
           if (MODEL != SSM_MODEL_RDM && go) {  // racing diffusion tests the threshold inside the step
             const float b = boundary_at<BOUND>(ix, dt, a, b1, b2, row);
 #pragma unroll
@@ -1029,8 +1040,15 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               for (int j = 0; j < MAXP; ++j) {
                 if (j < P) {
                   const float red = (-x[j]) + sum;  // race_models.pyx:851
-                  const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
-                  xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
+                  if constexpr (REPLAY) {
+                    const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
+                    xnew[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));  // :852-854
+                  } else {
+                    // native walks: the same update, (v - g x - b red) dt + s sqrt(dt) z, with dt
+                    // taken into the rates once per walk and each product fused with its sum
+                    const float dr = __builtin_fmaf(-leak_dt, x[j], __builtin_fmaf(-inh_dt, red, vdt[j]));
+                    xnew[j] = fmaxf(0.0f, x[j] + __builtin_fmaf(sst[j], nz[j], dr));
+                  }
                 } else {
                   xnew[j] = x[j];
                 }

@@ -56,11 +56,12 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
 
 template <bool LCA>
 __global__ void __launch_bounds__(256) probe_acc_step(int64_t iters, float* out) {
-  // one accumulator-model step per iteration, as the simulator's accumulator loop does it
-  // (bound test on every accumulator, four normals from one Philox block, update), without
-  // walk termination or refill: a finished walk restarts in place
+  // one accumulator-model step per iteration, as the simulator's native accumulator loop does
+  // it (bound test on every accumulator, four normals from one Philox block, the update; LCA:
+  // in its untested iterations, with the update's products fused), without walk termination
+  // or refill: a finished walk restarts in place
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const float dt = 0.001f, a = 2.0f, leak = 0.2f, inh = 0.1f, alpha = 2.0f, binv = 1.0f / 3.0f;
+  const float dt = 0.001f, a = 2.0f, leak_dt = 0.2f * dt, inh_dt = 0.1f * dt, alpha = 2.0f, binv = 1.0f / 3.0f;
   float x[4], vraw[4], vdt[4], sst[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(256) probe_acc_step(int64_t iters, float* out)
       const float p = __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(u));
       b = a * __builtin_amdgcn_exp2f(-1.4426950408889634f * p);
     }
-    bool go = ix < 20001 && t <= 20.f;
+    bool go = LCA || (ix < 20001 && t <= 20.f);
 #pragma unroll
     for (int j = 0; j < 4; ++j) go = go && !(x[j] > b);
     float nz[4];
@@ -91,8 +92,8 @@ __global__ void __launch_bounds__(256) probe_acc_step(int64_t iters, float* out)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const float red = (-x[j]) + sum;
-        const float drift = ((vraw[j] - (leak * x[j])) - (inh * red)) * dt;
-        xn[j] = fmaxf(0.0f, x[j] + (drift + sst[j] * nz[j]));
+        const float dr = __builtin_fmaf(-leak_dt, x[j], __builtin_fmaf(-inh_dt, red, vdt[j]));
+        xn[j] = fmaxf(0.0f, x[j] + __builtin_fmaf(sst[j], nz[j], dr));
       }
     } else {
 #pragma unroll
