@@ -429,14 +429,18 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
     pred = net(dxs).cpu().numpy()
     mae = float(np.mean(np.abs(pred - ref)))
     # the same network trained on the LAN leg's own rows (1024 sets x 1e5 simulations, 800 KDE +
-    # 200 uniform rows per set: the labelling workload as configured, 1,024,000 rows)
+    # 200 uniform rows per set: the labelling workload as configured, 1,024,000 rows). Its labels
+    # are noisier (1e5 draws per KDE, 32x fewer sets): a Huber delta of 0.1 (near the absolute
+    # error) and a larger first learning rate fit them better (0.080 -> 0.063 nats over 3 seeds;
+    # the larger label set above is best with its own settings, 0.046 against 0.049)
     mae_c4 = None
+    c4_fit = {"epochs": args.surrogate_epochs, "batch": 8192, "lr": 1e-2, "lr_final": 5e-5, "delta": 0.1}
     if config4_rows is not None:
         n4 = config4_rows.shape[0]
         perm4 = torch.randperm(n4, device="cuda", generator=torch.Generator(device="cuda").manual_seed(1))
         net4 = MLPSurrogate(seed=0)
-        net4.fit(config4_rows.index_select(0, perm4[n4 // 50:]), epochs=args.surrogate_epochs, batch=8192, lr=3e-3,
-                 lr_final=5e-5, val_rows=config4_rows.index_select(0, perm4[: n4 // 50]))
+        net4.fit(config4_rows.index_select(0, perm4[n4 // 50:]), val_rows=config4_rows.index_select(0, perm4[: n4 // 50]),
+                 **c4_fit)
         mae_c4 = float(np.mean(np.abs(net4(dxs).cpu().numpy() - ref)))
         del net4, perm4
     # the KDE labels' own error at the same points (simulated KDE per parameter set, 1e5 draws)
@@ -465,7 +469,8 @@ def surrogate_leg(args, lab, kde_evals_per_s, config4_rows=None):
            "over_gpu_kde": evals / kde_evals_per_s,
            "mfma_tflops_algorithmic": flops / 1e12, "mfma_frac_of_bf16_peak": flops / 2.5e15,
            "mae_vs_analytic_theta0": mae, "kde_mae_vs_analytic_theta0": kde_mae, "analytic_rows": int(ref.size),
-           "mae_vs_analytic_theta0_trained_on_1024x1e5_rows": mae_c4}
+           "mae_vs_analytic_theta0_trained_on_1024x1e5_rows": mae_c4,
+           "training_on_1024x1e5_rows": c4_fit}
     rec["inference_published_peak"] = {
         "mfma_bf16": {"unit": "TFLOP/s", "achieved": flops / 1e12, "peak": 2500.0, "frac": flops / 2.5e15,
                       "basis": "45,640 algorithmic FLOP per row; counters: MFMA busy 40.4% of SIMD cycles "
