@@ -162,7 +162,13 @@ int32_t ssm_device_cu_count(void);
  *   workspace_dev    uint64[SSM_EM_WORKSPACE_WORDS] scratch owned by the caller
  *                    ([0] work-queue head, [1] executed EM steps)
  * `launch` may be NULL (defaults). Two launches on `stream`: a workspace reset
- * kernel and the persistent simulation kernel.                             */
+ * kernel and the persistent simulation kernel.
+ * The walks read their times from a table of t after k steps (k sequential float32
+ * additions of delta_t, k = 0..n_steps) that the library builds once per (device,
+ * delta_t, n_steps) and keeps for the life of the process (4 (n_steps + 1) bytes).
+ * The first call of a (delta_t, n_steps) on a device allocates and uploads it, so
+ * that call may not run inside a stream capture (SSM_EINVAL if it does;
+ * ssm_em_graph_create builds the table before it captures).                */
 #define SSM_EM_WORKSPACE_WORDS 4
 int32_t ssm_em_simulate(const ssm_em_problem* prob, const ssm_launch_config* launch,
                         const float* set_params_dev, const int64_t* offsets_dev,
@@ -185,7 +191,8 @@ int32_t ssm_em_trajectory(const ssm_em_problem* prob, const float* set_params_de
                           uint64_t* workspace_dev, void* stream);
 
 /* Capture the two launches of ssm_em_simulate into a HIP graph executable.
- * The pointers are baked into the graph; replay with ssm_graph_launch.      */
+ * The pointers are baked into the graph; replay with ssm_graph_launch. The
+ * walks' time table (see ssm_em_simulate) is built before the capture.      */
 int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config* launch,
                             const float* set_params_dev, const int64_t* offsets_dev,
                             const float* table_dev, const float* replay_normals_dev,

@@ -43,6 +43,7 @@ int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int6
 int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const float* set_params,
                    const int64_t* offsets, const float* table, const float* rn, const float* ru,
                    float* rts, void* choices, uint64_t* ws, hipStream_t stream);
+int32_t em_time_grid(float dt, int32_t n_steps, hipStream_t stream, const float** out);
 int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
                               const float* table, const float* rn, const float* ru, float* traj, int32_t traj_rows,
                               uint64_t* ws, hipStream_t stream);
@@ -161,6 +162,11 @@ int32_t ssm_em_graph_create(const ssm_em_problem* prob, const ssm_launch_config*
                            replay_uniforms_dev, rts_dev, choices_dev, workspace_dev);
   if (rc) return rc;
   if (!workspace_dev) return fail(SSM_EINVAL, "workspace is NULL");
+  if (prob->n_total > 0) {  // the launch's time table is built here, not inside the capture
+    const float* t_grid = nullptr;
+    rc = em_time_grid(prob->dt, prob->n_steps, nullptr, &t_grid);
+    if (rc) return rc;
+  }
   hipStream_t s = nullptr;
   SSM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   GraphHandle* h = new GraphHandle();

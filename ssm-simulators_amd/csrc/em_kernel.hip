@@ -25,7 +25,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 #include "ssmhip_internal.h"
 
@@ -60,6 +64,10 @@ struct EmArgs {
   // of one walk-loop iteration; t_fast[2] the same over a long tail iteration (native mode);
   // NaN when n_steps > 2^24 (then no iteration runs fast)
   float t_fast[3];
+  // t_grid[k]: t after k steps (k float32 additions of dt from 0), k = 0..n_steps; the
+  // untested iterations read a walk's t from it instead of adding dt per step (NULL when
+  // n_steps > 2^24, where no iteration is untested)
+  const float* t_grid;
   uint32_t k0, k1;
 };
 
@@ -602,11 +610,10 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
             // rank -> lane: walking lanes send their lane id to lane r, the others fill K..63
             const int rank_lane = __builtin_amdgcn_ds_permute((alive ? r : K + lane - r) << 2, lane);
             // Chains of lane shifts: each walk takes a group of G lanes (G = 16 or 8 for up to 4
-            // or 8 walks) and steps G steps per chunk, lane i holding the position and time
-            // after i steps of the chunk. Both come from chains value(i) = value(i - 1) + e(i),
-            // e(i) the increment of step i - 1, value(0) the chunk's start: after G - 1 links
-            // every lane's value is formed by the same float additions, in the same order, as
-            // the walk's own steps. The G tests then run at once and the walk keeps the state at
+            // or 8 walks) and steps G steps per chunk, lane i holding the position after i steps
+            // of the chunk. It comes from a chain y(i) = y(i - 1) + e(i), e(i) the increment of
+            // step i - 1, y(0) the chunk's start: after G - 1 links every lane's value is formed
+            // by the same float additions, in the same order, as the walk's own steps. The G tests then run at once and the walk keeps the state at
             // its first failing step. One Philox pass (block j of rank w in lane j (64 / G) + w)
             // serves four chunks; the iteration ends after the chunk in which a walk ends.
             // A group's first lane: for G = 16 the row shift reads 0 there, and 0 + y == y but
@@ -626,7 +633,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
               // the walk of this group (groups past the last walk repeat walk 0 and never end)
               const bool gwalk = grp < K;
               const int wl = __shfl(rank_lane, gwalk ? grp : 0);
-              float Y = __shfl(x[0], wl), T = __shfl(t, wl);
+              float Y = __shfl(x[0], wl);
               const float F = __shfl(fx, wl);
               const float gvdt = __shfl(vdt[0], wl);
               float gv = 0.f, gleak = 0.f;
@@ -651,15 +658,14 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                 const float za = (gi & 1) ? z1 : z0, zb = (gi & 1) ? z3 : z2;
                 zc[c] = (gi & 2) ? zb : za;
               }
-              float yf = Y, tf = T, ff = F;
+              float yf = Y, ff = F;
               bool gexit = false;
 #pragma unroll
               for (int c = 0; c < 4; ++c) {
                 iters += c ? G / 4 : G / 4 - 1;  // blocks of four steps; the loop counted one
                 const int q = c * G + gi;  // this lane's step within the pass
                 const float z = zc[c];     // the normal of step q
-                const float et = first ? T : dt;
-                float V, W = et;
+                float V;
                 if constexpr (MODEL == SSM_MODEL_ORNSTEIN) {
                   // the increment depends on the position: one shift and one step per link
                   float zs = row_shr1(z);
@@ -671,11 +677,6 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                     dpp_fence(vs);
                     const float vn = vs + walk_increment<MODEL, REPLAY>(vs, gvdt, gv, gleak, dt, 0.f, zs);
                     V = first ? Y : vn;
-                    W = row_shr1(W) + et;
-                    if constexpr (G == 8) {
-                      dpp_fence(W);
-                      W = first ? T : W;
-                    }
                   }
                 } else {
                   const float e = row_shr1_or(Y, walk_increment<MODEL, REPLAY>(0.f, gvdt, gv, gleak, dt, 0.f, z));
@@ -684,19 +685,15 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
 #pragma unroll
                   for (int s = 1; s < G; ++s) {
                     V = row_shr1(V) + e;
-                    W = row_shr1(W) + et;
                     if constexpr (G == 8) {
                       dpp_fence(V);
-                      dpp_fence(W);
                       V = first ? Y : V;
-                      W = first ? T : W;
                     }
                   }
                 }
                 const bool okq = !gwalk || inside<MODEL>(V, boundary_at_f<BOUND>(F + (float)q, gix + q, dt, ga, gb1, gb2, grow));
                 // the state after this lane's step; lane G - 1's starts the next chunk
                 const float Vn = V + walk_increment<MODEL, REPLAY>(V, gvdt, gv, gleak, dt, 0.f, z);
-                const float Wn = W + dt;
                 const uint64_t fail = __ballot(!okq);
                 if (fail != 0) {
                   const uint32_t fg = (uint32_t)(fail >> (grp * G)) & ((1u << G) - 1u);
@@ -704,32 +701,25 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   const int j = gexit ? __builtin_ctz(fg) : G - 1;
                   const int sl = grp * G + j;
                   yf = __shfl(gexit ? V : Vn, sl);
-                  tf = __shfl(gexit ? W : Wn, sl);
                   ff = F + (float)(c * G + (gexit ? j : G));
                   break;
                 }
                 if constexpr (G == 16) {
                   Y = row_bcast15(Vn);
-                  T = row_bcast15(Wn);
                 } else {
-                  float y7 = row_bcast7(Vn), y15 = row_bcast15(Vn), t7 = row_bcast7(Wn), t15 = row_bcast15(Wn);
+                  float y7 = row_bcast7(Vn), y15 = row_bcast15(Vn);
                   dpp_fence(y7);
                   dpp_fence(y15);
-                  dpp_fence(t7);
-                  dpp_fence(t15);
                   Y = (lane & 8) ? y15 : y7;
-                  T = (lane & 8) ? t15 : t7;
                 }
                 yf = Y;
-                tf = T;
                 ff = F + (float)((c + 1) * G);
               }
               const int sg = alive ? r * G : 0;
-              const float ny = __shfl(yf, sg), nt = __shfl(tf, sg), nf = __shfl(ff, sg);
+              const float ny = __shfl(yf, sg), nf = __shfl(ff, sg);
               const bool nex = __shfl((int)gexit, sg) != 0;
-              if (alive) {
+              if (alive) {  // t follows from the step count after the iteration
                 x[0] = ny;
-                t = nt;
                 fx = nf;
                 alive = !nex;
               }
@@ -770,34 +760,30 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                 }
                 if (alive) {
                   // A wave with few walks left waits on each walk's step latency rather than on
-                  // issue slots. The block's four positions and times are the same float sums
-                  // whether or not a test fails, so they are formed first; the four tests then
-                  // run independently and the walk keeps the state at its first failing test
-                  // (fx + i is exact). Same steps, same rounding, a chain of four adds and four
-                  // selects per block instead of four test-and-branch rounds.
+                  // issue slots. The block's four positions are the same float sums whether or
+                  // not a test fails, so they are formed first; the four tests then run
+                  // independently and the walk keeps the state at its first failing test
+                  // (fx + i is exact; t comes from the table after the iteration). Same steps,
+                  // same rounding, a chain of four adds and four selects per block instead of
+                  // four test-and-branch rounds.
                   const int ixb = (int)fx;  // this block's first step index
-                  float yv[5], tv[5];
+                  float yv[5];
                   yv[0] = x[0];
-                  tv[0] = t;
 #pragma unroll
-                  for (int i = 0; i < 4; ++i) {
+                  for (int i = 0; i < 4; ++i)
                     yv[i + 1] = yv[i] + walk_increment<MODEL, REPLAY>(yv[i], vdt[0], vraw[0], leak, dt, sst[0], cur[i]);
-                    tv[i + 1] = tv[i] + dt;
-                  }
                   bool okb[4];
 #pragma unroll
                   for (int i = 0; i < 4; ++i)
                     okb[i] = inside<MODEL>(yv[i], boundary_at_f<BOUND>(i ? fx + (float)i : fx, ixb + i, dt, a, b1, b2, row));
-                  float yf = yv[4], tf = tv[4], ff = fx + 4.0f;
+                  float yf = yv[4], ff = fx + 4.0f;
 #pragma unroll
                   for (int i = 3; i >= 0; --i) {
                     yf = okb[i] ? yf : yv[i];
-                    tf = okb[i] ? tf : tv[i];
                     ff = okb[i] ? ff : (i ? fx + (float)i : fx);
                   }
                   alive = okb[0] && okb[1] && okb[2] && okb[3];
                   x[0] = yf;
-                  t = tf;
                   fx = ff;
                 }
 #pragma unroll
@@ -844,7 +830,6 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
                   // block costs three VALU instructions more than the exec-mask branch
                   asm volatile("");
                   x[0] += walk_increment<MODEL, REPLAY>(x[0], vdt[0], vraw[0], leak, dt, sst[0], nz[i]);
-                  t += dt;
                   fx += 1.0f;
                 } else {
                   alive = false;
@@ -854,7 +839,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
           }
         }
         }
-        if (state == kRunning) ix = (int)fx;
+        // t after ix steps, from the table of the same sequential float32 sums
+        if (state == kRunning) {
+          ix = (int)fx;
+          t = A.t_grid[ix];
+        }
       } else {
 #pragma unroll
       for (int sb = 0; sb < kSubBlocksR<REPLAY, RECORD>; ++sb) {
@@ -1197,6 +1186,51 @@ static void fill_t_fast(const ssm_em_problem* P, EmArgs& A) {
   }
 }
 
+// The walks' times after k steps, k = 0..n_steps, one table per (device, dt, n_steps), built
+// on the host (the same sequential float32 additions as the walk) and kept on the device for
+// the life of the process: 4 (n_steps + 1) bytes each. The first launch of a (dt, n_steps)
+// on a device allocates and uploads it, so it may not run inside a stream capture.
+int32_t em_time_grid(float dt, int32_t n_steps, hipStream_t stream, const float** out) {
+  *out = nullptr;
+  if (n_steps > (1 << 24)) return SSM_OK;  // no untested iterations: the table is never read
+  int dev = 0;
+  SSM_HIP(hipGetDevice(&dev));
+  uint32_t dt_bits;
+  std::memcpy(&dt_bits, &dt, 4);
+  static std::mutex mu;
+  static std::map<std::tuple<int, uint32_t, int32_t>, float*> grids;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_tuple(dev, dt_bits, n_steps);
+  const auto it = grids.find(key);
+  if (it != grids.end()) {
+    *out = it->second;
+    return SSM_OK;
+  }
+  if (stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    SSM_HIP(hipStreamIsCapturing(stream, &st));
+    if (st != hipStreamCaptureStatusNone)
+      return fail(SSM_EINVAL, "the first launch of this (delta_t, n_steps) on this device is inside a stream "
+                              "capture: launch it once outside the capture (it builds the walks' time table)");
+  }
+  std::vector<float> h((size_t)n_steps + 1);
+  float t = 0.f;
+  for (int32_t k = 0; k <= n_steps; ++k) {
+    h[k] = t;
+    t = t + dt;
+  }
+  float* d = nullptr;
+  SSM_HIP(hipMalloc(&d, h.size() * sizeof(float)));
+  const hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy (time table)");
+  }
+  grids.emplace(key, d);
+  *out = d;
+  return SSM_OK;
+}
+
 int32_t em_validate(const ssm_em_problem* P, const float* set_params, const int64_t* offsets,
                     const float* table, const float* rn, const float* ru, float* rts, void* choices,
                     uint64_t* ws) {
@@ -1255,11 +1289,17 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
     return fail(SSM_EINVAL, "queue_block must be in [0, 65536]");
   if (L && (L->set_order != nullptr) != (L->queue_offsets != nullptr))
     return fail(SSM_EINVAL, "set_order and queue_offsets go together");
+  const float* t_grid = nullptr;
+  if (P->n_total > 0) {
+    const int32_t rc = em_time_grid(P->dt, P->n_steps, stream, &t_grid);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(ws_reset_kernel, dim3(1), dim3(64), 0, stream, (unsigned long long*)ws);
   SSM_HIP(hipGetLastError());
   if (P->n_total == 0) return SSM_OK;
 
   EmArgs A;
+  A.t_grid = t_grid;
   A.set_params = set_params;
   A.set_stride = P->set_stride ? P->set_stride : SSM_SET_STRIDE;
   A.offsets = offsets;
@@ -1321,6 +1361,7 @@ int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, 
   hipLaunchKernelGGL(ws_reset_kernel, dim3(1), dim3(64), 0, stream, (unsigned long long*)ws);
   SSM_HIP(hipGetLastError());
   EmArgs A;
+  A.t_grid = nullptr;  // one walk, every step tested
   A.set_params = set_params;
   A.set_stride = P->set_stride ? P->set_stride : SSM_SET_STRIDE;
   A.offsets = offsets;

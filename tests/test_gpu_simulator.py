@@ -125,6 +125,33 @@ def test_graph_replay_matches_eager_launch(torch_gpu):
     sim.close()
 
 
+def test_graph_of_a_first_time_step_grid_matches_eager_launch(torch_gpu):
+    """The walks read their times from a table the library builds once per (device, dt, n_steps).
+    A graph captured for a (dt, max_t) no launch of this process has used builds it before the
+    capture; its replays equal an eager launch of the same problem afterwards, bit for bit."""
+    import torch
+
+    from ssmhip import batched
+
+    theta, ns = batched.sample_angle_ragged(48, seed=21, lo=50, hi=800)
+    dt, max_t = 0.000731, 3.37  # a step grid used nowhere else in the tests
+    graphed = batched.BatchedSimulation("angle", theta, ns, delta_t=dt, max_t=max_t, seed=5, launch=None)
+    print("[case] graph captured before any eager launch of its step grid", flush=True)
+    graphed.replay()
+    torch.cuda.synchronize()
+    got = (graphed.rts.cpu().numpy().copy(), graphed.choices.cpu().numpy().copy())
+    graphed.replay()
+    torch.cuda.synchronize()
+    assert np.array_equal(graphed.rts.cpu().numpy(), got[0])
+    graphed.close()
+    eager = batched.BatchedSimulation("angle", theta, ns, delta_t=dt, max_t=max_t, seed=5, launch=None)
+    eager.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(eager.rts.cpu().numpy(), got[0])
+    assert np.array_equal(eager.choices.cpu().numpy(), got[1])
+    assert np.all(got[0][got[0] > 0] <= max_t + 2.0)  # rt = t + non-decision time (< 2 s)
+
+
 def test_config2_shaped_autotuned_graph_replay_matches_oracle(torch_gpu):
     """1024 ragged angle sets drawn from the batched angle workload's parameter bounds, in one
     launch whose shape the autotuner picks on the batch itself, captured in a HIP graph and
