@@ -22,11 +22,12 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
   // the simulator's step as its iterations far from the end of t_s and from the deadline run
   // it, in the same form (float32 rounding of every operation, no contraction; exec-mask
   // branches): the bound test at b = a + (fx dt) slope with the step counted in float32,
-  // then the position, time and counter updates. The bounds are wide enough that no walk
-  // ends, so every lane steps every time (the simulator's walks end and are refilled).
+  // then the position and counter updates (such iterations take t from the step count once
+  // they end, not per step). The bounds are wide enough that no walk ends, so every lane
+  // steps every time (the simulator's walks end and are refilled).
 #pragma clang fp contract(off)
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float y = 0.1f * (float)(gid & 7), t = 0.f, fx = 0.f;
+  float y = 0.1f * (float)(gid & 7), fx = 0.f;
   const float dt = 0.001f, vdt = 0.0005f, sst = 0.0316f, a = 3.0e4f, slope = -0.2f;
   // native walks draw normals scaled by sqrt_st (Box–Muller radius sqrt(-2 ln u1 sqrt_st^2))
   const float bmc = kBoxMullerLog2 * (sst * sst);
@@ -42,7 +43,6 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
           if (fabsf(y) <= b) {
             asm volatile("");  // a branch, as in the simulator (not speculated into selects)
             y += vdt + nz[i];
-            t += dt;
             fx += 1.0f;
           } else {
             alive = false;
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) probe_em_block(int64_t iters, float* out)
       }
     }
   }
-  out[gid] = y + t + fx;
+  out[gid] = y + fx;
 }
 
 template <bool LCA>
