@@ -73,3 +73,30 @@ def ptr(x) -> int | None:
     if not x.is_contiguous():
         raise ValueError("expected a contiguous device tensor")
     return int(x.data_ptr())
+
+
+# outputs from 1 MiB up to 64 MiB are copied to the host through pinned buffers (torch's
+# caching host allocator): smaller ones gain less than the pinned buffer costs, and larger ones
+# go through an ordinary copy, so that results a caller keeps hold little page-locked memory
+PINNED_COPY_MIN_BYTES = 1 << 20
+PINNED_COPY_MAX_BYTES = 64 << 20
+
+
+def to_host(*tensors):
+    """Device tensors -> NumPy arrays, copied on the current stream and waited for once.
+    Outputs of 1-64 MiB land in pinned host buffers (a direct DMA, no staging copy); each
+    returned array owns its buffer."""
+    t = require_gpu()
+    out = []
+    pinned = []
+    for x in tensors:
+        if PINNED_COPY_MIN_BYTES <= x.numel() * x.element_size() <= PINNED_COPY_MAX_BYTES:
+            h = t.empty(x.shape, dtype=x.dtype, pin_memory=True)
+            h.copy_(x, non_blocking=True)
+            pinned.append(len(out))
+            out.append(h)
+        else:
+            out.append(None)
+    if pinned:
+        t.cuda.current_stream().synchronize()
+    return [out[i].numpy() if out[i] is not None else x.cpu().numpy() for i, x in enumerate(tensors)]

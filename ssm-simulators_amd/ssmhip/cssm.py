@@ -109,8 +109,9 @@ def _run(model, code, n_particles, rec, n_samples, n_trials, max_t, delta_t, see
     else:
         job.launch()
     total = n_samples * n_trials
-    rts = job.rts[:total].cpu().numpy().reshape(n_samples, n_trials, 1)
-    choices = job.choices[:total].cpu().numpy().astype(np.intc, copy=False).reshape(n_samples, n_trials, 1)
+    rts, choices = D.to_host(job.rts[:total], job.choices[:total])
+    rts = rts.reshape(n_samples, n_trials, 1)
+    choices = choices.astype(np.intc, copy=False).reshape(n_samples, n_trials, 1)
     traj = None
     if trajectory:
         if traj_dev is not None:

@@ -88,3 +88,20 @@ def test_simulator_outputs_unchanged_by_device_records(torch_gpu):
     got_c = out["choices"][:, :, 0].T.reshape(-1)
     assert np.array_equal(got_c, c0)
     assert np.array_equal(got_r, r0)
+
+
+def test_host_copies_equal_plain_copies_on_each_side_of_the_pinned_range(torch_gpu):
+    """Simulator outputs reach the host through pinned buffers from 1 MiB to 64 MiB and through
+    ordinary copies outside that range; every path gives the device values."""
+    import torch
+
+    from ssmhip import _device as D
+
+    for n in (1000, 300_000, 17_000_000):  # 4 KB, 1.2 MB, 68 MB of float32
+        print(f"[case] host copy of {n} floats", flush=True)
+        x = torch.arange(n, dtype=torch.float32, device="cuda") * 0.5
+        y = (torch.arange(n, dtype=torch.int32, device="cuda") % 7) - 3
+        a, b = D.to_host(x, y)
+        assert a.dtype == np.float32 and b.dtype == np.int32
+        assert np.array_equal(a, x.cpu().numpy()) and np.array_equal(b, y.cpu().numpy())
+        del x, y
