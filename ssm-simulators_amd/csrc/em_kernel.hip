@@ -68,6 +68,9 @@ struct EmArgs {
   // untested iterations read a walk's t from it instead of adding dt per step (NULL when
   // n_steps > 2^24, where no iteration is untested)
   const float* t_grid;
+  // most walks a wave runs at once (64: every lane); fewer lets a small job's walks take the
+  // short-latency tail modes from their first step
+  int32_t max_walks;
   uint32_t k0, k1;
 };
 
@@ -251,6 +254,8 @@ constexpr int kHelperWalks = 32;
 // steps.
 constexpr int kLoneWalks = 8;
 constexpr int kLoneSteps = 64;
+// walks per wave of a small job (at most 16 walks per SIMD): within reach of the chains
+constexpr int kSmallJobWalks = 4;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -324,6 +329,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
     const int n_idle = __popcll(idle);
     bool maint;
     if (exhausted) maint = __ballot(state == kDone) != 0;
+    else if (A.max_walks < 64) maint = 64 - n_idle < A.max_walks;  // room for another walk
     else maint = n_idle >= A.refill_min;
     if (maint) {
       if (state == kDone) {
@@ -389,7 +395,11 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         state = kEmpty;
       }
       if (!exhausted) {
-        const uint64_t want = __ballot(state == kEmpty);
+        uint64_t want = __ballot(state == kEmpty);
+        if (A.max_walks < 64) {  // the first empty lanes up to the cap take work; the rest stay empty
+          const int room = A.max_walks - __popcll(__ballot(state == kRunning));
+          want = __ballot(state == kEmpty && lanes_below(want) < room);
+        }
         const int cnt = __popcll(want);
         // the first r ranked lanes take the wave's local range, the rest a fresh block
         const int64_t lo_old = q_lo;
@@ -425,7 +435,7 @@ __global__ void __launch_bounds__(1024) em_kernel(EmArgs A) {
         if (q_lo < q_hi)
           while (q_k < A.n_sets - 1 && A.qoff[q_k + 1] <= q_lo) ++q_k;  // keep q_k = queue position of q_lo
         if (q_lo >= q_hi && queue_done) exhausted = true;
-        if (state == kEmpty) {
+        if ((want >> lane_id()) & 1ull) {
           const int j = lanes_below(want);
           int64_t gi = A.total;
           int kk = 0;
@@ -1330,8 +1340,16 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
 
   const int cus = device_cu_count();
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
+  // A job of at most 16 walks per SIMD is bound by its longest walk's latency, not by issue:
+  // the native Gaussian diffusions then run 4 walks per wave, so every walk steps in the
+  // lane-shift chains (~55 cycles per step instead of ~170). Larger jobs keep full waves, whose
+  // steps cost 4-5x fewer instructions (measured, one set: 1e3-1e4 walks 0.40 -> 0.20-0.26 ms;
+  // 3e4 walks slower). The other models and replay have no chains and keep full waves.
+  const bool chains = P->replay_stride == 0 && (P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM ||
+                                                P->model == SSM_MODEL_ORNSTEIN || P->model == SSM_MODEL_DDM_LEGACY);
+  A.max_walks = chains && P->n_total <= (int64_t)cus * 4 * 16 ? kSmallJobWalks : 64;
   int64_t grid = (int64_t)cus * per_cu;
-  const int64_t needed = (P->n_total + block - 1) / block;
+  const int64_t needed = (P->n_total * (64 / A.max_walks) + block - 1) / block;
   if (grid > needed) grid = needed;
   if (grid < 1) grid = 1;
   KernelFn fn = pick_kernel(P->model, P->boundary, P->n_particles, P->replay_stride > 0);
@@ -1384,6 +1402,7 @@ int32_t em_trajectory_enqueue(const ssm_em_problem* P, const float* set_params, 
   A.out_layout = 0;
   A.choice_bytes = P->choice_bytes;
   A.refill_min = 1;
+  A.max_walks = 64;
   A.queue_block = 1;
   A.dt = P->dt;
   fill_t_fast(P, A);

@@ -108,6 +108,44 @@ def test_ddm_replay_matches_oracle_trajectory_by_trajectory(torch_gpu, bname, us
     assert steps == int(s0.sum())
 
 
+@pytest.mark.parametrize("dt,max_t", [(0.000731, 3.0), (0.0025, 10.0), (0.01, 25.0)])
+def test_ddm_replay_times_at_other_step_sizes(torch_gpu, dt, max_t):
+    """Walks far from their deadlines take their time from the library's table of t after k
+    steps (k float32 additions of dt from 0). At step sizes other than 1e-3, with walks long
+    enough that t crosses several powers of two, every rt must still equal the oracle's
+    sequentially accumulated one."""
+    import torch
+
+    L, em = _ssm()
+    from ssmhip import boundary_functions as BF
+
+    K, N, L_ = 12, 200, 1031
+    print(f"[case] ddm replay at dt {dt}, max_t {max_t}", flush=True)
+    rng = np.random.default_rng(61)
+    v = rng.uniform(-0.4, 0.4, K).astype(F32)
+    a = rng.uniform(2.0, 3.0, K).astype(F32)
+    z = rng.uniform(0.3, 0.7, K).astype(F32)
+    t = rng.uniform(0.0, 0.5, K).astype(F32)
+    dt32 = float(F32(dt))
+    normals = rng.standard_normal((K * N, L_)).astype(F32)
+    unif = rng.uniform(size=K * N).astype(F32)
+    bp = {"a": a, "theta": rng.uniform(-0.1, 0.1, K).astype(F32)}
+    r0, c0, s0 = E.ddm_flexbound(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), n_samples=N, max_t=max_t,
+                                 delta_t=dt32, boundary="angle", bparams=bp, smooth_unif=True, normals=normals,
+                                 uniforms=unif)
+    rec = em.ddm_records(v, a, z, t, np.full(K, 999, F32), np.ones(K, F32), max_t, dt32, L.BOUNDARY_ANGLE, bp)
+    job = em.make_job(L.MODEL_DDM, L.BOUNDARY_ANGLE, 1, rec, em.uniform_offsets(K, N), delta_t=dt32, max_t=max_t,
+                      seed=1, smooth_unif=True, replay_normals=normals, replay_uniforms=unif)
+    job.launch()
+    torch.cuda.synchronize()
+    r1, c1 = job.rts[: K * N].cpu().numpy(), job.choices[: K * N].cpu().numpy()
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(r0, r1)
+    assert job.steps() == int(s0.sum())
+    # walks long enough for t to pass 1, 2 and 4 (t's float32 spacing changes at each)
+    assert np.max(s0) * dt32 > 4.0 or np.max(s0) >= BF.n_steps(max_t, dt32) - 1
+
+
 @pytest.mark.parametrize("bname", ["constant", "angle"])
 def test_ddm_replay_with_binding_deadlines_and_a_short_time_grid(torch_gpu, bname):
     """Walks that end by their deadline (t > deadline - t0) or by reaching the end of t_s,
@@ -303,6 +341,36 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
     for r, c, st in outs[1:]:
         assert np.array_equal(r, outs[0][0]) and np.array_equal(c, outs[0][1]) and st == outs[0][2]
     assert np.all(outs[0][0] > 0)
+
+
+@pytest.mark.parametrize("kind", ["ddm", "ornstein"])
+def test_small_job_in_four_walk_waves_equals_the_same_walks_in_full_waves(torch_gpu, kind):
+    """A job of at most 16 walks per SIMD runs its native Gaussian walks 4 to a wave (each
+    stepping in lane-shift chains from its first step); a larger job fills every lane. The same
+    10,000-walk set alone and as the first set of a 40,000-walk job must give identical rts and
+    choices."""
+    import torch
+
+    L, em = _ssm()
+    cus = int(L.load().ssm_device_cu_count())
+    n0, n1 = 10_000, 30_000
+    assert n0 <= 64 * cus < n0 + n1  # alone: 4-walk waves; together: full waves
+    v, a, z, t = (np.array(x, F32) for x in ([0.8, -0.5], [1.2, 1.6], [0.45, 0.55], [0.2, 0.3]))
+    bp = {"a": a, "theta": np.array([0.25, 0.1], F32)}
+    model = L.MODEL_DDM if kind == "ddm" else L.MODEL_ORNSTEIN
+    extra = {} if kind == "ddm" else {"g": np.array([0.3, -0.2], F32)}
+    rec = em.ddm_records(v, a, z, t, np.full(2, 999, F32), np.ones(2, F32), 20.0, DT, L.BOUNDARY_ANGLE, bp, **extra)
+    print(f"[case] small job in 4-walk waves, {kind}", flush=True)
+    alone = em.make_job(model, L.BOUNDARY_ANGLE, 1, rec[:1], np.array([0, n0], np.int64), delta_t=DT, max_t=20.0,
+                        seed=3, smooth_unif=True)
+    alone.launch()
+    both = em.make_job(model, L.BOUNDARY_ANGLE, 1, rec, np.array([0, n0, n0 + n1], np.int64), delta_t=DT,
+                       max_t=20.0, seed=3, smooth_unif=True)
+    both.launch()
+    torch.cuda.synchronize()
+    assert np.array_equal(alone.rts[:n0].cpu().numpy(), both.rts[:n0].cpu().numpy())
+    assert np.array_equal(alone.choices[:n0].cpu().numpy(), both.choices[:n0].cpu().numpy())
+    assert alone.steps() > 100 * n0  # walks of hundreds of steps
 
 
 @pytest.mark.parametrize("model", ["ddm", "ddm_deadline", "ddm_table", "ddm_legacy", "ornstein", "full_ddm"])
