@@ -26,6 +26,8 @@ DEFAULT_CANDIDATES = [
     (256, 8, 8, 64), (256, 8, 4, 64), (256, 8, 2, 64), (256, 8, 4, 32), (256, 8, 8, 32), (256, 8, 8, 128),
     (256, 8, 16, 64), (512, 4, 8, 64), (512, 4, 4, 64), (128, 16, 8, 64), (1024, 2, 8, 64), (256, 8, 4, 128),
     (256, 8, 32, 64), (512, 4, 16, 64), (1024, 2, 16, 64), (512, 4, 32, 64),
+    # resident waves at the kernel's 7-per-SIMD occupancy, and longer queue claims
+    (256, 7, 4, 128), (512, 3, 4, 128), (512, 3, 4, 256), (256, 8, 4, 256), (512, 4, 4, 128),
 ]
 
 _memo: dict[str, LaunchParams] = {}
