@@ -111,8 +111,8 @@ typedef struct ssm_launch_config {
   int32_t blocks_per_cu; /* resident blocks per CU in the persistent grid     */
   int32_t refill_min;    /* idle lanes (1..64) before a wavefront refills; a
                             native Gaussian-diffusion job of at most 64 x CUs
-                            trajectories runs 4 walks per wavefront instead
-                            and refills as soon as one ends                  */
+                            (160 x CUs) trajectories runs 4 (16) walks per
+                            wavefront instead and refills as soon as one ends */
   int32_t queue_block;   /* largest index range a wavefront claims from the
                             global queue per atomic (0 = 64, at most 65536)  */
   /* Optional visiting order of the sets (both NULL: index order). set_order

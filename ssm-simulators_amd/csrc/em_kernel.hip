@@ -254,8 +254,10 @@ constexpr int kHelperWalks = 32;
 // steps.
 constexpr int kLoneWalks = 8;
 constexpr int kLoneSteps = 64;
-// walks per wave of a small job (at most 16 walks per SIMD): within reach of the chains
+// walks per wave of a small job (at most 16 walks per SIMD): within reach of the chains;
+// of a job of at most 40 walks per SIMD: within reach of the shared Philox blocks
 constexpr int kSmallJobWalks = 4;
+constexpr int kMidJobWalks = 16;
 
 // Choice rule at a two-stage walk's exit (parallel_models.pyx:474-479, :547-552): a fair coin
 // when the bound has collapsed (b <= 0), else u <= (y + b) / (2 b) in float32.
@@ -1342,12 +1344,15 @@ int32_t em_enqueue(const ssm_em_problem* P, const ssm_launch_config* L, const fl
   if (cus <= 0) return fail(SSM_EHIP, "no HIP device");
   // A job of at most 16 walks per SIMD is bound by its longest walk's latency, not by issue:
   // the native Gaussian diffusions then run 4 walks per wave, so every walk steps in the
-  // lane-shift chains (~55 cycles per step instead of ~170). Larger jobs keep full waves, whose
-  // steps cost 4-5x fewer instructions (measured, one set: 1e3-1e4 walks 0.40 -> 0.20-0.26 ms;
-  // 3e4 walks slower). The other models and replay have no chains and keep full waves.
+  // lane-shift chains (~55 cycles per step instead of ~170); up to 40 walks per SIMD they run
+  // 16 per wave (the shared Philox blocks, ~150 cycles). Larger jobs keep full waves, whose
+  // steps cost fewer instructions (measured, one set: 1e3-1e4 walks 0.40 -> 0.20-0.26 ms,
+  // 2e4-3e4 walks 0.40 -> 0.33-0.36 ms; 6e4 walks and more slower capped). The other models
+  // and replay have no tail modes and keep full waves.
   const bool chains = P->replay_stride == 0 && (P->model == SSM_MODEL_DDM || P->model == SSM_MODEL_FULL_DDM ||
                                                 P->model == SSM_MODEL_ORNSTEIN || P->model == SSM_MODEL_DDM_LEGACY);
-  A.max_walks = chains && P->n_total <= (int64_t)cus * 4 * 16 ? kSmallJobWalks : 64;
+  const int64_t per_simd = (P->n_total + (int64_t)cus * 4 - 1) / ((int64_t)cus * 4);
+  A.max_walks = !chains ? 64 : per_simd <= 16 ? kSmallJobWalks : per_simd <= 40 ? kMidJobWalks : 64;
   int64_t grid = (int64_t)cus * per_cu;
   const int64_t needed = (P->n_total * (64 / A.max_walks) + block - 1) / block;
   if (grid > needed) grid = needed;

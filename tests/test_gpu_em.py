@@ -346,15 +346,15 @@ def test_native_output_is_independent_of_launch_shape_on_ragged_sets(torch_gpu, 
 @pytest.mark.parametrize("kind", ["ddm", "ornstein"])
 def test_small_job_in_four_walk_waves_equals_the_same_walks_in_full_waves(torch_gpu, kind):
     """A job of at most 16 walks per SIMD runs its native Gaussian walks 4 to a wave (each
-    stepping in lane-shift chains from its first step); a larger job fills every lane. The same
-    10,000-walk set alone and as the first set of a 40,000-walk job must give identical rts and
-    choices."""
+    stepping in lane-shift chains from its first step), up to 40 walks per SIMD 16 to a wave;
+    a larger job fills every lane. The same 10,000-walk set alone, as the first set of a
+    25,000-walk job and of a 60,000-walk job must give identical rts and choices."""
     import torch
 
     L, em = _ssm()
     cus = int(L.load().ssm_device_cu_count())
-    n0, n1 = 10_000, 30_000
-    assert n0 <= 64 * cus < n0 + n1  # alone: 4-walk waves; together: full waves
+    n0 = 10_000
+    assert n0 <= 64 * cus < 25_000 <= 160 * cus < 60_000  # 4-walk, 16-walk and full waves
     v, a, z, t = (np.array(x, F32) for x in ([0.8, -0.5], [1.2, 1.6], [0.45, 0.55], [0.2, 0.3]))
     bp = {"a": a, "theta": np.array([0.25, 0.1], F32)}
     model = L.MODEL_DDM if kind == "ddm" else L.MODEL_ORNSTEIN
@@ -364,12 +364,14 @@ def test_small_job_in_four_walk_waves_equals_the_same_walks_in_full_waves(torch_
     alone = em.make_job(model, L.BOUNDARY_ANGLE, 1, rec[:1], np.array([0, n0], np.int64), delta_t=DT, max_t=20.0,
                         seed=3, smooth_unif=True)
     alone.launch()
-    both = em.make_job(model, L.BOUNDARY_ANGLE, 1, rec, np.array([0, n0, n0 + n1], np.int64), delta_t=DT,
-                       max_t=20.0, seed=3, smooth_unif=True)
-    both.launch()
     torch.cuda.synchronize()
-    assert np.array_equal(alone.rts[:n0].cpu().numpy(), both.rts[:n0].cpu().numpy())
-    assert np.array_equal(alone.choices[:n0].cpu().numpy(), both.choices[:n0].cpu().numpy())
+    for total in (25_000, 60_000):
+        both = em.make_job(model, L.BOUNDARY_ANGLE, 1, rec, np.array([0, n0, total], np.int64), delta_t=DT,
+                           max_t=20.0, seed=3, smooth_unif=True)
+        both.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(alone.rts[:n0].cpu().numpy(), both.rts[:n0].cpu().numpy())
+        assert np.array_equal(alone.choices[:n0].cpu().numpy(), both.choices[:n0].cpu().numpy())
     assert alone.steps() > 100 * n0  # walks of hundreds of steps
 
 
