@@ -53,7 +53,8 @@ def angle_slope32(theta: np.ndarray) -> np.ndarray:
         if starts.size < th.size // 8:
             starts = np.concatenate([[0], starts])
             runs = np.diff(np.append(starts, th.size))
-            return np.repeat(_slope(th[starts]), runs)
+            # runs of one length (a parameter broadcast over a trial axis) repeat faster by a count
+            return np.repeat(_slope(th[starts]), runs[0] if np.all(runs == runs[0]) else runs)
     return _slope(th)
 
 

@@ -18,7 +18,7 @@ from typing import Any, Callable
 import numpy as np
 
 from .config import get as _get_config
-from .simulator import simulator
+from .simulator import OwnedTheta, simulator
 
 _logger = logging.getLogger(__name__)
 
@@ -126,6 +126,7 @@ def hssm_sim_wrapper(simulator_fun, theta, model, n_replicas, random_state, **kw
 
 def _build_decorated_simulator(model_name: str, choices: list, obs_dim: int = 2) -> Callable:
     wrapped = partial(hssm_sim_wrapper, simulator_fun=simulator, model=model_name)
+    wrapped.takes_owned_theta = True  # rng_fn may hand it its per-call theta without a copy
     return decorate_atomic_simulator(model_name=model_name, choices=choices, obs_dim=obs_dim)(wrapped)
 
 
@@ -169,6 +170,8 @@ def rng_fn(arg_arrays, size, rng, simulator_fun: Callable, obs_dim_int: int, *ar
     if kwargs.get("extra_fields") is not None and not is_all_args_scalar and max_shape is not None:
         kwargs = {**kwargs, "extra_fields": _broadcast_extra_fields(kwargs["extra_fields"], max_shape,
                                                                     new_data_size)}
+    if getattr(simulator_fun, "takes_owned_theta", False):
+        theta = theta.view(OwnedTheta)  # built above for this call only: no copy needed
     sims_out = simulator_fun(theta=theta, random_state=seed, n_replicas=n_replicas, **kwargs)
     if not is_all_args_scalar:
         sims_out = sims_out.reshape(_reshape_sims_out(max_shape, n_replicas, obs_dim_int))

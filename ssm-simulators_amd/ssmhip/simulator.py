@@ -64,6 +64,12 @@ def _make_valid_dict(d: dict) -> dict:
     return d
 
 
+class OwnedTheta(np.ndarray):
+    """A theta matrix handed over by its owner (hssm_support.rng_fn builds one per call):
+    Simulator.simulate uses a float32 one as it is instead of taking the reference's
+    defensive float32 copy, since no other code holds it."""
+
+
 def _preprocess_theta(theta):
     # simulator.py:221-254
     try:
@@ -73,6 +79,8 @@ def _preprocess_theta(theta):
     if isinstance(theta, list):
         return np.asarray(theta).astype(np.float32)
     if isinstance(theta, np.ndarray):
+        if type(theta) is OwnedTheta and theta.dtype == np.float32:
+            return theta.view(np.ndarray)
         return theta.astype(np.float32)
     if isinstance(theta, dict):
         return _make_valid_dict(deepcopy(theta))
