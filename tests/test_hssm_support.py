@@ -33,6 +33,25 @@ def test_prepare_theta_vector_parameters_broadcast_to_the_largest():
     assert np.all(theta[:, 2] == 0.5)
 
 
+def test_only_an_owned_float32_theta_is_used_without_a_copy():
+    """The reference copies theta to float32 before simulating (simulator.py:221-254); rng_fn's
+    own per-call matrix, marked OwnedTheta, is used as it is, and every other array is still
+    copied, so a caller's array never aliases the simulator's input."""
+    from ssmhip.simulator import OwnedTheta, _preprocess_theta
+
+    theta = np.asfortranarray(np.arange(12, dtype=np.float32).reshape(4, 3))
+    copied = _preprocess_theta(theta)
+    assert not np.shares_memory(copied, theta) and np.array_equal(copied, theta)
+    owned = _preprocess_theta(theta.view(OwnedTheta))
+    assert np.shares_memory(owned, theta) and type(owned) is np.ndarray and owned.dtype == np.float32
+    theta64 = theta.astype(np.float64).view(OwnedTheta)
+    out = _preprocess_theta(theta64)
+    assert out.dtype == np.float32 and not np.shares_memory(out, theta64)
+    H = _hs()
+    assert getattr(H.get_simulator_fun_internal("angle"), "takes_owned_theta", False)
+    assert not getattr(H.get_simulator_fun_internal(lambda **kw: kw), "takes_owned_theta", False)
+
+
 def test_replicas_sizes_and_output_shape():
     H = _hs()
     assert H._calculate_n_replicas(True, 10, None) == 1
